@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark: xcorr correlation patches/s on one MI355X.
+
+Workload: a 10800 (range) x 27648 (azimuth) complex-int16
+SLC pair generated on the GPU (speckle model, offset field dx = 2.1 + 1e-4 x,
+dy = -7.6 + 3e-4 y, coherence per 512 x 512 block uniform in [0.3, 0.95]),
+locations from get_locations with -nx 100 -ny 400 -xsearch 64 -ysearch 64
+(40,000 patches, 256 x 256 windows, range interpolation x2, 16x high-res).
+
+One step = the whole hot path over all 40,000 locations with the images and the
+window list already resident in HBM and the results left in HBM.
+
+Also reported:
+  roofline      dominant kernel: algorithmic bytes per launch (524,308 B per
+                patch: two 256 x 256 complex-int16 windows + 20 B of output)
+                / its mean launch time, measured with HIP events on the
+                engine's stream, against 8 TB/s
+  cpu_baseline  the FP32 CPU restatement (oracle) on a bounded sample of the
+                same workload, on the host cores of this machine
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+METRIC = "correlation patches/sec on 1 GPU (128², 16× interp); HBM roofline %; proposal vs plain"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+BYTES_PER_PATCH = 2 * 256 * 256 * 4 + 20          # 524,308
+FLOP_PER_PATCH = 33.39e6                          # 5 N log2 N over the path's FFTs
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=4096, help="locations per engine batch")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--small", action="store_true", help="config 1 (1024 x 2048, 1000 patches) instead")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the xcorr path has independent patches and this build "
+              "runs on a single MI355X (--gpus %d requested)" % args.gpus, flush=True)
+        return 2
+
+    import torch
+    import importlib
+    import __graft_entry__
+    xg = __graft_entry__.load_package()
+    synth = importlib.import_module("gmtsar_gpu_amd.synth")
+    if not torch.cuda.is_available():
+        print("no GPU available", file=sys.stderr)
+        return 1
+    dev = torch.device("cuda:0")
+
+    t0 = time.time()
+    if args.small:
+        n_range, n_az, nxl, nyl = 1024, 2048, 20, 50
+        mh, sh = synth.constant_shift_pair(n_range, n_az, 3.37, -12.81, 0.8)
+        m, s = torch.from_numpy(mh).to(dev), torch.from_numpy(sh).to(dev)
+        workload = "plain xcorr, 1024 x 2048 cint16 pair, -nx 20 -ny 50 -xsearch 64 -ysearch 64, interp 16"
+    else:
+        n_range, n_az, nxl, nyl = 10800, 27648, 100, 400
+        m, s = synth.strip_pair_torch(n_range, n_az, lambda x: 2.1 + 1e-4 * x, lambda y: -7.6 + 3e-4 * y,
+                                      device="cuda")
+        workload = "plain xcorr, 10800 x 27648 cint16 pair, -nx 100 -ny 400 -xsearch 64 -ysearch 64, interp 16"
+    torch.cuda.synchronize()
+    log("synthetic pair ready in %.1f s" % (time.time() - t0))
+
+    p = xg.make_params(64, 64)
+    x, y = xg.get_locations(n_range, n_az, nxl, nyl, 64, 64)
+    w = xg.make_windows(x, y, p.npx, p.npy)
+    n = len(w)
+    dw = torch.from_numpy(w.view(np.int32).reshape(-1, 4).copy()).to(dev)
+    dout = torch.zeros(n * xg.RESULT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    eng = xg.Engine(p, max_batch=min(args.batch, n))
+    eng.set_images(m, s)
+
+    for _ in range(args.warmup):
+        eng.correlate(dw, out=dout, flags=xg.NO_SYNC)
+    eng.sync()
+    torch.cuda.synchronize()
+    log("warmup done")
+
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.correlate(dw, out=dout, flags=xg.NO_SYNC)
+    eng.sync()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    step_s = (t2 - t1) / args.steps
+    value = n / step_s
+    log("timed %d steps: %.3f ms/step, %.0f patches/s" % (args.steps, step_s * 1e3, value))
+
+    # per-kernel HIP-event timing on the engine's stream (separate, untimed step)
+    eng.correlate(dw, out=dout, flags=xg.TIMING)
+    kt = eng.kernel_times()
+    dom = max(kt.items(), key=lambda kv: kv[1][0])
+    dom_name, (dom_ms, dom_launches) = dom
+    per_launch_ms = dom_ms / dom_launches
+    patches_per_launch = n / dom_launches
+    achieved = BYTES_PER_PATCH * patches_per_launch / (per_launch_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": dom_name, "launch_ms": round(per_launch_ms, 4),
+                "patches_per_launch": patches_per_launch}
+    kernels = {k: {"ms_total": round(v[0], 4), "launches": v[1]} for k, v in kt.items()}
+    path = {"hbm_GBs": round(BYTES_PER_PATCH * value / 1e9, 2),
+            "hbm_frac": round(BYTES_PER_PATCH * value / 1e9 / HBM_PEAK_GBS, 5),
+            "fp32_TFLOPs": round(FLOP_PER_PATCH * value / 1e12, 3),
+            "fp32_frac": round(FLOP_PER_PATCH * value / 1e12 / FP32_PEAK_TFLOPS, 5)}
+    log("kernels:", json.dumps(kernels))
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        import oracle
+        oracle.lib()
+        got = dout.cpu().numpy().view(xg.RESULT_DTYPE)
+        nthreads = min(16, os.cpu_count() or 1)
+        # host copy of the rows the sample needs
+        op = oracle.make_params(64, 64)
+        probe = min(n, max(nthreads, 16))
+        rows_lo = int(min(w["my0"][:probe].min(), w["sy0"][:probe].min()))
+        rows_hi = int(max(w["my0"][:probe].max(), w["sy0"][:probe].max())) + p.npy
+        lo = max(rows_lo, 0)
+        mh = m[lo:rows_hi].cpu().numpy()
+        shh = s[lo:rows_hi].cpu().numpy()
+        locs = np.stack([w["mx0"], w["my0"] - lo, w["sx0"], w["sy0"] - lo], 1).astype(np.int32)
+        tp = time.perf_counter()
+        oracle.run(op, mh, shh, locs[:probe], precision=32, nthreads=nthreads)
+        per = (time.perf_counter() - tp) / probe
+        sample = int(min(n, max(probe, args.cpu_seconds / per)))
+        rows_hi = int(max(w["my0"][:sample].max(), w["sy0"][:sample].max())) + p.npy
+        mh = m[lo:rows_hi].cpu().numpy()
+        shh = s[lo:rows_hi].cpu().numpy()
+        tp = time.perf_counter()
+        ref = oracle.run(op, mh, shh, locs[:sample], precision=32, nthreads=nthreads)
+        cpu_s = time.perf_counter() - tp
+        cpu = {"value": round(sample / cpu_s, 2), "unit": "patches/s", "cores": nthreads, "kind": "port",
+               "sample": "first %d of the %d locations (FP32 CPU restatement, OpenMP over patches, %.1f s)"
+                         % (sample, n, cpu_s)}
+        same = True
+        for f in ("xoff", "yoff", "corr", "xfrac", "yfrac", "ip", "jp"):
+            a, b = got[f][:sample], ref[f]
+            if a.dtype.kind == "f":
+                same &= bool(np.array_equal(a.view(np.uint32), b.view(np.uint32)))
+            else:
+                same &= bool(np.array_equal(a, b))
+        parity = {"sample": sample, "bit_exact_vs_cpu": same}
+        log("cpu baseline: %.1f patches/s on %d threads; parity %s" % (cpu["value"], nthreads, same))
+
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "patches/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": workload, "patches": n, "xsearch": 64, "ysearch": 64, "ri": 2,
+                      "interp_factor": 16, "batch": min(args.batch, n)},
+           "roofline": roofline, "cpu_baseline": cpu, "path_roofline": path, "kernels": kernels,
+           "parity": parity}
+    print(json.dumps(out), flush=True)
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,983 @@
+// MI355X (gfx950) xcorr engine: kernels for the four stages of GMTSAR xcorr's
+// offset-tracking path and the C ABI declared in include/xcorr_hip.h.
+//
+// Data flow per batch of B locations (all buffers device-resident):
+//   k_range_ampl     stage (a)  window gather, range interpolation, amplitude,
+//                               exact fixed-point patch sums
+//   k_rowfft         stage (b)  demean/mask, z = c1 + i c2, FFT along x
+//   k_colfft         stage (b)  FFT along y
+//   k_product_colifft stage (b) C3 = C1 conj(C2) (-1)^(kx+ky), IFFT along y,
+//                               keep the lag rows
+//   k_rowifft_surf   stage (b)  IFFT along x, |c3| on the lag window, argmax
+//   k_time_search    stage (c)  (-time/-time4) exact full search
+//   k_finish         stage (c)+(d) NCC at the peak, high-res refinement, result
+#include "../../include/xcorr_hip.h"
+#include "fft_spec.hpp"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+namespace xc {
+
+constexpr int kThreads = 256;
+constexpr int kTile = 2048;     // complex elements per LDS FFT buffer
+
+struct DevParams {
+  int NX, NY, N;
+  int xs, ys, nx_corr, ny_corr, nxc, nyc;
+  int ri, interp_flag, ifc, n2x, n2y, Lx, Ly;
+  int corr_flag, format;
+  int G1, G4, C3;                 // rows per block (k_range_ampl/k_rowfft, k_rowifft_surf), columns per block
+  int groups4;                    // k_rowifft_surf blocks per patch
+  int tw_nx, tw_ny, tw_lx, tw_ly, phi;   // offsets into the table buffer
+  int m_nx, m_ny, s_nx, s_ny;
+};
+
+// --------------------------------------------------------------------------
+// reductions
+// --------------------------------------------------------------------------
+__device__ __forceinline__ long long wave_sum_i64(long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long w = __shfl_xor(v, o);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ double wave_sum_f64(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+// Block-wide exact int64 sum; result valid in every thread.
+__device__ long long block_sum_i64(long long v, long long* red) {
+  v = wave_sum_i64(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  long long t = 0;
+  for (int w = 0; w < nw; w++) t += red[w];
+  return t;
+}
+__device__ unsigned long long block_max_u64(unsigned long long v, unsigned long long* red) {
+  v = wave_max_u64(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  unsigned long long t = 0;
+  for (int w = 0; w < nw; w++) t = red[w] > t ? red[w] : t;
+  return t;
+}
+__device__ double block_sum_f64(double v, double* red) {
+  v = wave_sum_f64(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < nw; w++) t += red[w];
+  return t;
+}
+
+// argmax key: first-index maximum of strictly positive values (the scan
+// "if (v > best)" from best = 0, index 0).  Non-positive and NaN give 0.
+__device__ __forceinline__ unsigned long long peak_key(float v, unsigned idx) {
+  return v > 0.0f ? ((unsigned long long)__float_as_uint(v) << 32) | (unsigned long long)(~idx) : 0ull;
+}
+__device__ __forceinline__ void key_decode(unsigned long long key, float* v, unsigned* idx) {
+  if (key == 0ull) { *v = 0.0f; *idx = 0u; return; }
+  *v = __uint_as_float((unsigned)(key >> 32));
+  *idx = ~(unsigned)(key & 0xffffffffull);
+}
+
+__device__ __forceinline__ double patch_mean(unsigned long long s, int N) {
+  return (double)(long long)s * (1.0 / 1048576.0) / (double)N;
+}
+
+// --------------------------------------------------------------------------
+// stage (a): gather + range interpolation + amplitude + fixed-point sums
+// xcorr.cu:237-292, fft_interpolate_routines.cu:76-113, conv_ampl_demean.cu:98-108, 703-726
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_range_ampl(DevParams P, const void* __restrict__ master,
+                                                         const void* __restrict__ slave,
+                                                         const xcorr_hip_window* __restrict__ win,
+                                                         float* __restrict__ amp,
+                                                         unsigned long long* __restrict__ sums,
+                                                         const float2* __restrict__ tabs) {
+  extern __shared__ float2 smem[];
+  __shared__ long long red[kThreads / 64];
+  const int NX = P.NX, G = P.G1, nel = G * NX;
+  const int groups = P.NY / G;
+  const int g = blockIdx.x % groups;
+  const int rest = blockIdx.x / groups;
+  const int img = rest & 1;
+  const int b = rest >> 1;
+  const xcorr_hip_window w = win[b];
+  const int x0 = img ? w.sx0 : w.mx0;
+  const int y0 = (img ? w.sy0 : w.my0) + g * G;
+  const int inx = img ? P.s_nx : P.m_nx, iny = img ? P.s_ny : P.m_ny;
+  const void* src = img ? slave : master;
+  const int ri = P.ri;
+  float2* A = smem;
+  float2* B = A + nel;
+  float2* Cb = B + nel;                       // only with ri > 2
+  float* ab = (float*)(smem + (ri > 2 ? 3 : 2) * nel);
+
+  for (int e = threadIdx.x; e < nel; e += kThreads) {
+    const int row = e / NX, x = e - row * NX;
+    const int gy = y0 + row, gx = x0 + x;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (gy >= 0 && gy < iny && gx >= 0 && gx < inx) {
+      const size_t o = (size_t)gy * inx + gx;
+      if (P.format == XCORR_HIP_CINT16) {
+        const unsigned u = ((const unsigned*)src)[o];
+        v.x = (float)(short)(u & 0xffffu);
+        v.y = (float)(short)(u >> 16);
+      } else {
+        v.x = ((const float*)src)[o];
+      }
+    }
+    A[e] = v;
+  }
+  __syncthreads();
+  if (ri == 1) {
+    for (int e = threadIdx.x; e < nel; e += kThreads) ab[e] = ampl(A[e]);
+  } else {
+    for (int e = threadIdx.x; e < nel; e += kThreads) {
+      const int row = e / NX, j = e - row * NX;
+      const int t = NX / 2 + j;
+      if (t % ri == 0) ab[e] = ampl(A[row * NX + t / ri]);
+    }
+    const float2* twn = tabs + P.tw_nx;
+    float2* X = lds_fft_rows<-1>(A, B, NX, G, twn);
+    float2* other = (X == A) ? B : A;
+    const float sc = 1.0f / (float)NX;
+    for (int r = 1; r < ri; r++) {
+      const float2* phi = tabs + P.phi + (size_t)(r - 1) * NX;
+      float2* Y = other;
+      float2* Ys = (ri > 2) ? Cb : X;         // scratch for the inverse passes
+      for (int e = threadIdx.x; e < nel; e += kThreads) {
+        const int k = e & (NX - 1);
+        Y[e] = cmul(X[e], phi[k]);
+      }
+      __syncthreads();
+      float2* Rr = lds_fft_rows<+1>(Y, Ys, NX, G, twn);
+      for (int e = threadIdx.x; e < nel; e += kThreads) {
+        const int row = e / NX, j = e - row * NX;
+        const int t = NX / 2 + j;
+        if (t % ri == r) {
+          float2 v = Rr[row * NX + t / ri];
+          v.x = v.x * sc;
+          v.y = v.y * sc;
+          ab[e] = ampl(v);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  float* out = amp + ((size_t)(b * 2 + img) * P.NY + (size_t)g * G) * NX;
+  long long s = 0;
+  for (int e = threadIdx.x; e < nel; e += kThreads) {
+    const float a = ab[e];
+    out[e] = a;
+    s += (long long)(a * 1048576.0f);
+  }
+  s = block_sum_i64(s, red);
+  if (threadIdx.x == 0) atomicAdd(&sums[b * 2 + img], (unsigned long long)s);
+}
+
+// --------------------------------------------------------------------------
+// stage (b): demean, mask, pack z = c1 + i c2, FFT along x
+// conv_ampl_demean.cu:111-139, xcorr.cu:385-401, do_freq_xcorr.cu:90-101
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_rowfft(DevParams P, const float* __restrict__ amp,
+                                                     const unsigned long long* __restrict__ sums,
+                                                     float2* __restrict__ Z, const float2* __restrict__ tabs) {
+  extern __shared__ float2 smem[];
+  const int NX = P.NX, NY = P.NY, G = P.G1, nel = G * NX;
+  const int groups = NY / G;
+  const int g = blockIdx.x % groups, b = blockIdx.x / groups;
+  const float m1f = (float)patch_mean(sums[2 * b], P.N);
+  const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
+  const float* a1 = amp + (size_t)(2 * b) * P.N + (size_t)g * G * NX;
+  const float* a2 = a1 + P.N;
+  float2* A = smem;
+  float2* Bf = A + nel;
+  for (int e = threadIdx.x; e < nel; e += kThreads) {
+    const int row = e / NX, x = e - row * NX;
+    const int y = g * G + row;
+    const float c1 = a1[e] - m1f;
+    float c2 = a2[e] - m2f;
+    const bool in = y >= P.ys && y < NY - P.ys && x >= P.xs && x < NX - P.xs;
+    if (!in) c2 = 0.0f;
+    A[e] = make_float2(c1, c2);
+  }
+  __syncthreads();
+  float2* R = lds_fft_rows<-1>(A, Bf, NX, G, tabs + P.tw_nx);
+  float2* out = Z + (size_t)b * P.N + (size_t)g * G * NX;
+  for (int e = threadIdx.x; e < nel; e += kThreads) out[e] = R[e];
+}
+
+// FFT along y, in place on Z (columns tile per block).
+__global__ void __launch_bounds__(kThreads) k_colfft(DevParams P, float2* __restrict__ Z,
+                                                     const float2* __restrict__ tabs) {
+  extern __shared__ float2 smem[];
+  const int NX = P.NX, NY = P.NY, C = P.C3, nel = C * NY;
+  const int tiles = NX / C;
+  const int ct = blockIdx.x % tiles, b = blockIdx.x / tiles;
+  float2* zb = Z + (size_t)b * P.N + ct * C;
+  float2* A = smem;
+  float2* Bf = A + nel;
+  for (int e = threadIdx.x; e < nel; e += kThreads) {
+    const int y = e / C, c = e - y * C;
+    A[c * NY + y] = zb[(size_t)y * NX + c];
+  }
+  __syncthreads();
+  float2* R = lds_fft_rows<-1>(A, Bf, NY, C, tabs + P.tw_ny);
+  for (int e = threadIdx.x; e < nel; e += kThreads) {
+    const int y = e / C, c = e - y * C;
+    zb[(size_t)y * NX + c] = R[c * NY + y];
+  }
+}
+
+// C3 = C1 conj(C2) (-1)^(kx+ky) with C1, C2 split from Z = FFT2(c1 + i c2);
+// IFFT along y; keep rows [ny_corr/2, ny_corr/2 + nyc).   do_freq_xcorr.cu:46-64, 103-113
+__global__ void __launch_bounds__(kThreads) k_product_colifft(DevParams P, const float2* __restrict__ Z,
+                                                              float2* __restrict__ D,
+                                                              const float2* __restrict__ tabs) {
+  extern __shared__ float2 smem[];
+  const int NX = P.NX, NY = P.NY, C = P.C3, nel = C * NY;
+  const int tiles = NX / C;
+  const int ct = blockIdx.x % tiles, b = blockIdx.x / tiles;
+  const float2* zb = Z + (size_t)b * P.N;
+  float2* A = smem;
+  float2* M = A + nel;
+  float2* O = M + nel;
+  for (int e = threadIdx.x; e < nel; e += kThreads) {
+    const int y = e / C, c = e - y * C;
+    const int kx = ct * C + c;
+    const int mx = (NX - kx) & (NX - 1);
+    A[c * NY + y] = zb[(size_t)y * NX + kx];
+    M[c * NY + y] = zb[(size_t)y * NX + mx];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nel; e += kThreads) {
+    const int c = e / NY, ky = e - c * NY;
+    const int kx = ct * C + c;
+    const float2 a = A[e];
+    const float2 zm = M[c * NY + ((NY - ky) & (NY - 1))];
+    const float2 bb = make_float2(zm.x, -zm.y);
+    const float2 c1 = make_float2((a.x + bb.x) * 0.5f, (a.y + bb.y) * 0.5f);
+    const float2 d = csub(a, bb);
+    const float2 c2c = make_float2(0.5f * d.y, 0.5f * d.x);
+    float2 p = cmul(c1, c2c);
+    if ((kx + ky) & 1) { p.x = -p.x; p.y = -p.y; }
+    O[e] = p;
+  }
+  __syncthreads();
+  float2* R = lds_fft_rows<+1>(O, A, NY, C, tabs + P.tw_ny);
+  const int r0 = P.ny_corr / 2;
+  float2* db = D + (size_t)b * P.nyc * NX + ct * C;
+  for (int e = threadIdx.x; e < P.nyc * C; e += kThreads) {
+    const int iy = e / C, c = e - iy * C;
+    db[(size_t)iy * NX + c] = R[c * NY + r0 + iy];
+  }
+}
+
+// IFFT along x of the kept rows, |c3|/(NX NY) on the lag window, argmax and
+// partial sums.   do_freq_xcorr.cu:67-84, 113-114; conv_ampl_demean.cu:434-554
+__global__ void __launch_bounds__(kThreads) k_rowifft_surf(DevParams P, const float2* __restrict__ D,
+                                                           float* __restrict__ surf,
+                                                           unsigned long long* __restrict__ peak,
+                                                           double* __restrict__ partial,
+                                                           const float2* __restrict__ tabs) {
+  extern __shared__ float2 smem[];
+  __shared__ unsigned long long redk[kThreads / 64];
+  __shared__ double redd[kThreads / 64];
+  const int NX = P.NX, G = P.G4, nel = G * NX, nxc = P.nxc;
+  const int groups = P.groups4;
+  const int g = blockIdx.x % groups, b = blockIdx.x / groups;
+  const float2* db = D + ((size_t)b * P.nyc + (size_t)g * G) * NX;
+  float2* A = smem;
+  float2* Bf = A + nel;
+  for (int e = threadIdx.x; e < nel; e += kThreads) A[e] = db[e];
+  __syncthreads();
+  float2* R = lds_fft_rows<+1>(A, Bf, NX, G, tabs + P.tw_nx);
+  const int q0 = P.nx_corr / 2;
+  const float sc = 1.0f / (float)P.N;
+  float* sb = surf + (size_t)b * P.nyc * nxc;
+  unsigned long long best = 0ull;
+  double sum = 0.0;
+  for (int e = threadIdx.x; e < G * nxc; e += kThreads) {
+    const int row = e / nxc, ix = e - row * nxc;
+    const float2 v = R[row * NX + q0 + ix];
+    const float re = v.x * sc, im = v.y * sc;
+    const double dr = re, di = im;
+    const float cv = (float)sqrt(dr * dr + di * di);
+    const unsigned idx = (unsigned)((g * G + row) * nxc + ix);
+    sb[idx] = cv;
+    const unsigned long long k = peak_key(cv, idx);
+    best = k > best ? k : best;
+    sum += (double)cv;
+  }
+  best = block_max_u64(best, redk);
+  sum = block_sum_f64(sum, redd);
+  if (threadIdx.x == 0) {
+    atomicMax(&peak[b], best);
+    partial[(size_t)b * groups + g] = sum;
+  }
+}
+
+// --------------------------------------------------------------------------
+// stages (c) + (d): NCC at the peak, high-res refinement, results
+// do_freq_xcorr.cu:150-200, do_time_int_xcorr.cu:71-100, highres_corr.cu:23-79,
+// conv_ampl_demean.cu:75-95, fft_interpolate_routines.cu:118-147
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* __restrict__ amp,
+                                                     const unsigned long long* __restrict__ sums,
+                                                     const float* __restrict__ surf,
+                                                     const double* __restrict__ surfd,
+                                                     const unsigned long long* __restrict__ peak,
+                                                     const double* __restrict__ partial,
+                                                     const int* __restrict__ tpeak,
+                                                     xcorr_hip_result* __restrict__ out,
+                                                     const float2* __restrict__ tabs) {
+  extern __shared__ float2 smem[];
+  __shared__ long long redl[kThreads / 64];
+  __shared__ unsigned long long redk[kThreads / 64];
+  __shared__ float md[64];
+  __shared__ float2 Xr[64];
+  __shared__ double s_msc;
+  const int b = blockIdx.x;
+  const int NX = P.NX, nxc = P.nxc, nyc = P.nyc;
+  const bool freq = P.corr_flag == XCORR_HIP_FREQ;
+  xcorr_hip_result res;
+  int ip, jp;
+  float cmax;
+  if (freq) {
+    float v;
+    unsigned idx;
+    key_decode(peak[b], &v, &idx);
+    cmax = v;
+    ip = (int)idx / nxc;
+    jp = (int)idx % nxc;
+    double s = 0.0;
+    for (int g = 0; g < P.groups4; g++) s += partial[(size_t)b * P.groups4 + g];
+    res.cave = s / (double)(nxc * nyc);
+    const int ipeak = ip - nyc / 2, jpeak = jp - nxc / 2;
+    const float m1f = (float)patch_mean(sums[2 * b], P.N);
+    const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
+    const float* a1 = amp + (size_t)(2 * b) * P.N;
+    const float* a2 = a1 + P.N;
+    long long num = 0, d1 = 0, d2 = 0;
+    const int nt = P.nx_corr * P.ny_corr;
+    for (int e = threadIdx.x; e < nt; e += kThreads) {
+      const int r = e / P.nx_corr, c = e - r * P.nx_corr;
+      const long long av = (int)(a1[(size_t)(P.ys + ipeak + r) * NX + P.xs + jpeak + c] - m1f);
+      const long long bv = (int)(a2[(size_t)(P.ys + r) * NX + P.xs + c] - m2f);
+      num += av * bv;
+      d1 += av * av;
+      d2 += bv * bv;
+    }
+    num = block_sum_i64(num, redl);
+    d1 = block_sum_i64(d1, redl);
+    d2 = block_sum_i64(d2, redl);
+    const double den = sqrt(1.0 * (double)d1 * (double)d2);
+    const double gamma = (den == 0.0) ? 0.0 : 100.0 * fabs((double)num / den);
+    if (threadIdx.x == 0) s_msc = (double)cmax / gamma;
+    res.xoff = (float)(-jpeak);
+    res.yoff = (float)(-ipeak);
+    res.corr = (float)gamma;
+  } else {
+    ip = tpeak[4 * b + 0];
+    jp = tpeak[4 * b + 1];
+    cmax = __int_as_float(tpeak[4 * b + 2]);
+    res.cave = 0.0;
+    double s = 0.0;
+    const double* sd = surfd + (size_t)b * nxc * nyc;
+    if (threadIdx.x == 0) {
+      for (int i = 0; i < nxc * nyc; i++) s += sd[i];
+    }
+    res.cave = s / (double)(nxc * nyc);
+    res.xoff = (float)(-(jp - nxc / 2));
+    res.yoff = (float)(-(ip - nyc / 2));
+    res.corr = cmax;
+  }
+  __syncthreads();
+  res.ip = ip;
+  res.jp = jp;
+  res.cmax = cmax;
+  res.xfrac = 0.0f;
+  res.yfrac = 0.0f;
+  res.hip = -1;
+  res.hjp = -1;
+
+  if (P.interp_flag) {
+    const int Lx = P.Lx, Ly = P.Ly;
+    float2* rows = smem;                 // 8 x Lx
+    float2* Xc = rows + 8 * Lx;          // Lx x 8
+    if (threadIdx.x < 64) {
+      const int iy = threadIdx.x >> 3, ix = threadIdx.x & 7;
+      const long k = (long)(ip - 4 + iy) * nxc + (jp - 4 + ix);
+      float q = 0.0f;
+      if (k >= 0 && k < (long)nxc * nyc) {
+        const float v = freq ? (float)((double)surf[(size_t)b * nxc * nyc + k] / s_msc)
+                             : (float)surfd[(size_t)b * nxc * nyc + k];
+        q = sqrtf(sqrtf(v));
+      }
+      md[threadIdx.x] = q;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+      float2 v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) v[k] = make_float2(md[threadIdx.x * 8 + k], 0.0f);
+      Dft<8, -1>::run(v);
+#pragma unroll
+      for (int k = 0; k < 8; k++) Xr[threadIdx.x * 8 + k] = v[k];
+    }
+    __syncthreads();
+    const float2* twx = tabs + P.tw_lx;
+    const float2* twy = tabs + P.tw_ly;
+    const float sx = 1.0f / 8.0f, sy = 1.0f / 8.0f;
+    for (int e = threadIdx.x; e < 8 * Lx; e += kThreads) {
+      const int iy = e / Lx, t = e - iy * Lx;
+      float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int kt = k < 4 ? k : k - 8;
+        int m = (kt * t) % Lx;
+        if (m < 0) m += Lx;
+        const float2 ex = twx[m];
+        const float2 X = Xr[iy * 8 + k];
+        acc.x = __builtin_fmaf(X.x, ex.x, __builtin_fmaf(-X.y, ex.y, acc.x));
+        acc.y = __builtin_fmaf(X.x, ex.y, __builtin_fmaf(X.y, ex.x, acc.y));
+      }
+      rows[iy * Lx + t] = make_float2(acc.x * sx, acc.y * sx);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < Lx; t += kThreads) {
+      float2 v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) v[k] = rows[k * Lx + t];
+      Dft<8, -1>::run(v);
+#pragma unroll
+      for (int k = 0; k < 8; k++) Xc[t * 8 + k] = v[k];
+    }
+    __syncthreads();
+    unsigned long long best = 0ull;
+    for (int e = threadIdx.x; e < Lx * Ly; e += kThreads) {
+      const int s = e / Lx, t = e - s * Lx;
+      float acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int kt = k < 4 ? k : k - 8;
+        int m = (kt * s) % Ly;
+        if (m < 0) m += Ly;
+        const float2 ey = twy[m];
+        const float2 X = Xc[t * 8 + k];
+        acc = __builtin_fmaf(X.x, ey.x, __builtin_fmaf(-X.y, ey.y, acc));
+      }
+      const unsigned long long kk = peak_key(acc * sy, (unsigned)e);
+      best = kk > best ? kk : best;
+    }
+    best = block_max_u64(best, redk);
+    float bv;
+    unsigned bidx;
+    key_decode(best, &bv, &bidx);
+    const int hp = (int)bidx / Lx, hq = (int)bidx % Lx;
+    const float half = (float)Ly / 2.0f;
+    res.xfrac = ((float)hq - half) / (float)P.ifc;
+    res.yfrac = ((float)hp - half) / (float)P.ifc;
+    res.hip = hp;
+    res.hjp = hq;
+  }
+  if (threadIdx.x == 0) out[b] = res;
+}
+
+__global__ void k_init(unsigned long long* __restrict__ sums, unsigned long long* __restrict__ peak, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2 * n) sums[i] = 0ull;
+  if (i < n) peak[i] = 0ull;
+}
+
+// --------------------------------------------------------------------------
+// host-side tables (same construction as the CPU restatement's twiddles)
+// --------------------------------------------------------------------------
+static void twiddle64(int n, long m, double* c, double* s) {
+  long mm = m % n;
+  if (mm < 0) mm += n;
+  if (n < 4) {
+    *c = (mm == 0) ? 1.0 : -1.0;
+    *s = 0.0;
+    return;
+  }
+  const long q = (4 * mm) / n;
+  const long r = mm - q * (n / 4);
+  double c0, s0;
+  if (r == 0) {
+    c0 = 1.0; s0 = 0.0;
+  } else if (8 * r == n) {
+    c0 = s0 = 0.70710678118654752440;
+  } else if (8 * r < n) {
+    const double t = 2.0 * M_PI * (double)r / (double)n;
+    c0 = cos(t); s0 = sin(t);
+  } else {
+    const double t = 2.0 * M_PI * (double)(n / 4 - r) / (double)n;
+    c0 = sin(t); s0 = cos(t);
+  }
+  switch (q) {
+    case 0: *c = c0; *s = s0; break;
+    case 1: *c = -s0; *s = c0; break;
+    case 2: *c = -c0; *s = -s0; break;
+    default: *c = s0; *s = -c0; break;
+  }
+}
+static float2 twf(int n, long m) {
+  double c, s;
+  twiddle64(n, m, &c, &s);
+  return make_float2((float)c, (float)s);
+}
+
+}  // namespace xc
+
+using namespace xc;
+
+// --------------------------------------------------------------------------
+// context
+// --------------------------------------------------------------------------
+struct KernelTimer {
+  const char* name;
+  float ms;
+  int launches;
+};
+
+struct xcorr_hip_ctx {
+  xcorr_hip_params p{};
+  DevParams dp{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int max_batch = 0;
+  const void* d_master = nullptr;
+  const void* d_slave = nullptr;
+  void* own_master = nullptr;
+  void* own_slave = nullptr;
+  bool images = false;
+  float* amp = nullptr;
+  unsigned long long* sums = nullptr;
+  float2* Z = nullptr;
+  float2* D = nullptr;
+  float* surf = nullptr;
+  double* surfd = nullptr;
+  unsigned long long* peak = nullptr;
+  double* partial = nullptr;
+  int* tpeak = nullptr;
+  int32_t* ints = nullptr;
+  xcorr_hip_window* d_win = nullptr;
+  xcorr_hip_result* d_out = nullptr;
+  float2* tabs = nullptr;
+  std::string err;
+  std::vector<KernelTimer> timers;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int last_n = 0;
+};
+
+static std::string g_create_err;
+
+static int fail(xcorr_hip_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  else g_create_err = msg;
+  return code;
+}
+
+#define HIPCK(ctx, call)                                                                  \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(ctx, XCORR_HIP_EDEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+extern "C" int xcorr_hip_abi_version(void) { return XCORR_HIP_ABI_VERSION; }
+
+extern "C" void xcorr_hip_default_params(xcorr_hip_params* p, int xsearch, int ysearch) {
+  memset(p, 0, sizeof(*p));
+  p->xsearch = xsearch;
+  p->ysearch = ysearch;
+  p->nx_corr = 2 * xsearch;
+  p->ny_corr = 2 * ysearch;
+  p->npx = p->nx_corr + 2 * xsearch;
+  p->npy = p->ny_corr + 2 * ysearch;
+  p->nxc = 2 * xsearch;
+  p->nyc = 2 * ysearch;
+  p->ri = 2;
+  p->interp_flag = 1;
+  p->interp_factor = 16;
+  p->n2x = 8;
+  p->n2y = 8;
+  p->corr_flag = XCORR_HIP_FREQ;
+  p->format = XCORR_HIP_CINT16;
+}
+
+static int validate(const xcorr_hip_params* p, std::string* why) {
+  auto bad = [&](const char* m) { *why = m; return XCORR_HIP_EINVAL; };
+  if (!is_pow2(p->xsearch) || !is_pow2(p->ysearch) || p->xsearch < 4 || p->ysearch < 4 ||
+      p->xsearch > 256 || p->ysearch > 256)
+    return bad("xsearch/ysearch must be powers of two in [4, 256]");
+  if (p->nx_corr != 2 * p->xsearch || p->ny_corr != 2 * p->ysearch || p->npx != 4 * p->xsearch ||
+      p->npy != 4 * p->ysearch || p->nxc != 2 * p->xsearch || p->nyc != 2 * p->ysearch)
+    return bad("derived sizes must follow parse_xcorr_input.c (nx_corr = nxc = 2*xsearch, npx = 4*xsearch)");
+  if (!is_pow2(p->ri) || p->ri > 16) return bad("ri must be 1 or a power of two <= 16");
+  if (p->interp_flag && (!is_pow2(p->interp_factor) || p->interp_factor > 64))
+    return bad("interp_factor must be a power of two <= 64");
+  if (p->n2x != 8 || p->n2y != 8) return bad("n2x/n2y must be 8");
+  if (p->corr_flag != XCORR_HIP_FREQ && p->corr_flag != XCORR_HIP_TIME && p->corr_flag != XCORR_HIP_TIME4)
+    return bad("corr_flag must be 0, 1 or 2");
+  if (p->format != XCORR_HIP_CINT16 && p->format != XCORR_HIP_FLOAT32) return bad("format must be 0 or 1");
+  return 0;
+}
+
+static void free_scratch(xcorr_hip_ctx* c) {
+  void* ptrs[] = {c->amp, c->sums, c->Z, c->D, c->surf, c->surfd, c->peak, c->partial, c->tpeak,
+                  c->ints, c->d_win, c->d_out, c->tabs};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  c->amp = nullptr; c->sums = nullptr; c->Z = nullptr; c->D = nullptr; c->surf = nullptr;
+  c->surfd = nullptr; c->peak = nullptr; c->partial = nullptr; c->tpeak = nullptr; c->ints = nullptr;
+  c->d_win = nullptr; c->d_out = nullptr; c->tabs = nullptr;
+}
+
+extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int device, xcorr_hip_ctx** out) {
+  if (!p || !out) return fail(nullptr, XCORR_HIP_EINVAL, "null argument");
+  *out = nullptr;
+  std::string why;
+  if (validate(p, &why)) return fail(nullptr, XCORR_HIP_EINVAL, why);
+  xcorr_hip_ctx* c = new xcorr_hip_ctx();
+  c->p = *p;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    delete c;
+    return fail(nullptr, XCORR_HIP_EDEVICE, "no HIP device");
+  }
+  if (device < 0) {
+    if (hipGetDevice(&device) != hipSuccess) device = 0;
+  }
+  if (device >= ndev) {
+    delete c;
+    return fail(nullptr, XCORR_HIP_EINVAL, "device index out of range");
+  }
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(nullptr, XCORR_HIP_EDEVICE, "stream creation failed");
+  }
+  DevParams& d = c->dp;
+  d.NX = p->npx; d.NY = p->npy; d.N = d.NX * d.NY;
+  d.xs = p->xsearch; d.ys = p->ysearch; d.nx_corr = p->nx_corr; d.ny_corr = p->ny_corr;
+  d.nxc = p->nxc; d.nyc = p->nyc; d.ri = p->ri; d.interp_flag = p->interp_flag;
+  d.ifc = p->interp_factor; d.n2x = p->n2x; d.n2y = p->n2y;
+  d.Lx = p->n2x * p->interp_factor; d.Ly = p->n2y * p->interp_factor;
+  d.corr_flag = p->corr_flag; d.format = p->format;
+  d.G1 = std::min(d.NY, std::max(1, kTile / d.NX));
+  d.G4 = std::min(d.nyc, std::max(1, kTile / d.NX));
+  d.C3 = std::min(d.NX, std::max(1, kTile / d.NY));
+  d.groups4 = d.nyc / d.G4;
+
+  // tables: e^{+2 pi i m/n} for n = NX, NY, Lx, Ly; range-interp phases
+  std::vector<float2> tab;
+  auto add_table = [&](int n) {
+    int off = (int)tab.size();
+    for (int m = 0; m < n; m++) tab.push_back(twf(n, m));
+    return off;
+  };
+  d.tw_nx = add_table(d.NX);
+  d.tw_ny = add_table(d.NY);
+  d.tw_lx = add_table(d.Lx);
+  d.tw_ly = add_table(d.Ly);
+  d.phi = (int)tab.size();
+  const int M = d.ri * d.NX;
+  for (int r = 1; r < d.ri; r++)
+    for (int k = 0; k < d.NX; k++) {
+      const long kt = (k < d.NX / 2) ? k : k - d.NX;
+      tab.push_back(twf(M, kt * r));
+    }
+  if (tab.empty()) tab.push_back(make_float2(0.f, 0.f));
+
+  if (max_batch <= 0) max_batch = 2048;
+  c->max_batch = max_batch;
+  const size_t B = (size_t)max_batch;
+  const size_t N = (size_t)d.N;
+  const size_t ns = (size_t)d.nxc * d.nyc;
+  hipError_t e = hipSuccess;
+  auto alloc = [&](void** ptr, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(ptr, bytes ? bytes : 16);
+  };
+  alloc((void**)&c->amp, B * 2 * N * sizeof(float));
+  alloc((void**)&c->sums, B * 2 * sizeof(unsigned long long));
+  if (p->corr_flag == XCORR_HIP_FREQ) {
+    alloc((void**)&c->Z, B * N * sizeof(float2));
+    alloc((void**)&c->D, B * d.nyc * d.NX * sizeof(float2));
+    alloc((void**)&c->surf, B * ns * sizeof(float));
+    alloc((void**)&c->partial, B * d.groups4 * sizeof(double));
+  } else {
+    alloc((void**)&c->surfd, B * ns * sizeof(double));
+    alloc((void**)&c->tpeak, B * 4 * sizeof(int));
+  }
+  alloc((void**)&c->peak, B * sizeof(unsigned long long));
+  alloc((void**)&c->d_win, B * sizeof(xcorr_hip_window));
+  alloc((void**)&c->d_out, B * sizeof(xcorr_hip_result));
+  alloc((void**)&c->tabs, tab.size() * sizeof(float2));
+  if (e == hipSuccess) e = hipMemcpy(c->tabs, tab.data(), tab.size() * sizeof(float2), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) {
+    std::string m = std::string("device allocation failed: ") + hipGetErrorString(e);
+    free_scratch(c);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(nullptr, XCORR_HIP_ENOMEM, m);
+  }
+  *out = c;
+  return 0;
+}
+
+extern "C" void xcorr_hip_destroy(xcorr_hip_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_scratch(c);
+  if (c->own_master) (void)hipFree(c->own_master);
+  if (c->own_slave) (void)hipFree(c->own_slave);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" const char* xcorr_hip_last_error(const xcorr_hip_ctx* c) {
+  return c ? c->err.c_str() : g_create_err.c_str();
+}
+
+extern "C" void* xcorr_hip_stream(xcorr_hip_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+extern "C" int xcorr_hip_sync(xcorr_hip_ctx* c) {
+  if (!c) return XCORR_HIP_EINVAL;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int xcorr_hip_set_images(xcorr_hip_ctx* c, const void* master, int m_nx, int m_ny, const void* slave,
+                                    int s_nx, int s_ny, unsigned flags) {
+  if (!c) return XCORR_HIP_EINVAL;
+  if (!master || !slave || m_nx <= 0 || m_ny <= 0 || s_nx <= 0 || s_ny <= 0)
+    return fail(c, XCORR_HIP_EINVAL, "set_images: bad image arguments");
+  HIPCK(c, hipSetDevice(c->device));
+  if (c->own_master) { (void)hipFree(c->own_master); c->own_master = nullptr; }
+  if (c->own_slave) { (void)hipFree(c->own_slave); c->own_slave = nullptr; }
+  const size_t es = c->p.format == XCORR_HIP_CINT16 ? 4 : 4;
+  if (flags & XCORR_HIP_DEVICE_PTRS) {
+    c->d_master = master;
+    c->d_slave = slave;
+  } else {
+    const size_t mb = (size_t)m_nx * m_ny * es, sb = (size_t)s_nx * s_ny * es;
+    if (hipMalloc(&c->own_master, mb) != hipSuccess || hipMalloc(&c->own_slave, sb) != hipSuccess)
+      return fail(c, XCORR_HIP_ENOMEM, "set_images: device allocation failed");
+    HIPCK(c, hipMemcpyAsync(c->own_master, master, mb, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(c->own_slave, slave, sb, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    c->d_master = c->own_master;
+    c->d_slave = c->own_slave;
+  }
+  c->dp.m_nx = m_nx; c->dp.m_ny = m_ny; c->dp.s_nx = s_nx; c->dp.s_ny = s_ny;
+  c->images = true;
+  return 0;
+}
+
+namespace {
+struct Launch {
+  xcorr_hip_ctx* c;
+  bool timing;
+  int slot = 0;
+  hipEvent_t evs[16] = {};
+  int nev = 0;
+};
+}  // namespace
+
+static int timer_index(xcorr_hip_ctx* c, const char* name) {
+  for (size_t i = 0; i < c->timers.size(); i++)
+    if (!strcmp(c->timers[i].name, name)) return (int)i;
+  c->timers.push_back({name, 0.0f, 0});
+  return (int)c->timers.size() - 1;
+}
+
+// Launch helper: records events around the kernel when timing is on.
+template <typename F>
+static int launch_timed(xcorr_hip_ctx* c, bool timing, const char* name, F&& f) {
+  if (!timing) {
+    f();
+    HIPCK(c, hipGetLastError());
+    return 0;
+  }
+  HIPCK(c, hipEventRecord(c->ev0, c->stream));
+  f();
+  HIPCK(c, hipGetLastError());
+  HIPCK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCK(c, hipEventSynchronize(c->ev1));
+  float ms = 0.0f;
+  HIPCK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  const int i = timer_index(c, name);
+  c->timers[i].ms += ms;
+  c->timers[i].launches += 1;
+  return 0;
+}
+
+static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xcorr_hip_result* d_out, bool timing);
+
+extern "C" int xcorr_hip_correlate(xcorr_hip_ctx* c, const xcorr_hip_window* win, int n, xcorr_hip_result* out,
+                                   unsigned flags) {
+  if (!c) return XCORR_HIP_EINVAL;
+  if (n < 0 || (n > 0 && (!win || !out))) return fail(c, XCORR_HIP_EINVAL, "correlate: bad arguments");
+  if (!c->images) return fail(c, XCORR_HIP_ESTATE, "correlate: no images set");
+  HIPCK(c, hipSetDevice(c->device));
+  const bool dev = (flags & XCORR_HIP_DEVICE_PTRS) != 0;
+  const bool timing = (flags & XCORR_HIP_TIMING) != 0;
+  if (timing)
+    for (auto& t : c->timers) { t.ms = 0.0f; t.launches = 0; }
+  for (int off = 0; off < n; off += c->max_batch) {
+    const int nb = std::min(c->max_batch, n - off);
+    const xcorr_hip_window* dw = dev ? win + off : c->d_win;
+    xcorr_hip_result* dout = dev ? out + off : c->d_out;
+    if (!dev) HIPCK(c, hipMemcpyAsync(c->d_win, win + off, nb * sizeof(xcorr_hip_window), hipMemcpyHostToDevice, c->stream));
+    int rc = run_batch(c, dw, nb, dout, timing);
+    if (rc) return rc;
+    if (!dev) {
+      HIPCK(c, hipMemcpyAsync(out + off, c->d_out, nb * sizeof(xcorr_hip_result), hipMemcpyDeviceToHost, c->stream));
+      HIPCK(c, hipStreamSynchronize(c->stream));
+    }
+  }
+  c->last_n = std::min(n, c->max_batch);
+  if (!(flags & XCORR_HIP_NO_SYNC)) HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int time_search(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, bool timing);
+
+static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xcorr_hip_result* d_out, bool timing) {
+  const DevParams& d = c->dp;
+  const int rc0 = launch_timed(c, timing, "k_init", [&] {
+    hipLaunchKernelGGL(k_init, dim3((2 * nb + 255) / 256), dim3(256), 0, c->stream, c->sums, c->peak, nb);
+  });
+  if (rc0) return rc0;
+  const size_t nel1 = (size_t)d.G1 * d.NX;
+  const size_t sh1 = nel1 * sizeof(float2) * (d.ri > 2 ? 3 : 2) + nel1 * sizeof(float);
+  int rc = launch_timed(c, timing, "k_range_ampl", [&] {
+    hipLaunchKernelGGL(k_range_ampl, dim3(nb * 2 * (d.NY / d.G1)), dim3(kThreads), sh1, c->stream, d, c->d_master,
+                       c->d_slave, d_win, c->amp, c->sums, (const float2*)c->tabs);
+  });
+  if (rc) return rc;
+  if (d.corr_flag == XCORR_HIP_FREQ) {
+    rc = launch_timed(c, timing, "k_rowfft", [&] {
+      hipLaunchKernelGGL(k_rowfft, dim3(nb * (d.NY / d.G1)), dim3(kThreads), nel1 * sizeof(float2) * 2, c->stream, d,
+                         (const float*)c->amp, (const unsigned long long*)c->sums, c->Z, (const float2*)c->tabs);
+    });
+    if (rc) return rc;
+    const size_t nel3 = (size_t)d.C3 * d.NY;
+    rc = launch_timed(c, timing, "k_colfft", [&] {
+      hipLaunchKernelGGL(k_colfft, dim3(nb * (d.NX / d.C3)), dim3(kThreads), nel3 * sizeof(float2) * 2, c->stream, d,
+                         c->Z, (const float2*)c->tabs);
+    });
+    if (rc) return rc;
+    rc = launch_timed(c, timing, "k_product_colifft", [&] {
+      hipLaunchKernelGGL(k_product_colifft, dim3(nb * (d.NX / d.C3)), dim3(kThreads), nel3 * sizeof(float2) * 3,
+                         c->stream, d, (const float2*)c->Z, c->D, (const float2*)c->tabs);
+    });
+    if (rc) return rc;
+    const size_t nel4 = (size_t)d.G4 * d.NX;
+    rc = launch_timed(c, timing, "k_rowifft_surf", [&] {
+      hipLaunchKernelGGL(k_rowifft_surf, dim3(nb * d.groups4), dim3(kThreads), nel4 * sizeof(float2) * 2, c->stream, d,
+                         (const float2*)c->D, c->surf, c->peak, c->partial, (const float2*)c->tabs);
+    });
+    if (rc) return rc;
+  } else {
+    rc = time_search(c, d_win, nb, timing);
+    if (rc) return rc;
+  }
+  const size_t sh5 = d.interp_flag ? (size_t)d.Lx * 16 * sizeof(float2) : 16;
+  rc = launch_timed(c, timing, "k_finish", [&] {
+    hipLaunchKernelGGL(k_finish, dim3(nb), dim3(kThreads), sh5, c->stream, d, (const float*)c->amp,
+                       (const unsigned long long*)c->sums, (const float*)c->surf, (const double*)c->surfd,
+                       (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)c->tpeak, d_out,
+                       (const float2*)c->tabs);
+  });
+  return rc;
+}
+
+static int time_search(xcorr_hip_ctx* c, const xcorr_hip_window*, int, bool) {
+  return fail(c, XCORR_HIP_EINVAL, "time-domain correlation is not available in this build");
+}
+
+extern "C" int xcorr_hip_kernel_times(xcorr_hip_ctx* c, const char** names, float* ms, int* launches, int cap) {
+  if (!c) return XCORR_HIP_EINVAL;
+  const int n = (int)c->timers.size();
+  for (int i = 0; i < n && i < cap; i++) {
+    if (names) names[i] = c->timers[i].name;
+    if (ms) ms[i] = c->timers[i].ms;
+    if (launches) launches[i] = c->timers[i].launches;
+  }
+  return n;
+}
+
+extern "C" long xcorr_hip_dump(xcorr_hip_ctx* c, int what, int slot, void* host, size_t bytes) {
+  if (!c || !host) return XCORR_HIP_EINVAL;
+  if (slot < 0 || slot >= c->max_batch) return fail(c, XCORR_HIP_EINVAL, "dump: slot out of range");
+  HIPCK(c, hipSetDevice(c->device));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  const DevParams& d = c->dp;
+  const void* src = nullptr;
+  size_t nbytes = 0;
+  std::vector<unsigned long long> tmp;
+  double means[2];
+  switch (what) {
+    case 0:
+    case 1:
+      src = c->amp + ((size_t)slot * 2 + what) * d.N;
+      nbytes = (size_t)d.N * sizeof(float);
+      break;
+    case 2:
+      if (d.corr_flag == XCORR_HIP_FREQ) {
+        src = c->surf + (size_t)slot * d.nxc * d.nyc;
+        nbytes = (size_t)d.nxc * d.nyc * sizeof(float);
+      } else {
+        src = c->surfd + (size_t)slot * d.nxc * d.nyc;
+        nbytes = (size_t)d.nxc * d.nyc * sizeof(double);
+      }
+      break;
+    case 3: {
+      unsigned long long s[2];
+      HIPCK(c, hipMemcpy(s, c->sums + 2 * slot, sizeof(s), hipMemcpyDeviceToHost));
+      for (int i = 0; i < 2; i++) means[i] = (double)(long long)s[i] * (1.0 / 1048576.0) / (double)d.N;
+      if (bytes < sizeof(means)) return fail(c, XCORR_HIP_EINVAL, "dump: buffer too small");
+      memcpy(host, means, sizeof(means));
+      return (long)sizeof(means);
+    }
+    default:
+      return fail(c, XCORR_HIP_EINVAL, "dump: unknown buffer");
+  }
+  if (bytes < nbytes) return fail(c, XCORR_HIP_EINVAL, "dump: buffer too small");
+  HIPCK(c, hipMemcpy(host, src, nbytes, hipMemcpyDeviceToHost));
+  return (long)nbytes;
+}

@@ -1,0 +1,124 @@
+"""Synthetic SAR SLC pairs with known offsets (the reference ships no data).
+
+Speckle model: m = (g_r + i g_i)/sqrt(2) * exp(sigma*n - sigma^2/2), sigma = 0.5,
+g, n iid N(0,1), scaled so I and Q have std ~ 500, rounded and clipped to int16.
+The aligned image is gamma * shift(m) + sqrt(1 - gamma^2) * w with w complex
+Gaussian at matched power.  Sign convention: aligned(x) = master(x - D), so a
+feature at master column c appears at aligned column c + D and xcorr reports
+dx = D (likewise dy).
+
+Images are returned as (lines, samples, 2) int16 arrays (I, Q interleaved), the
+layout of a GMTSAR SLC file (read_xcorr_data.c:11-29).
+"""
+import numpy as np
+
+SIGMA_LN = 0.5
+STD_IQ = 500.0
+
+
+def speckle(shape, rng):
+    g = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2.0)
+    tex = np.exp(SIGMA_LN * rng.standard_normal(shape) - SIGMA_LN ** 2 / 2.0)
+    return g * tex
+
+
+def fourier_shift(img, dx, dy):
+    """Shift a complex image so that out(x) = img(x - dx) (periodic, band-limited)."""
+    ny, nx = img.shape
+    kx = np.fft.fftfreq(nx)[None, :]
+    ky = np.fft.fftfreq(ny)[:, None]
+    F = np.fft.fft2(img)
+    F *= np.exp(-2j * np.pi * (kx * dx + ky * dy))
+    return np.fft.ifft2(F)
+
+
+def to_cint16(z, scale):
+    out = np.empty(z.shape + (2,), np.int16)
+    out[..., 0] = np.clip(np.rint(z.real * scale), -32768, 32767)
+    out[..., 1] = np.clip(np.rint(z.imag * scale), -32768, 32767)
+    return out
+
+
+def constant_shift_pair(n_range, n_azimuth, dx, dy, coherence, seed=1, noise_seed=2):
+    """Master/aligned pair with a constant (dx, dy) shift (BASELINE config 1 uses
+    1024 x 2048, dx = 3.37, dy = -12.81, coherence 0.8)."""
+    rng = np.random.default_rng(seed)
+    m = speckle((n_azimuth, n_range), rng)
+    s = fourier_shift(m, dx, dy)
+    rngw = np.random.default_rng(noise_seed)
+    pw = np.sqrt(np.mean(np.abs(m) ** 2))
+    w = (rngw.standard_normal(m.shape) + 1j * rngw.standard_normal(m.shape)) / np.sqrt(2.0) * pw
+    s = coherence * s + np.sqrt(1.0 - coherence ** 2) * w
+    scale = STD_IQ / np.sqrt(np.mean(np.abs(m) ** 2) / 2.0)
+    return to_cint16(m, scale), to_cint16(s, scale)
+
+
+def write_slc(path, img):
+    np.ascontiguousarray(img, dtype="<i2").tofile(path)
+
+
+def write_prm(path, slc_file, num_rng_bins, num_lines, rshift=0, ashift=0, prf=1679.9):
+    """Minimal PRM with the keys xcorr reads (sio_struct.c:141-222)."""
+    with open(path, "w") as f:
+        f.write("SLC_file = %s\n" % slc_file)
+        f.write("num_rng_bins = %d\n" % num_rng_bins)
+        f.write("num_patches = 1\n")
+        f.write("num_valid_az = %d\n" % num_lines)
+        f.write("rshift = %d\n" % rshift)
+        f.write("ashift = %d\n" % ashift)
+        f.write("PRF = %.6f\n" % prf)
+
+
+def strip_pair_torch(n_range, n_azimuth, dx_fn, dy_fn, coh_range=(0.3, 0.95), block=512, margin=64,
+                     seed=7, device="cuda"):
+    """Full-strip pair generated on the GPU (10800 x 27648,
+    dx = 2.1 + 1e-4 x, dy = -7.6 + 3e-4 y, coherence per 512 x 512 block uniform
+    in [0.3, 0.95]).  Each block is Fourier-shifted by the offset at its centre
+    (per-block Fourier resampling with a `margin`-sample apron).  Returns
+    (master, slave) as (lines, samples, 2) int16 tensors on `device`."""
+    import math
+
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    ny, nx = n_azimuth, n_range
+
+    def randn(*shape):
+        return torch.randn(*shape, generator=g, device=device, dtype=torch.float32)
+
+    m = torch.complex(randn(ny, nx), randn(ny, nx)) / math.sqrt(2.0)
+    m = m * torch.exp(SIGMA_LN * randn(ny, nx) - SIGMA_LN ** 2 / 2.0)
+    power = float((m.real.square() + m.imag.square()).mean())
+    T = block + 2 * margin
+    nby, nbx = -(-ny // block), -(-nx // block)
+    mp = torch.zeros(nby * block + 2 * margin, nbx * block + 2 * margin, dtype=torch.complex64, device=device)
+    mp[margin:margin + ny, margin:margin + nx] = m
+    s = torch.empty(nby * block, nbx * block, dtype=torch.complex64, device=device)
+    kx = torch.fft.fftfreq(T, device=device)[None, None, :]
+    ky = torch.fft.fftfreq(T, device=device)[None, :, None]
+    coh = coh_range[0] + (coh_range[1] - coh_range[0]) * torch.rand(nby, nbx, generator=g, device=device)
+    xc = torch.arange(nbx, device=device, dtype=torch.float64) * block + block / 2.0
+    for by in range(nby):
+        strip = mp[by * block: by * block + T]
+        tiles = strip.unfold(1, T, block).permute(1, 0, 2)            # (nbx, T, T)
+        yc = by * block + block / 2.0
+        dx = torch.as_tensor([dx_fn(float(v)) for v in xc], device=device, dtype=torch.float32)[:, None, None]
+        dy = float(dy_fn(yc))
+        F = torch.fft.fft2(tiles)
+        F = F * torch.exp(-2j * math.pi * (kx * dx + ky * dy)).to(torch.complex64)
+        sh = torch.fft.ifft2(F)[:, margin:margin + block, margin:margin + block]   # (nbx, block, block)
+        gam = coh[by][:, None, None]
+        w = torch.complex(randn(nbx, block, block), randn(nbx, block, block)) * math.sqrt(power / 2.0)
+        sh = gam * sh + torch.sqrt(1.0 - gam * gam) * w
+        s[by * block:(by + 1) * block] = sh.permute(1, 0, 2).reshape(block, nbx * block)
+    s = s[:ny, :nx]
+    scale = STD_IQ / math.sqrt(power / 2.0)
+
+    def q(z):
+        out = torch.empty(ny, nx, 2, dtype=torch.int16, device=device)
+        out[..., 0] = torch.clamp(torch.round(z.real * scale), -32768, 32767).to(torch.int16)
+        out[..., 1] = torch.clamp(torch.round(z.imag * scale), -32768, 32767).to(torch.int16)
+        return out
+
+    return q(m), q(s)
