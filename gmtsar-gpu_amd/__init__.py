@@ -234,3 +234,39 @@ def offsets(res, ri, rshift=0, ashift=0, y=None, astretcha=0.0):
     dx = (res["xoff"] / f32(ri)) - (res["xfrac"] / f32(ri)) + f32(rshift)
     dy = (res["yoff"] - res["yfrac"]) + f32(ashift) + ishft.astype(f32)
     return dx.astype(f32), dy.astype(f32)
+
+
+# ---------------------------------------------------------------------------
+# host library (libxcorr_host.so): the drop-in's CLI / PRM / location / output code
+# ---------------------------------------------------------------------------
+class HostConfig(ctypes.Structure):
+    """``xh_config`` of host/xcorr_host.h."""
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "format", "corr_flag", "offset_flag", "interp_flag", "interp_factor", "nx_corr", "ny_corr", "xsearch",
+        "ysearch", "m_nx", "m_ny", "s_nx", "s_ny", "nxc", "nyc", "npx", "npy", "nxl", "nyl", "x_offset",
+        "y_offset", "nlocs", "x_inc", "y_inc", "ri", "n2x", "n2y", "verbose")] + [
+        ("astretcha", ctypes.c_double), ("data1_name", ctypes.c_char * 256), ("data2_name", ctypes.c_char * 256),
+        ("filename", ctypes.c_char * 128)]
+
+
+_hlib = None
+
+
+def host_library():
+    global _hlib
+    if _hlib is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise ImportError("libxcorr_host.so is not built")
+        h = ctypes.CDLL(HOST_LIB_PATH)
+        C = ctypes.POINTER(HostConfig)
+        h.xh_set_defaults.argtypes = [C]
+        h.xh_parse_command_line.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), C, ctypes.c_char_p]
+        h.xh_handle_prm.argtypes = [ctypes.c_char_p, ctypes.c_char_p, C, ctypes.c_char_p, ctypes.c_size_t]
+        h.xh_get_locations.argtypes = [C, ctypes.c_void_p, ctypes.c_int]
+        I = ctypes.POINTER(ctypes.c_int32)
+        h.xh_window.argtypes = [C, ctypes.c_int, ctypes.c_int, I, I, I, I]
+        h.xh_window.restype = None
+        h.xh_format_result.argtypes = [C, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 5 + [ctypes.c_char_p,
+                                                                                                  ctypes.c_size_t]
+        _hlib = h
+    return _hlib

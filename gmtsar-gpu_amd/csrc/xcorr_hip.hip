@@ -37,6 +37,7 @@ struct DevParams {
   int corr_flag, format;
   int G1, G4, C3;                 // rows per block (k_range_ampl/k_rowfft, k_rowifft_surf), columns per block
   int groups4;                    // k_rowifft_surf blocks per patch
+  int tLR, tRC;                   // k_time_search: surface rows per block, template rows per LDS chunk
   int tw_nx, tw_ny, tw_lx, tw_ly, phi;   // offsets into the table buffer
   int m_nx, m_ny, s_nx, s_ny;
 };
@@ -353,6 +354,7 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
                                                      const unsigned long long* __restrict__ peak,
                                                      const double* __restrict__ partial,
                                                      const int* __restrict__ tpeak,
+                                                     const double* __restrict__ tcave,
                                                      xcorr_hip_result* __restrict__ out,
                                                      const float2* __restrict__ tabs) {
   extern __shared__ float2 smem[];
@@ -405,13 +407,7 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
     ip = tpeak[4 * b + 0];
     jp = tpeak[4 * b + 1];
     cmax = __int_as_float(tpeak[4 * b + 2]);
-    res.cave = 0.0;
-    double s = 0.0;
-    const double* sd = surfd + (size_t)b * nxc * nyc;
-    if (threadIdx.x == 0) {
-      for (int i = 0; i < nxc * nyc; i++) s += sd[i];
-    }
-    res.cave = s / (double)(nxc * nyc);
+    res.cave = tcave[b];
     res.xoff = (float)(-(jp - nxc / 2));
     res.yoff = (float)(-(ip - nyc / 2));
     res.corr = cmax;
@@ -508,6 +504,178 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
   if (threadIdx.x == 0) out[b] = res;
 }
 
+// --------------------------------------------------------------------------
+// stage (c), time domain: full search over all (2 ysearch) x (2 xsearch) lags
+// do_time_int_xcorr.cu:30-68 (calc_time_corr), 103-144 (calc_time_corr_hat),
+// 146-189 (do_time_corr).  Sums are exact: FP64 accumulation of integer
+// products for -time (|partial sums| < 2^53), 128-bit integers for -time4.
+// --------------------------------------------------------------------------
+typedef unsigned __int128 u128;
+
+// correctly rounded u64 -> double: hi*2^32 is exact, one rounding in the add
+__device__ __forceinline__ double u64_to_double(unsigned long long t) {
+  return (double)(unsigned)(t >> 32) * 4294967296.0 + (double)(unsigned)(t & 0xffffffffull);
+}
+// correctly rounded u128 -> double: top 64 bits with a sticky bit
+__device__ __forceinline__ double u128_to_double(u128 x) {
+  const unsigned long long hi = (unsigned long long)(x >> 64);
+  if (hi == 0ull) return u64_to_double((unsigned long long)x);
+  const int shift = 64 - __clzll((long long)hi);
+  unsigned long long t = (unsigned long long)(x >> shift);
+  const u128 mask = (((u128)1) << shift) - 1;
+  if (x & mask) t |= 1ull;
+  return ldexp(u64_to_double(t), shift);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kThreads) k_time_search(DevParams P, const float* __restrict__ amp,
+                                                          const unsigned long long* __restrict__ sums,
+                                                          double* __restrict__ surfd) {
+  extern __shared__ int ism[];
+  __shared__ long long redl[kThreads / 64];
+  __shared__ u128 red128[kThreads];
+  const int NX = P.NX, nxc = P.nxc, LR = P.tLR, RC = P.tRC, ncx = P.nx_corr, ncy = P.ny_corr;
+  const int nblk = P.nyc / LR;
+  const int blk = blockIdx.x % nblk, b = blockIdx.x / nblk;
+  const int ic0 = blk * LR;                       // first surface row of this block
+  const float m1f = (float)patch_mean(sums[2 * b], P.N);
+  const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
+  const float* a1 = amp + (size_t)(2 * b) * P.N;
+  const float* a2 = a1 + P.N;
+  int* Mb = ism;                                  // (RC + LR - 1) x NX master ints
+  int* Tb = ism + (RC + LR - 1) * NX;             // RC x nx_corr template ints
+  const int nlag = LR * nxc;
+  double ab[2] = {0.0, 0.0}, aa[2] = {0.0, 0.0};
+  u128 n4[2] = {0, 0}, a4[2] = {0, 0};
+  long long b2 = 0;
+  u128 b4 = 0;
+  for (int r0 = 0; r0 < ncy; r0 += RC) {
+    const int rc = min(RC, ncy - r0);
+    __syncthreads();
+    const int mrows = rc + LR - 1;
+    const size_t mrow0 = (size_t)(ic0 + r0);      // = ysearch + ioff0 + r0 with ioff0 = ic0 - ysearch
+    for (int e = threadIdx.x; e < mrows * NX; e += kThreads) {
+      const int rr = e / NX, x = e - rr * NX;
+      Mb[e] = (int)(a1[(mrow0 + rr) * NX + x] - m1f);
+    }
+    for (int e = threadIdx.x; e < rc * ncx; e += kThreads) {
+      const int rr = e / ncx, c = e - rr * ncx;
+      const int v = (int)(a2[(size_t)(P.ys + r0 + rr) * NX + P.xs + c] - m2f);
+      Tb[e] = v;
+      const long long v2 = (long long)v * v;
+      if (MODE == 0) b2 += v2;
+      else b4 += (u128)(unsigned long long)v2 * (unsigned long long)v2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int t = threadIdx.x + q * kThreads;
+      if (t >= nlag) break;
+      const int lr = t / nxc, jc = t - lr * nxc;
+      for (int rr = 0; rr < rc; rr++) {
+        const int* arow = Mb + (lr + rr) * NX + jc;
+        const int* brow = Tb + rr * ncx;
+        if (MODE == 0) {
+          double s_ab = ab[q], s_aa = aa[q];
+          for (int c = 0; c < ncx; c++) {
+            const double av = (double)arow[c], bv = (double)brow[c];
+            s_ab = __builtin_fma(av, bv, s_ab);
+            s_aa = __builtin_fma(av, av, s_aa);
+          }
+          ab[q] = s_ab;
+          aa[q] = s_aa;
+        } else {
+          u128 s_n = n4[q], s_a = a4[q];
+          for (int c = 0; c < ncx; c++) {
+            const long long av = arow[c], bv = brow[c];
+            const unsigned long long ua = (unsigned long long)(av * av), ub = (unsigned long long)(bv * bv);
+            s_n += (u128)ua * ub;
+            s_a += (u128)ua * ua;
+          }
+          n4[q] = s_n;
+          a4[q] = s_a;
+        }
+      }
+    }
+  }
+  double d2d;
+  if (MODE == 0) {
+    d2d = (double)block_sum_i64(b2, redl);
+  } else {
+    __syncthreads();
+    red128[threadIdx.x] = b4;
+    __syncthreads();
+    u128 t4 = 0;
+    for (int i = 0; i < kThreads; i++) t4 += red128[i];
+    d2d = u128_to_double(t4);
+  }
+  double* sd = surfd + (size_t)b * nxc * P.nyc;
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int t = threadIdx.x + q * kThreads;
+    if (t >= nlag) break;
+    const int lr = t / nxc, jc = t - lr * nxc;
+    double g;
+    if (MODE == 0) {
+      const double den = sqrt(1.0 * aa[q] * d2d);
+      g = (den == 0.0) ? 0.0 : 100.0 * fabs(ab[q] / den);
+    } else {
+      const double den = sqrt(u128_to_double(a4[q]) * d2d);
+      const double gg = (den == 0.0) ? 0.0 : fabs(u128_to_double(n4[q]) / den);
+      g = (gg <= 0.5) ? 0.0 : 100.0 * sqrt(gg * 2.0 - 1.0);
+    }
+    sd[(size_t)(ic0 + lr) * nxc + jc] = g;
+  }
+}
+
+// Peak of the time-domain surface with do_time_corr's scan semantics
+// (do_time_int_xcorr.cu:162-180): m = -1; for each lag in row-major order,
+// if |g| > m then m = (float)|g| and the lag becomes the peak.  Equivalently the
+// peak is the last lag whose g exceeds (float)(max of all earlier g); the
+// reported correlation is (float)(max g).
+__global__ void __launch_bounds__(kThreads) k_time_peak(DevParams P, const double* __restrict__ surfd,
+                                                        int* __restrict__ tpeak, double* __restrict__ tcave) {
+  __shared__ double smax[kThreads];
+  __shared__ double ssum[kThreads];
+  __shared__ unsigned long long redk[kThreads / 64];
+  const int b = blockIdx.x;
+  const int total = P.nxc * P.nyc;
+  const int chunk = (total + kThreads - 1) / kThreads;
+  const int lo = min(total, (int)threadIdx.x * chunk), hi = min(total, lo + chunk);
+  const double* g = surfd + (size_t)b * total;
+  double mx = -1.0, sm = 0.0;
+  for (int i = lo; i < hi; i++) {
+    const double v = fabs(g[i]);
+    mx = v > mx ? v : mx;
+    sm += g[i];
+  }
+  smax[threadIdx.x] = mx;
+  ssum[threadIdx.x] = sm;
+  __syncthreads();
+  double pm = -1.0;
+  for (int u = 0; u < (int)threadIdx.x; u++) pm = smax[u] > pm ? smax[u] : pm;
+  long long last = -1;
+  for (int i = lo; i < hi; i++) {
+    const double v = fabs(g[i]);
+    if (v > (double)(float)pm) last = i;
+    pm = v > pm ? v : pm;
+  }
+  const unsigned long long key = block_max_u64((unsigned long long)(last + 1), redk);
+  if (threadIdx.x == 0) {
+    double gmax = -1.0, s = 0.0;
+    for (int u = 0; u < kThreads; u++) {
+      gmax = smax[u] > gmax ? smax[u] : gmax;
+      s += ssum[u];
+    }
+    const int idx = (int)key - 1;
+    tpeak[4 * b + 0] = idx / P.nxc;
+    tpeak[4 * b + 1] = idx % P.nxc;
+    tpeak[4 * b + 2] = __float_as_int((float)gmax);
+    tpeak[4 * b + 3] = 0;
+    tcave[b] = s / (double)total;
+  }
+}
+
 __global__ void k_init(unsigned long long* __restrict__ sums, unsigned long long* __restrict__ peak, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < 2 * n) sums[i] = 0ull;
@@ -585,7 +753,7 @@ struct xcorr_hip_ctx {
   unsigned long long* peak = nullptr;
   double* partial = nullptr;
   int* tpeak = nullptr;
-  int32_t* ints = nullptr;
+  double* tcave = nullptr;
   xcorr_hip_window* d_win = nullptr;
   xcorr_hip_result* d_out = nullptr;
   float2* tabs = nullptr;
@@ -642,8 +810,8 @@ static int validate(const xcorr_hip_params* p, std::string* why) {
       p->npy != 4 * p->ysearch || p->nxc != 2 * p->xsearch || p->nyc != 2 * p->ysearch)
     return bad("derived sizes must follow parse_xcorr_input.c (nx_corr = nxc = 2*xsearch, npx = 4*xsearch)");
   if (!is_pow2(p->ri) || p->ri > 16) return bad("ri must be 1 or a power of two <= 16");
-  if (p->interp_flag && (!is_pow2(p->interp_factor) || p->interp_factor > 64))
-    return bad("interp_factor must be a power of two <= 64");
+  if (p->interp_flag && (!is_pow2(p->interp_factor) || p->interp_factor > 32))
+    return bad("interp_factor must be a power of two <= 32");
   if (p->n2x != 8 || p->n2y != 8) return bad("n2x/n2y must be 8");
   if (p->corr_flag != XCORR_HIP_FREQ && p->corr_flag != XCORR_HIP_TIME && p->corr_flag != XCORR_HIP_TIME4)
     return bad("corr_flag must be 0, 1 or 2");
@@ -653,11 +821,11 @@ static int validate(const xcorr_hip_params* p, std::string* why) {
 
 static void free_scratch(xcorr_hip_ctx* c) {
   void* ptrs[] = {c->amp, c->sums, c->Z, c->D, c->surf, c->surfd, c->peak, c->partial, c->tpeak,
-                  c->ints, c->d_win, c->d_out, c->tabs};
+                  c->tcave, c->d_win, c->d_out, c->tabs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   c->amp = nullptr; c->sums = nullptr; c->Z = nullptr; c->D = nullptr; c->surf = nullptr;
-  c->surfd = nullptr; c->peak = nullptr; c->partial = nullptr; c->tpeak = nullptr; c->ints = nullptr;
+  c->surfd = nullptr; c->peak = nullptr; c->partial = nullptr; c->tpeak = nullptr; c->tcave = nullptr;
   c->d_win = nullptr; c->d_out = nullptr; c->tabs = nullptr;
 }
 
@@ -696,6 +864,13 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   d.G4 = std::min(d.nyc, std::max(1, kTile / d.NX));
   d.C3 = std::min(d.NX, std::max(1, kTile / d.NY));
   d.groups4 = d.nyc / d.G4;
+  // time search: LR surface rows x nxc lags per block (<= 2 lags per thread),
+  // template rows staged RC at a time within 64 KB of LDS
+  d.tLR = std::min(d.nyc, std::max(1, kThreads / d.nxc));
+  {
+    const int budget = 65536 / 4 - (d.tLR - 1) * d.NX;
+    d.tRC = std::max(1, std::min(16, budget / (d.NX + d.nx_corr)));
+  }
 
   // tables: e^{+2 pi i m/n} for n = NX, NY, Lx, Ly; range-interp phases
   std::vector<float2> tab;
@@ -736,6 +911,7 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   } else {
     alloc((void**)&c->surfd, B * ns * sizeof(double));
     alloc((void**)&c->tpeak, B * 4 * sizeof(int));
+    alloc((void**)&c->tcave, B * sizeof(double));
   }
   alloc((void**)&c->peak, B * sizeof(unsigned long long));
   alloc((void**)&c->d_win, B * sizeof(xcorr_hip_window));
@@ -920,14 +1096,32 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
   rc = launch_timed(c, timing, "k_finish", [&] {
     hipLaunchKernelGGL(k_finish, dim3(nb), dim3(kThreads), sh5, c->stream, d, (const float*)c->amp,
                        (const unsigned long long*)c->sums, (const float*)c->surf, (const double*)c->surfd,
-                       (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)c->tpeak, d_out,
-                       (const float2*)c->tabs);
+                       (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)c->tpeak,
+                       (const double*)c->tcave, d_out, (const float2*)c->tabs);
   });
   return rc;
 }
 
-static int time_search(xcorr_hip_ctx* c, const xcorr_hip_window*, int, bool) {
-  return fail(c, XCORR_HIP_EINVAL, "time-domain correlation is not available in this build");
+static int time_search(xcorr_hip_ctx* c, const xcorr_hip_window*, int nb, bool timing) {
+  const DevParams& d = c->dp;
+  const size_t sh = ((size_t)(d.tRC + d.tLR - 1) * d.NX + (size_t)d.tRC * d.nx_corr) * sizeof(int);
+  const dim3 grid(nb * (d.nyc / d.tLR));
+  int rc;
+  if (d.corr_flag == XCORR_HIP_TIME)
+    rc = launch_timed(c, timing, "k_time_search", [&] {
+      hipLaunchKernelGGL(k_time_search<0>, grid, dim3(kThreads), sh, c->stream, d, (const float*)c->amp,
+                         (const unsigned long long*)c->sums, c->surfd);
+    });
+  else
+    rc = launch_timed(c, timing, "k_time_search4", [&] {
+      hipLaunchKernelGGL(k_time_search<1>, grid, dim3(kThreads), sh, c->stream, d, (const float*)c->amp,
+                         (const unsigned long long*)c->sums, c->surfd);
+    });
+  if (rc) return rc;
+  return launch_timed(c, timing, "k_time_peak", [&] {
+    hipLaunchKernelGGL(k_time_peak, dim3(nb), dim3(kThreads), 0, c->stream, d, (const double*)c->surfd, c->tpeak,
+                       c->tcave);
+  });
 }
 
 extern "C" int xcorr_hip_kernel_times(xcorr_hip_ctx* c, const char** names, float* ms, int* launches, int cap) {

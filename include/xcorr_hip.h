@@ -58,7 +58,7 @@ typedef struct xcorr_hip_params {
   int32_t nxc, nyc;               /* correlation surface size */
   int32_t ri;                     /* range interpolation factor: 1 (-norange) or a power of two <= 16 */
   int32_t interp_flag;            /* 1: high-res sub-pixel refinement, 0: -nointerp */
-  int32_t interp_factor;          /* high-res factor, power of two in [1, 64] */
+  int32_t interp_factor;          /* high-res factor, power of two in [1, 32] */
   int32_t n2x, n2y;               /* high-res neighbourhood, 8 x 8 */
   int32_t corr_flag;              /* XCORR_HIP_TIME / TIME4 / FREQ */
   int32_t format;                 /* XCORR_HIP_CINT16 / XCORR_HIP_FLOAT32 */

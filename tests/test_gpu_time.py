@@ -1,0 +1,72 @@
+"""GPU parity of the time-domain search (-time, -time4): exact integer
+correlation over every lag (do_time_int_xcorr.cu:146-189), the scan-order peak,
+then the high-res refinement of the correlation surface.  Bit-exact against the
+FP32 CPU restatement; the full-search surface against an int64 brute force."""
+import numpy as np
+import pytest
+
+from conftest import gpu_present
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pair(synth):
+    if not gpu_present():
+        pytest.skip("no GPU")
+    return synth.constant_shift_pair(512, 768, 2.4, -1.7, 0.7, seed=11, noise_seed=12)
+
+
+def _run(xg, orc, pair, xs, ys, corr_flag, ri=2, ifc=16, interp=1, nxl=4, nyl=3):
+    m, s = pair
+    p = xg.make_params(xs, ys, ri=ri, interp_factor=ifc, interp_flag=interp, corr_flag=corr_flag)
+    x, y = xg.get_locations(512, 768, nxl, nyl, xs, ys)
+    w = xg.make_windows(x, y, p.npx, p.npy)
+    with xg.Engine(p, max_batch=5) as eng:
+        eng.set_images(m, s)
+        got = eng.correlate(w)
+        surf0 = eng.dump(2, 0)
+        amp = (eng.dump(0, 0), eng.dump(1, 0), eng.dump(3, 0))
+    op = orc.make_params(xs, ys, ri=ri, interp_factor=ifc, interp_flag=interp, corr_flag=corr_flag)
+    locs = np.stack([w["mx0"], w["my0"], w["sx0"], w["sy0"]], 1).astype(np.int32)
+    ref = orc.run(op, m, s, locs, precision=32)
+    return p, got, ref, surf0, amp
+
+
+def _exact(got, ref):
+    for f in ("xoff", "yoff", "corr", "xfrac", "yfrac", "cmax", "ip", "jp", "hip", "hjp"):
+        a, b = got[f], ref[f]
+        same = a.view(np.uint32) == b.view(np.uint32) if a.dtype.kind == "f" else a == b
+        assert np.all(same), (f, np.nonzero(~same)[0][:5], a[~same][:5], b[~same][:5])
+    np.testing.assert_allclose(got["cave"], ref["cave"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("xs,ys", [(8, 8), (16, 8), (16, 32)])
+@pytest.mark.parametrize("corr_flag", [0, 1], ids=["time", "time4"])
+def test_time_bit_exact(xg, orc, pair, xs, ys, corr_flag):
+    p, got, ref, _, _ = _run(xg, orc, pair, xs, ys, corr_flag)
+    _exact(got, ref)
+    assert np.all(got["corr"] > 0)
+
+
+def test_time_surface_matches_int64_bruteforce(xg, orc, pair):
+    p, got, ref, surf, (a1, a2, means) = _run(xg, orc, pair, 8, 8, 0, nxl=2, nyl=1)
+    m1f, m2f = np.float32(means[0]), np.float32(means[1])
+    i1 = (a1 - m1f).astype(np.int64)            # C truncation toward zero
+    i2 = (a2 - m2f).astype(np.int64)
+    xs = ys = 8
+    b = i2[ys:ys + 2 * ys, xs:xs + 2 * xs]
+    brute = np.zeros((2 * ys, 2 * xs))
+    for ic in range(2 * ys):
+        for jc in range(2 * xs):
+            a = i1[ic:ic + 2 * ys, jc:jc + 2 * xs]
+            num, d1, d2 = int((a * b).sum()), int((a * a).sum()), int((b * b).sum())
+            den = np.sqrt(float(d1) * float(d2))
+            brute[ic, jc] = 0.0 if den == 0 else 100.0 * abs(num / den)
+    assert np.array_equal(surf, brute)
+
+
+def test_time_nointerp_and_norange(xg, orc, pair):
+    _, got, ref, _, _ = _run(xg, orc, pair, 16, 16, 0, ri=1, interp=0)
+    _exact(got, ref)
+    assert np.all(got["xfrac"] == 0) and np.all(got["hip"] == -1)
