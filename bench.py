@@ -49,7 +49,21 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="config 1 (1024 x 2048, 1000 patches) instead")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
+    ap.add_argument("--traffic", default="", metavar="JSON",
+                    help="per-launch HBM bytes of the dominant kernel from rocprofv3 FETCH_SIZE/WRITE_SIZE runs")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        nbytes = args.copy_probe << 20
+        src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the xcorr path has independent patches and this build "
               "runs on a single MI355X (--gpus %d requested)" % args.gpus, flush=True)
@@ -112,8 +126,12 @@ def main():
     per_launch_ms = dom_ms / dom_launches
     patches_per_launch = n / dom_launches
     achieved = BYTES_PER_PATCH * patches_per_launch / (per_launch_ms * 1e-3) / 1e9
+    traffic = None
+    if args.traffic:
+        tr = json.loads(args.traffic) if args.traffic.lstrip().startswith("{") else json.load(open(args.traffic))
+        traffic = tr.get(dom_name)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": dom_name, "launch_ms": round(per_launch_ms, 4),
                 "patches_per_launch": patches_per_launch}
     kernels = {k: {"ms_total": round(v[0], 4), "launches": v[1]} for k, v in kt.items()}

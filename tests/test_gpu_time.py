@@ -46,7 +46,8 @@ def _exact(got, ref):
 def test_time_bit_exact(xg, orc, pair, xs, ys, corr_flag):
     p, got, ref, _, _ = _run(xg, orc, pair, xs, ys, corr_flag)
     _exact(got, ref)
-    assert np.all(got["corr"] > 0)
+    # -time4 reports 0 where the 4th-power ratio is <= 0.5 (do_time_int_xcorr.cu:135-139)
+    assert np.all(got["corr"] > 0) if corr_flag == 0 else np.all(got["corr"] >= 0)
 
 
 def test_time_surface_matches_int64_bruteforce(xg, orc, pair):
