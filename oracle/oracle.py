@@ -166,3 +166,17 @@ def run(params, master, slave, locs, precision=32, fmt=0, nthreads=0):
     if rc != 0:
         raise RuntimeError("oracle run failed rc=%d" % rc)
     return out
+
+
+def interp2d(md, ifc, precision=32):
+    """Real part of the ifc-fold 2-D zero-pad interpolation of a real 8x8 (or
+    n2y x n2x) neighbourhood."""
+    ft = np.float32 if precision == 32 else np.float64
+    a = np.ascontiguousarray(md, dtype=ft)
+    n2y, n2x = a.shape
+    out = np.zeros((n2y * ifc, n2x * ifc), ft)
+    f = lib().xo32_interp2d if precision == 32 else lib().xo64_interp2d
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    if f(_ptr(a), n2x, n2y, ifc, _ptr(out)) != 0:
+        raise RuntimeError("interp2d failed")
+    return out

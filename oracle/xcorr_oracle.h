@@ -90,6 +90,10 @@ int  xo32_fft(xo_c32 *x, int n, int dir);                      /* in place; dir 
 int  xo64_fft(xo_c64 *x, int n, int dir);
 int  xo32_range_interp_row(const xo_c32 *x, int n, int ri, float *amp);
 int  xo64_range_interp_row(const xo_c64 *x, int n, int ri, double *amp);
+/* high-res: real part of the ifc-fold 2-D zero-pad interpolation of a real
+ * n2y x n2x neighbourhood, (n2y*ifc) x (n2x*ifc) row-major */
+int  xo32_interp2d(const float *md, int n2x, int n2y, int ifc, float *hres);
+int  xo64_interp2d(const double *md, int n2x, int n2y, int ifc, double *hres);
 
 /* --- one patch: windows are npy*npx complex samples (row-major) --- */
 int xo32_patch(const xo_params *p, const xo_c32 *mwin, const xo_c32 *swin,

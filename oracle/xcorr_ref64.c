@@ -89,6 +89,29 @@ int xo64_range_interp_row(const c64 *x, int n, int ri, double *amp) {
   return 0;
 }
 
+/* 2-D zero-pad interpolation by ifc: 1-D interpolation of the rows, then of
+ * the columns of the complex result (fft_interpolate_routines.cu:118-147);
+ * returns the real part */
+int xo64_interp2d(const double *md, int n2x, int n2y, int ifc, double *hres) {
+  const int Lx = n2x * ifc, Ly = n2y * ifc;
+  c64 *row = (c64 *)malloc(sizeof(c64) * n2x);
+  c64 *t1 = (c64 *)malloc(sizeof(c64) * n2y * Lx);
+  c64 *colv = (c64 *)malloc(sizeof(c64) * n2y);
+  c64 *colo = (c64 *)malloc(sizeof(c64) * Ly);
+  if (!row || !t1 || !colv || !colo) { free(row); free(t1); free(colv); free(colo); return -2; }
+  for (int iy = 0; iy < n2y; iy++) {
+    for (int k = 0; k < n2x; k++) { row[k].re = md[iy * n2x + k]; row[k].im = 0.0; }
+    interp1d(row, n2x, ifc, t1 + (size_t)iy * Lx);
+  }
+  for (int t = 0; t < Lx; t++) {
+    for (int k = 0; k < n2y; k++) colv[k] = t1[(size_t)k * Lx + t];
+    interp1d(colv, n2y, ifc, colo);
+    for (int s = 0; s < Ly; s++) hres[(size_t)s * Lx + t] = colo[s].re;
+  }
+  free(row); free(t1); free(colv); free(colo);
+  return 0;
+}
+
 static void fft2(c64 *a, int NX, int NY, int dir, c64 *col) {
   for (int y = 0; y < NY; y++) xo64_fft(a + (size_t)y * NX, NX, dir);
   for (int x = 0; x < NX; x++) {
@@ -184,6 +207,7 @@ int xo64_patch(const xo_params *p, const xo_c32 *mwin, const xo_c32 *swin, xo_re
     jp = idx % nxc;
     gamma = ncc64(iv[0], iv[1], p, ip - nyc / 2, jp - nxc / 2, 0);
     msc = cmax / gamma;
+    if (dump && dump->surf) memcpy(dump->surf, surf, sizeof(double) * nxc * nyc);
     for (int i = 0; i < nxc * nyc; i++) surf[i] /= msc;
     out->xoff = (float)(-(jp - nxc / 2));
     out->yoff = (float)(-(ip - nyc / 2));
@@ -199,6 +223,7 @@ int xo64_patch(const xo_params *p, const xo_c32 *mwin, const xo_c32 *swin, xo_re
         if (fabs(g) > best) { best = fabs(g); ipk = ic; jpk = jc; }
       }
     ip = ipk; jp = jpk; cmax = best; cave = sum / (nxc * nyc);
+    if (dump && dump->surf) memcpy(dump->surf, surf, sizeof(double) * nxc * nyc);
     out->xoff = (float)(-(jp - nxc / 2));
     out->yoff = (float)(-(ip - nyc / 2));
     out->corr = (float)best;
@@ -213,45 +238,32 @@ int xo64_patch(const xo_params *p, const xo_c32 *mwin, const xo_c32 *swin, xo_re
   if (p->interp_flag) {
     const int n2x = p->n2x, n2y = p->n2y, ifc = p->interp_factor;
     const int Lx = n2x * ifc, Ly = n2y * ifc;
-    c64 *md = (c64 *)calloc((size_t)n2x * n2y, sizeof(c64));
-    c64 *t1 = (c64 *)malloc(sizeof(c64) * n2y * Lx);
-    c64 *t3 = (c64 *)malloc(sizeof(c64) * Lx * Ly);
-    c64 *colv = (c64 *)malloc(sizeof(c64) * n2y);
-    c64 *colo = (c64 *)malloc(sizeof(c64) * Ly);
+    double *md = (double *)calloc((size_t)n2x * n2y, sizeof(double));
     double *hres = (double *)malloc(sizeof(double) * Lx * Ly);
-    if (!md || !t1 || !t3 || !colv || !colo || !hres) { rc = -2; goto hdone; }
+    if (!md || !hres) { free(md); free(hres); rc = -2; goto done; }
     for (int iy = 0; iy < n2y; iy++)
       for (int ix = 0; ix < n2x; ix++) {
         long k = (long)(ip - n2y / 2 + iy) * nxc + (jp - n2x / 2 + ix);
-        if (k >= 0 && k < (long)nxc * nyc) md[iy * n2x + ix].re = pow(surf[k], 0.25);
+        if (k >= 0 && k < (long)nxc * nyc) md[iy * n2x + ix] = pow(surf[k], 0.25);
       }
-    for (int iy = 0; iy < n2y; iy++) interp1d(md + iy * n2x, n2x, ifc, t1 + (size_t)iy * Lx);
-    for (int t = 0; t < Lx; t++) {
-      for (int k = 0; k < n2y; k++) colv[k] = t1[(size_t)k * Lx + t];
-      interp1d(colv, n2y, ifc, colo);
-      for (int s = 0; s < Ly; s++) t3[(size_t)s * Lx + t] = colo[s];
-    }
+    xo64_interp2d(md, n2x, n2y, ifc, hres);
     int idx = 0;
     double best = 0.0;
-    for (int i = 0; i < Lx * Ly; i++) {
-      hres[i] = t3[i].re;
+    for (int i = 0; i < Lx * Ly; i++)
       if (hres[i] > best) { best = hres[i]; idx = i; }
-    }
     int hp = idx / Lx, hq = idx % Lx;
     out->xfrac = (float)((hq - Ly / 2.0) / ifc);
     out->yfrac = (float)((hp - Ly / 2.0) / ifc);
     out->hip = hp;
     out->hjp = hq;
-    if (dump && dump->md) for (int i = 0; i < n2x * n2y; i++) dump->md[i] = md[i].re;
+    if (dump && dump->md) memcpy(dump->md, md, sizeof(double) * n2x * n2y);
     if (dump && dump->hres) memcpy(dump->hres, hres, sizeof(double) * Lx * Ly);
-  hdone:
-    free(md); free(t1); free(t3); free(colv); free(colo); free(hres);
+    free(md); free(hres);
   }
   if (dump) {
     if (dump->amp1) memcpy(dump->amp1, amp[0], sizeof(double) * N);
     if (dump->amp2) memcpy(dump->amp2, amp[1], sizeof(double) * N);
     if (dump->mean) { dump->mean[0] = mean[0]; dump->mean[1] = mean[1]; }
-    if (dump->surf) memcpy(dump->surf, surf, sizeof(double) * nxc * nyc);
   }
 done:
   free(amp[0]); free(amp[1]); free(c[0]); free(c[1]); free(iv[0]); free(iv[1]); free(row); free(surf);

@@ -478,11 +478,9 @@ int xo32_patch(const xo_params *p, const c32 *mwin, const c32 *swin, xo_result *
   if (p->interp_flag) {
     const int n2x = p->n2x, n2y = p->n2y, ifc = p->interp_factor;
     const int Lx = n2x * ifc, Ly = n2y * ifc;
-    c32 *md = (c32 *)malloc(sizeof(c32) * n2x * n2y);
-    c32 *rows = (c32 *)malloc(sizeof(c32) * n2y * Lx);
+    float *md = (float *)malloc(sizeof(float) * n2x * n2y);
     float *hres = (float *)malloc(sizeof(float) * Lx * Ly);
-    c32 *X = (c32 *)malloc(sizeof(c32) * (n2x > n2y ? n2x : n2y));
-    if (!md || !rows || !hres || !X) { free(md); free(rows); free(hres); free(X); rc = -2; goto done; }
+    if (!md || !hres) { free(md); free(hres); rc = -2; goto done; }
     for (int iy = 0; iy < n2y; iy++) {
       for (int ix = 0; ix < n2x; ix++) {
         long k = (long)(ip - n2y / 2 + iy) * nxc + (jp - n2x / 2 + ix);
@@ -491,44 +489,10 @@ int xo32_patch(const xo_params *p, const c32 *mwin, const c32 *swin, xo_result *
           float v = surf_is_time ? (float)tsurf[k] : (float)((double)surf[k] / msc);
           q = sqrtf(sqrtf(v));
         }
-        md[iy * n2x + ix].re = q;
-        md[iy * n2x + ix].im = 0.0f;
+        md[iy * n2x + ix] = q;
       }
     }
-    const float sx = 1.0f / (float)n2x, sy = 1.0f / (float)n2y;
-    for (int iy = 0; iy < n2y; iy++) {
-      for (int k = 0; k < n2x; k++) X[k] = md[iy * n2x + k];
-      xo32_fft(X, n2x, -1);
-      for (int t = 0; t < Lx; t++) {
-        c32 acc = {0.0f, 0.0f};
-        for (int k = 0; k < n2x; k++) {
-          long kt = (k < n2x / 2) ? k : k - n2x;
-          long m = (kt * t) % Lx;
-          if (m < 0) m += Lx;
-          c32 e = twf(Lx, m, +1);
-          acc.re = fmaf(X[k].re, e.re, fmaf(-X[k].im, e.im, acc.re));
-          acc.im = fmaf(X[k].re, e.im, fmaf(X[k].im, e.re, acc.im));
-        }
-        acc.re = acc.re * sx;
-        acc.im = acc.im * sx;
-        rows[(size_t)iy * Lx + t] = acc;
-      }
-    }
-    for (int t = 0; t < Lx; t++) {
-      for (int k = 0; k < n2y; k++) X[k] = rows[(size_t)k * Lx + t];
-      xo32_fft(X, n2y, -1);
-      for (int s = 0; s < Ly; s++) {
-        float acc = 0.0f;
-        for (int k = 0; k < n2y; k++) {
-          long kt = (k < n2y / 2) ? k : k - n2y;
-          long m = (kt * s) % Ly;
-          if (m < 0) m += Ly;
-          c32 e = twf(Ly, m, +1);
-          acc = fmaf(X[k].re, e.re, fmaf(-X[k].im, e.im, acc));
-        }
-        hres[(size_t)s * Lx + t] = acc * sy;
-      }
-    }
+    if (xo32_interp2d(md, n2x, n2y, ifc, hres)) { free(md); free(hres); rc = -2; goto done; }
     int idx = 0;
     float best = 0.0f;
     for (int i = 0; i < Lx * Ly; i++)
@@ -539,10 +503,9 @@ int xo32_patch(const xo_params *p, const c32 *mwin, const c32 *swin, xo_result *
     out->yfrac = ((float)hp - half) / (float)ifc;
     out->hip = hp;
     out->hjp = hq;
-    if (dump && dump->md)
-      for (int i = 0; i < n2x * n2y; i++) dump->md[i] = md[i].re;
+    if (dump && dump->md) memcpy(dump->md, md, sizeof(float) * n2x * n2y);
     if (dump && dump->hres) memcpy(dump->hres, hres, sizeof(float) * Lx * Ly);
-    free(md); free(rows); free(hres); free(X);
+    free(md); free(hres);
   }
 
   if (dump) {
@@ -554,6 +517,56 @@ int xo32_patch(const xo_params *p, const c32 *mwin, const c32 *swin, xo_result *
 done:
   free(amp1); free(amp2); free(i1); free(i2); free(z); free(col); free(c3); free(surf); free(tsurf);
   return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 2-D zero-pad interpolation of a real n2y x n2x neighbourhood by ifc      */
+/* (fft_interpolate_routines.cu:76-147: rows, then columns, Nyquist bin at  */
+/* -n/2, each pass * 1/n); returns the real part.  Each 1-D pass is the    */
+/* n-term sum sum_k X[k] e^{2 pi i ktilde t / L} / n, evaluated directly.  */
+/* ------------------------------------------------------------------------ */
+int xo32_interp2d(const float *md, int n2x, int n2y, int ifc, float *hres) {
+  const int Lx = n2x * ifc, Ly = n2y * ifc;
+  c32 *rows = (c32 *)malloc(sizeof(c32) * n2y * Lx);
+  c32 *X = (c32 *)malloc(sizeof(c32) * (n2x > n2y ? n2x : n2y));
+  if (!rows || !X) { free(rows); free(X); return -2; }
+  const float sx = 1.0f / (float)n2x, sy = 1.0f / (float)n2y;
+  for (int iy = 0; iy < n2y; iy++) {
+    for (int k = 0; k < n2x; k++) { X[k].re = md[iy * n2x + k]; X[k].im = 0.0f; }
+    xo32_fft(X, n2x, -1);
+    for (int t = 0; t < Lx; t++) {
+      c32 acc = {0.0f, 0.0f};
+      for (int k = 0; k < n2x; k++) {
+        long kt = (k < n2x / 2) ? k : k - n2x;
+        long m = (kt * t) % Lx;
+        if (m < 0) m += Lx;
+        c32 e = twf(Lx, m, +1);
+        acc.re = fmaf(X[k].re, e.re, fmaf(-X[k].im, e.im, acc.re));
+        acc.im = fmaf(X[k].re, e.im, fmaf(X[k].im, e.re, acc.im));
+      }
+      acc.re = acc.re * sx;
+      acc.im = acc.im * sx;
+      rows[(size_t)iy * Lx + t] = acc;
+    }
+  }
+  for (int t = 0; t < Lx; t++) {
+    for (int k = 0; k < n2y; k++) X[k] = rows[(size_t)k * Lx + t];
+    xo32_fft(X, n2y, -1);
+    for (int s = 0; s < Ly; s++) {
+      float acc = 0.0f;
+      for (int k = 0; k < n2y; k++) {
+        long kt = (k < n2y / 2) ? k : k - n2y;
+        long m = (kt * s) % Ly;
+        if (m < 0) m += Ly;
+        c32 e = twf(Ly, m, +1);
+        acc = fmaf(X[k].re, e.re, fmaf(-X[k].im, e.im, acc));
+      }
+      hres[(size_t)s * Lx + t] = acc * sy;
+    }
+  }
+  free(rows);
+  free(X);
+  return 0;
 }
 
 /* ------------------------------------------------------------------------ */

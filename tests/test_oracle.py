@@ -1,0 +1,281 @@
+"""Pins of the CPU oracle (oracle/): every function is checked against
+something the mathematics fixes -- numpy.fft and a direct DFT, closed forms of
+the zero-pad interpolation, direct (brute-force) correlation sums, int64
+brute force, bandlimited test functions -- chosen so that a dropped term, a
+wrong sign or index, or a transposed operand fails at least one check."""
+import numpy as np
+import pytest
+
+SIZES = [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
+
+
+def _rand(rng, n):
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+# --------------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", SIZES)
+def test_fft_matches_numpy(orc, n):
+    rng = np.random.default_rng(n)
+    x = _rand(rng, n).astype(np.complex64)
+    ref = np.fft.fft(x.astype(np.complex128))
+    refi = np.fft.ifft(x.astype(np.complex128)) * n
+    scale = np.abs(ref).max()
+    assert np.abs(orc.fft32(x, -1) - ref).max() / scale < 3e-7 * np.log2(n) + 1e-7
+    assert np.abs(orc.fft32(x, +1) - refi).max() / scale < 3e-7 * np.log2(n) + 1e-7
+    assert np.abs(orc.fft64(x, -1) - ref).max() / scale < 1e-14
+    assert np.abs(orc.fft64(x, +1) - refi).max() / scale < 1e-14
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32, 64])
+def test_fft_matches_direct_dft(orc, n):
+    rng = np.random.default_rng(100 + n)
+    x = _rand(rng, n)
+    k = np.arange(n)
+    W = np.exp(-2j * np.pi * np.outer(k, k) / n)
+    ref = W @ x
+    assert np.abs(orc.fft64(x, -1) - ref).max() < 1e-12 * np.abs(x).sum()
+    assert np.abs(orc.fft32(x.astype(np.complex64), -1) - ref).max() < 1e-5 * np.abs(x).sum()
+
+
+def test_fft_parseval_and_shift_theorem(orc):
+    rng = np.random.default_rng(7)
+    n = 256
+    x = _rand(rng, n)
+    X = orc.fft64(x, -1)
+    assert abs(np.sum(np.abs(X) ** 2) - n * np.sum(np.abs(x) ** 2)) < 1e-10 * n * np.sum(np.abs(x) ** 2)
+    s = 37
+    Xs = orc.fft64(np.roll(x, s), -1)
+    k = np.arange(n)
+    assert np.abs(Xs - X * np.exp(-2j * np.pi * k * s / n)).max() < 1e-10 * np.abs(X).max()
+    # the FP32 spec FFT inverts its own forward transform
+    y = orc.fft32(orc.fft32(x.astype(np.complex64), -1), +1) / n
+    assert np.abs(y - x).max() < 1e-5 * np.abs(x).max()
+
+
+def test_twiddles_exact_on_axes_and_diagonals(orc):
+    for n in (8, 64, 256, 4096):
+        assert orc.twiddle64(n, 0) == (1.0, 0.0)
+        assert orc.twiddle64(n, n // 4) == (0.0, 1.0)
+        assert orc.twiddle64(n, n // 2) == (-1.0, 0.0)
+        assert orc.twiddle64(n, 3 * n // 4) == (0.0, -1.0)
+        c, s = orc.twiddle64(n, n // 8)
+        assert c == s
+        pi = np.longdouble("3.14159265358979323846264338327950288")
+        for m in range(0, n, max(1, n // 64)):
+            c, s = orc.twiddle64(n, m)
+            t = 2 * pi * np.longdouble(m) / np.longdouble(n)        # extended-precision reference
+            assert abs(np.longdouble(c) - np.cos(t)) < 2.3e-16 and abs(np.longdouble(s) - np.sin(t)) < 2.3e-16
+
+
+# ------------------------------------------------------------ range interpolation
+@pytest.mark.parametrize("ri", [2, 4])
+def test_range_interp_reproduces_original_samples(orc, ri):
+    """interp[ri*q] == x[q]: output j = ri*q - N/2 (fft_interpolate_routines.cu:76-113,
+    extraction xcorr.cu:245)."""
+    rng = np.random.default_rng(3)
+    n = 256
+    x = (_rand(rng, n) * 300).astype(np.complex64)
+    a64 = orc.range_interp64(x, ri)
+    a32 = orc.range_interp32(x, ri)
+    for j in range(n):
+        t = n // 2 + j
+        if t % ri == 0:
+            assert abs(a64[j] - abs(complex(x[t // ri]))) < 1e-9 * np.abs(x).max()
+            assert a32[j] == np.float32(np.sqrt(np.float32(x[t // ri].real) ** 2 + np.float32(x[t // ri].imag) ** 2)) \
+                or abs(a32[j] - abs(x[t // ri])) < 1e-6 * abs(x[t // ri])
+    assert np.abs(a32 - a64).max() < 2e-6 * a64.max()
+
+
+@pytest.mark.parametrize("f", [1, 5, 17, 100, -3, -127])
+def test_range_interp_exponential_has_unit_modulus(orc, f):
+    """A sampled e^{2 pi i f n/N} with |f| < N/2 interpolates to itself: |.| == 1."""
+    n = 256
+    x = np.exp(2j * np.pi * f * np.arange(n) / n) * 50
+    for ri in (2, 4, 8):
+        assert np.abs(orc.range_interp64(x, ri) - 50).max() < 1e-9
+        assert np.abs(orc.range_interp32(x.astype(np.complex64), ri) - 50).max() < 2e-4
+
+
+def test_range_interp_nyquist_bin_not_split(orc):
+    """(-1)^n puts all energy in bin N/2, which the reference places at M - N/2
+    (fft_interpolate_routines.cu:96-97): the interpolant is e^{-i pi t/ri}, |.| == 1
+    everywhere (a split Nyquist bin would give |cos(pi t/2)|, i.e. 0 at odd t)."""
+    n = 64
+    x = ((-1.0) ** np.arange(n) * 10).astype(np.complex128)
+    assert np.abs(orc.range_interp64(x, 2) - 10).max() < 1e-9
+    assert np.abs(orc.range_interp32(x.astype(np.complex64), 2) - 10).max() < 1e-4
+
+
+def test_range_interp_impulse_index_map(orc):
+    n = 128
+    for ri in (2, 4):
+        for q0 in range(n // (2 * ri) + 1, 3 * n // (2 * ri) - 1, 7):
+            x = np.zeros(n, np.complex64)
+            x[q0] = 1.0
+            a = orc.range_interp64(x, ri)
+            assert int(np.argmax(a)) == ri * q0 - n // 2
+            assert abs(a.max() - 1.0) < 1e-12
+
+
+# ---------------------------------------------------------------- whole patches
+def _windows(synth, xs, ys, dx=2.6, dy=-4.3, coh=0.8, seed=5):
+    npx, npy = 4 * xs, 4 * ys
+    m, s = synth.constant_shift_pair(npx + 64, npy + 64, dx, dy, coh, seed=seed, noise_seed=seed + 1)
+    c = lambda a: (a[32:32 + npy, 32:32 + npx, 0].astype(np.float32)
+                   + 1j * a[32:32 + npy, 32:32 + npx, 1].astype(np.float32))
+    return c(m), c(s)
+
+
+def test_surface_matches_direct_correlation(orc, synth):
+    """|c3| on the lag window equals |sum_b c1[b+tau] c2[b]| computed directly
+    (do_freq_xcorr.cu:46-84: conj product, checkerboard shift, extraction)."""
+    xs = ys = 8
+    p = orc.make_params(xs, ys)
+    mw, sw = _windows(synth, xs, ys)
+    rec, d = orc.patch(p, mw, sw, precision=64, dump=True)
+    c1 = d["amp1"] - d["mean"][0]
+    c2 = d["amp2"] - d["mean"][1]
+    mask = np.zeros_like(c2)
+    mask[ys:3 * ys, xs:3 * xs] = 1
+    c2 = c2 * mask
+    brute = np.zeros((2 * ys, 2 * xs))
+    tmpl = c2[ys:3 * ys, xs:3 * xs]
+    for iy in range(2 * ys):
+        for ix in range(2 * xs):
+            ty, tx = iy - ys, ix - xs
+            brute[iy, ix] = abs(np.sum(c1[ys + ty:3 * ys + ty, xs + tx:3 * xs + tx] * tmpl))
+    np.testing.assert_allclose(d["surf"], brute, rtol=1e-9, atol=1e-9 * brute.max())
+    assert (rec["ip"], rec["jp"]) == np.unravel_index(np.argmax(brute), brute.shape)
+    # the FP32 spec path is within FP32 accuracy of it
+    r32, d32 = orc.patch(p, mw, sw, precision=32, dump=True)
+    assert np.abs(d32["surf"] - brute).max() < 1e-5 * brute.max()
+
+
+def test_ncc_at_peak_matches_int64_bruteforce(orc, synth):
+    xs = ys = 16
+    p = orc.make_params(xs, ys)
+    mw, sw = _windows(synth, xs, ys, seed=9)
+    rec, d = orc.patch(p, mw, sw, precision=32, dump=True)
+    m1f, m2f = np.float32(d["mean"][0]), np.float32(d["mean"][1])
+    i1 = (d["amp1"] - m1f).astype(np.int64)
+    i2 = (d["amp2"] - m2f).astype(np.int64)
+    ip, jp = int(rec["ip"]) - 2 * ys // 2, int(rec["jp"]) - 2 * xs // 2
+    a = i1[ys + ip:3 * ys + ip, xs + jp:3 * xs + jp]
+    b = i2[ys:3 * ys, xs:3 * xs]
+    num, d1, d2 = int((a * b).sum()), int((a * a).sum()), int((b * b).sum())
+    gamma = 100.0 * abs(num / np.sqrt(float(d1) * float(d2)))
+    assert rec["corr"] == np.float32(gamma)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_time_search_matches_int64_bruteforce(orc, synth, precision):
+    xs = ys = 4
+    mw, sw = _windows(synth, xs, ys, seed=13)
+    for flag in (0, 1):
+        p = orc.make_params(xs, ys, corr_flag=flag)
+        rec, d = orc.patch(p, mw, sw, precision=precision, dump=True)
+        if precision == 32:
+            m1f, m2f = np.float32(d["mean"][0]), np.float32(d["mean"][1])
+            i1 = [[int(v) for v in row] for row in (d["amp1"] - m1f).astype(np.int64)]
+            i2 = [[int(v) for v in row] for row in (d["amp2"] - m2f).astype(np.int64)]
+        else:
+            i1 = [[int(v) for v in row] for row in (d["amp1"] - d["mean"][0]).astype(np.int64)]
+            i2 = [[int(v) for v in row] for row in (d["amp2"] - d["mean"][1]).astype(np.int64)]
+        surf = np.zeros((2 * ys, 2 * xs))
+        for ic in range(2 * ys):
+            for jc in range(2 * xs):
+                num = d1 = d2 = 0
+                for r in range(2 * ys):
+                    for c in range(2 * xs):
+                        a, b = i1[ic + r][jc + c], i2[ys + r][xs + c]
+                        if flag == 0:
+                            num += a * b; d1 += a * a; d2 += b * b
+                        else:
+                            num += a * a * b * b; d1 += a ** 4; d2 += b ** 4
+                den = np.sqrt(float(d1) * float(d2))
+                g = 0.0 if den == 0 else abs(float(num) / den)
+                surf[ic, jc] = 100.0 * g if flag == 0 else (0.0 if g <= 0.5 else 100.0 * np.sqrt(2 * g - 1))
+        np.testing.assert_allclose(d["surf"], surf, rtol=1e-12 if precision == 64 else 1e-6)
+        # do_time_corr scan: first maximum in row-major order (float-rounded running max)
+        ic, jc = np.unravel_index(np.argmax(surf), surf.shape)
+        assert (rec["ip"], rec["jp"]) == (ic, jc)
+        assert rec["xoff"] == -(jc - xs) and rec["yoff"] == -(ic - ys)
+
+
+# ------------------------------------------------------------------- high-res
+@pytest.mark.parametrize("ifc", [2, 8, 16])
+def test_interp2d_reproduces_samples(orc, ifc):
+    rng = np.random.default_rng(ifc)
+    md = rng.random((8, 8)) + 0.5
+    for prec, tol in ((64, 1e-12), (32, 2e-6)):
+        h = orc.interp2d(md, ifc, prec)
+        np.testing.assert_allclose(h[::ifc, ::ifc], md, rtol=tol, atol=tol)
+
+
+def test_interp2d_bandlimited_function(orc):
+    """Frequencies below the Nyquist band interpolate exactly at fractional
+    positions; this fixes the sign and the k -> k - n map of the exponentials."""
+    ifc = 16
+    g = np.arange(8)
+    f = lambda y, x: 1.5 + np.cos(2 * np.pi * (x + 0.3) / 8) * np.cos(2 * np.pi * 3 * (y - 0.2) / 8) \
+        + 0.4 * np.sin(2 * np.pi * (2 * x + y) / 8)
+    md = f(g[:, None], g[None, :])
+    t = np.arange(8 * ifc) / ifc
+    exact = f(t[:, None], t[None, :])
+    assert np.abs(orc.interp2d(md, ifc, 64) - exact).max() < 1e-12
+    assert np.abs(orc.interp2d(md, ifc, 32) - exact).max() < 1e-5
+
+
+def test_fp32_path_agrees_with_fp64(orc, synth):
+    for seed, (xs, ys) in enumerate([(16, 16), (32, 16), (16, 32), (32, 32)]):
+        p = orc.make_params(xs, ys)
+        mw, sw = _windows(synth, xs, ys, seed=20 + seed)
+        r32, _ = orc.patch(p, mw, sw, precision=32)
+        r64, _ = orc.patch(p, mw, sw, precision=64)
+        assert (r32["ip"], r32["jp"]) == (r64["ip"], r64["jp"])
+        assert abs(r32["corr"] - r64["corr"]) < 0.05
+        assert abs(r32["xfrac"] - r64["xfrac"]) <= 1 / 16 + 1e-6
+        assert abs(r32["yfrac"] - r64["yfrac"]) <= 1 / 16 + 1e-6
+
+
+def test_integer_shift_recovered_exactly(orc, synth):
+    """Aligned = master shifted by whole pixels (no noise): the integer lag is the
+    shift (range lag in 1/ri pixels) and gamma is ~100."""
+    xs = ys = 16
+    npx, npy = 4 * xs, 4 * ys
+    rng = np.random.default_rng(4)
+    big = (rng.standard_normal((npy + 40, npx + 40)) + 1j * rng.standard_normal((npy + 40, npx + 40))) * 300
+    D = (3, -5)   # aligned(x) = master(x - D)
+    mw = big[20:20 + npy, 20:20 + npx]
+    sw = big[20 - D[1]:20 - D[1] + npy, 20 - D[0]:20 - D[0] + npx]
+    p = orc.make_params(xs, ys)
+    for prec in (32, 64):
+        rec, _ = orc.patch(p, mw.astype(np.complex64), sw.astype(np.complex64), precision=prec)
+        assert rec["xoff"] == -2 * D[0] * -1 and rec["yoff"] == D[1]
+        assert rec["corr"] > 99.0
+        dx = rec["xoff"] / 2 - rec["xfrac"] / 2
+        dy = rec["yoff"] - rec["yfrac"]
+        assert abs(dx - D[0]) <= 1 / 32 + 1e-6 and abs(dy - D[1]) <= 1 / 16 + 1e-6
+
+
+def test_fourier_shift_recovered(orc, synth):
+    xs = ys = 32
+    p = orc.make_params(xs, ys)
+    dxs, dys = [], []
+    for seed in range(6):
+        mw, sw = _windows(synth, xs, ys, dx=2.6, dy=-4.3, coh=0.9, seed=40 + seed)
+        rec, _ = orc.patch(p, mw, sw, precision=64)
+        dxs.append(rec["xoff"] / 2 - rec["xfrac"] / 2)
+        dys.append(rec["yoff"] - rec["yfrac"])
+    # the corr^(1/4) sub-pixel estimator pulls toward whole lags (peak locking),
+    # strongest in azimuth where there is no range oversampling
+    assert abs(np.median(dxs) - 2.6) < 0.1 and abs(np.median(dys) + 4.3) < 0.3
+
+
+def test_zero_windows_are_defined(orc):
+    p = orc.make_params(8, 8)
+    z = np.zeros((32, 32), np.complex64)
+    for prec in (32, 64):
+        rec, _ = orc.patch(p, z, z, precision=prec)
+        assert rec["corr"] == 0.0 and rec["ip"] == 0 and rec["jp"] == 0
