@@ -13,6 +13,7 @@
 //   k_finish         stage (c)+(d) NCC at the peak, high-res refinement, result
 #include "../../include/xcorr_hip.h"
 #include "fft_spec.hpp"
+#include "fft256.hpp"
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -201,6 +202,338 @@ __global__ void __launch_bounds__(kThreads) k_range_ampl(DevParams P, const void
 }
 
 // --------------------------------------------------------------------------
+// 256-sample fast paths (npx = 256 for the row kernels, npy = 256 for the
+// column kernels).  Same arithmetic as the generic kernels, register-resident
+// transforms (fft256.hpp); one block per patch (or per 16-column tile) so the
+// per-lane twiddles and phase factors are loaded once.
+// --------------------------------------------------------------------------
+
+// Stage (a) for npx = 256, ri = 2: one block per (patch, image), 16 groups of
+// 16 lanes, one row per group and iteration.  Output columns 2q-128, 2q-127
+// are the original sample q and the half-sample interpolant at q + 1/2.
+__global__ void __launch_bounds__(kThreads) k_range256(DevParams P, const void* __restrict__ master,
+                                                       const void* __restrict__ slave,
+                                                       const xcorr_hip_window* __restrict__ win,
+                                                       float* __restrict__ amp,
+                                                       unsigned long long* __restrict__ sums,
+                                                       const float2* __restrict__ tabs) {
+  __shared__ float2 tiles[16 * kT256Tile];
+  __shared__ long long red[kThreads / 64];
+  const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+  const int img = blockIdx.x & 1, b = blockIdx.x >> 1;
+  const xcorr_hip_window w = win[b];
+  const int x0 = img ? w.sx0 : w.mx0, y0 = img ? w.sy0 : w.my0;
+  const int inx = img ? P.s_nx : P.m_nx, iny = img ? P.s_ny : P.m_ny;
+  const void* src = img ? slave : master;
+  float2* tile = tiles + g * kT256Tile;
+  float2 t[15];
+  load_tw256(tabs + P.tw_nx, j, t);
+  float2 ph[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++) ph[r] = tabs[P.phi + j + 16 * r];
+  const bool inside = x0 >= 0 && x0 + 256 <= inx && y0 >= 0 && y0 + P.NY <= iny;
+  float2* outb = (float2*)(amp + (size_t)(2 * b + img) * P.N);
+  long long s = 0;
+  for (int row = g; row < P.NY; row += 16) {
+    const int gy = y0 + row;
+    float2 v[16];
+    if (P.format == XCORR_HIP_CINT16) {
+      const unsigned* im = (const unsigned*)src;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int gx = x0 + j + 16 * r;
+        unsigned u = 0u;
+        if (inside || (gy >= 0 && gy < iny && gx >= 0 && gx < inx)) u = im[(size_t)gy * inx + gx];
+        v[r] = make_float2((float)(short)(u & 0xffffu), (float)(short)(u >> 16));
+      }
+    } else {
+      const float* im = (const float*)src;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int gx = x0 + j + 16 * r;
+        float u = 0.0f;
+        if (inside || (gy >= 0 && gy < iny && gx >= 0 && gx < inx)) u = im[(size_t)gy * inx + gx];
+        v[r] = make_float2(u, 0.0f);
+      }
+    }
+    float ea[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) ea[q] = ampl(v[q + 4]);
+    fft256_group<-1>(v, j, tile, t);
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = cmul(v[r], ph[r]);
+    fft256_group<+1>(v, j, tile, t);
+    const float sc = 1.0f / 256.0f;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      float2 o = v[q + 4];
+      o.x = o.x * sc;
+      o.y = o.y * sc;
+      const float oa = ampl(o);
+      outb[(size_t)row * 128 + j + 16 * q] = make_float2(ea[q], oa);
+      s += (long long)(ea[q] * 1048576.0f) + (long long)(oa * 1048576.0f);
+    }
+  }
+  s = block_sum_i64(s, red);
+  if (threadIdx.x == 0) sums[2 * b + img] = (unsigned long long)s;
+}
+
+// Stage (b) rows for npx = 256: one block per patch.
+__global__ void __launch_bounds__(kThreads) k_rowfft256(DevParams P, const float* __restrict__ amp,
+                                                        const unsigned long long* __restrict__ sums,
+                                                        float2* __restrict__ Z, const float2* __restrict__ tabs) {
+  __shared__ float2 tiles[16 * kT256Tile];
+  const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+  const int b = blockIdx.x;
+  const float m1f = (float)patch_mean(sums[2 * b], P.N);
+  const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
+  const float* a1 = amp + (size_t)(2 * b) * P.N;
+  const float* a2 = a1 + P.N;
+  float2* zb = Z + (size_t)b * P.N;
+  float2* tile = tiles + g * kT256Tile;
+  float2 t[15];
+  load_tw256(tabs + P.tw_nx, j, t);
+  for (int row = g; row < P.NY; row += 16) {
+    const bool in_y = row >= P.ys && row < P.NY - P.ys;
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int x = j + 16 * r;
+      const float c1 = a1[row * 256 + x] - m1f;
+      float c2 = a2[row * 256 + x] - m2f;
+      if (!(in_y && x >= P.xs && x < 256 - P.xs)) c2 = 0.0f;
+      v[r] = make_float2(c1, c2);
+    }
+    fft256_group<-1>(v, j, tile, t);
+#pragma unroll
+    for (int r = 0; r < 16; r++) zb[(size_t)row * 256 + j + 16 * r] = v[r];
+  }
+}
+
+// Stage (b) columns for npy = 256: FFT along y of a 16-column tile, in place.
+// The tile is staged through LDS (coalesced 128-byte rows) and reused as the
+// groups' transpose scratch.
+__global__ void __launch_bounds__(kThreads) k_colfft256(DevParams P, float2* __restrict__ Z,
+                                                        const float2* __restrict__ tabs) {
+  __shared__ float2 T[256 * kT256Pitch];
+  const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+  const int NX = P.NX;
+  const int tiles = NX >> 4;
+  const int ct = blockIdx.x % tiles, b = blockIdx.x / tiles;
+  float2* zb = Z + (size_t)b * P.N + ct * 16;
+  float2 t[15];
+  load_tw256(tabs + P.tw_ny, j, t);
+  for (int e = threadIdx.x; e < 256 * 16; e += kThreads) T[(e >> 4) * kT256Pitch + (e & 15)] = zb[(size_t)(e >> 4) * NX + (e & 15)];
+  __syncthreads();
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++) v[r] = T[(j + 16 * r) * kT256Pitch + g];
+  __syncthreads();
+  fft256_group<-1>(v, j, T + g * kT256Tile, t);
+#pragma unroll
+  for (int r = 0; r < 16; r++) T[(j + 16 * r) * kT256Pitch + g] = v[r];
+  __syncthreads();
+  for (int e = threadIdx.x; e < 256 * 16; e += kThreads) zb[(size_t)(e >> 4) * NX + (e & 15)] = T[(e >> 4) * kT256Pitch + (e & 15)];
+}
+
+// Stage (b) product + column IFFT for npy = 256 (ny_corr/2 = 64, nyc = 128).
+__global__ void __launch_bounds__(kThreads) k_product_colifft256(DevParams P, const float2* __restrict__ Z,
+                                                                 float2* __restrict__ D,
+                                                                 const float2* __restrict__ tabs) {
+  __shared__ float2 T[256 * kT256Pitch];
+  const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+  const int NX = P.NX;
+  const int tiles = NX >> 4;
+  const int ct = blockIdx.x % tiles, b = blockIdx.x / tiles;
+  const float2* zb = Z + (size_t)b * P.N;
+  float2 t[15];
+  load_tw256(tabs + P.tw_ny, j, t);
+  // this block's columns
+  for (int e = threadIdx.x; e < 256 * 16; e += kThreads)
+    T[(e >> 4) * kT256Pitch + (e & 15)] = zb[(size_t)(e >> 4) * NX + ct * 16 + (e & 15)];
+  __syncthreads();
+  const int kx = ct * 16 + g;
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++) v[r] = T[(j + 16 * r) * kT256Pitch + g];
+  __syncthreads();
+  // mirror columns (NX - kx) mod NX of the tile: staged in tile order c' = g
+  for (int e = threadIdx.x; e < 256 * 16; e += kThreads) {
+    const int c = e & 15;
+    const int mx = (NX - (ct * 16 + c)) & (NX - 1);
+    T[(e >> 4) * kT256Pitch + c] = zb[(size_t)(e >> 4) * NX + mx];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int ky = j + 16 * r;
+    const float2 a = v[r];
+    const float2 zm = T[((256 - ky) & 255) * kT256Pitch + g];
+    const float2 bb = make_float2(zm.x, -zm.y);
+    const float2 c1 = make_float2((a.x + bb.x) * 0.5f, (a.y + bb.y) * 0.5f);
+    const float2 d = csub(a, bb);
+    const float2 c2c = make_float2(0.5f * d.y, 0.5f * d.x);
+    float2 p = cmul(c1, c2c);
+    if ((kx + ky) & 1) { p.x = -p.x; p.y = -p.y; }
+    v[r] = p;
+  }
+  __syncthreads();
+  fft256_group<+1>(v, j, T + g * kT256Tile, t);
+  // keep rows 64..191 (ny_corr/2 + [0, nyc)): r = 4..11
+#pragma unroll
+  for (int q = 0; q < 8; q++) T[(j + 16 * q) * kT256Pitch + g] = v[q + 4];
+  __syncthreads();
+  float2* db = D + (size_t)b * 128 * NX + ct * 16;
+  for (int e = threadIdx.x; e < 128 * 16; e += kThreads) db[(size_t)(e >> 4) * NX + (e & 15)] = T[(e >> 4) * kT256Pitch + (e & 15)];
+}
+
+// C3 at one bin from A = Z[ky][kx] and Zm = Z[-ky][-kx] (spec of k_product_colifft).
+__device__ __forceinline__ float2 c3_bin(float2 a, float2 zm, int kx, int ky) {
+  const float2 bb = make_float2(zm.x, -zm.y);
+  const float2 c1 = make_float2((a.x + bb.x) * 0.5f, (a.y + bb.y) * 0.5f);
+  const float2 d = csub(a, bb);
+  const float2 c2c = make_float2(0.5f * d.y, 0.5f * d.x);
+  float2 p = cmul(c1, c2c);
+  if ((kx + ky) & 1) { p.x = -p.x; p.y = -p.y; }
+  return p;
+}
+
+// Stage (b) columns, fused, for npy = 256 (ny_corr/2 = 64, nyc = 128): FFT along y,
+// C3 = C1 conj(C2) (-1)^(kx+ky), IFFT along y, keep lag rows 64..191.  Block p
+// owns columns T = [16p, 16p+16) and their mirrors M = {npx - 16p - g} (with
+// npx/2 in place of the duplicate column 0 for p = 0), so every column is
+// transformed once and C3 needs no second pass over Z.  Requires npx >= 32.
+__global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const float2* __restrict__ Z,
+                                                         float2* __restrict__ D, const float2* __restrict__ tabs) {
+  __shared__ float2 T[256 * kT256Pitch];
+  __shared__ float2 sx[256];                                 // spectrum of a self-mirrored column
+  const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+  const int NX = P.NX;
+  const int pairs = NX >> 5;
+  const int p = blockIdx.x % pairs, b = blockIdx.x / pairs;
+  const float2* zb = Z + (size_t)b * P.N;
+  const int kxT = 16 * p + g;
+  const int kxM = (p == 0 && g == 0) ? NX / 2 : NX - 16 * p - g;
+  const bool self = (p == 0 && g == 0);                      // columns 0 and npx/2 mirror onto themselves
+  float2 t[15];
+  load_tw256(tabs + P.tw_ny, j, t);
+  float2 vT[16], vM[16];
+  // stage and load the T columns, then the M columns
+  for (int e = threadIdx.x; e < 256 * 16; e += kThreads)
+    T[(e >> 4) * kT256Pitch + (e & 15)] = zb[(size_t)(e >> 4) * NX + 16 * p + (e & 15)];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; r++) vT[r] = T[(j + 16 * r) * kT256Pitch + g];
+  __syncthreads();
+  for (int e = threadIdx.x; e < 256 * 16; e += kThreads) {
+    const int c = e & 15;
+    const int mx = (p == 0 && c == 0) ? NX / 2 : NX - 16 * p - c;
+    T[(e >> 4) * kT256Pitch + c] = zb[(size_t)(e >> 4) * NX + mx];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; r++) vM[r] = T[(j + 16 * r) * kT256Pitch + g];
+  __syncthreads();
+  fft256_group<-1>(vT, j, T + g * kT256Tile, t);
+  fft256_group<-1>(vM, j, T + g * kT256Tile, t);
+  // T products need the M spectra at the mirrored rows
+#pragma unroll
+  for (int r = 0; r < 16; r++) T[(j + 16 * r) * kT256Pitch + g] = vM[r];
+  if (self)
+#pragma unroll
+    for (int r = 0; r < 16; r++) sx[j + 16 * r] = vT[r];
+  __syncthreads();
+  float2 pT[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int ky = j + 16 * r;
+    const int my = (256 - ky) & 255;
+    const float2 zm = self ? sx[my] : T[my * kT256Pitch + g];
+    pT[r] = c3_bin(vT[r], zm, kxT, ky);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; r++) T[(j + 16 * r) * kT256Pitch + g] = vT[r];
+  if (self)
+#pragma unroll
+    for (int r = 0; r < 16; r++) sx[j + 16 * r] = vM[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int ky = j + 16 * r;
+    const int my = (256 - ky) & 255;
+    const float2 zm = self ? sx[my] : T[my * kT256Pitch + g];
+    vM[r] = c3_bin(vM[r], zm, kxM, ky);
+  }
+  __syncthreads();
+  fft256_group<+1>(pT, j, T + g * kT256Tile, t);
+  fft256_group<+1>(vM, j, T + g * kT256Tile, t);
+  // lag rows 64..191 (r = 4..11) of both columns: staged as [row][32 columns]
+  constexpr int kP = 33;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    T[(j + 16 * q) * kP + g] = pT[q + 4];
+    T[(j + 16 * q) * kP + 16 + g] = vM[q + 4];
+  }
+  __syncthreads();
+  float2* db = D + (size_t)b * 128 * NX;
+  for (int e = threadIdx.x; e < 128 * 32; e += kThreads) {
+    const int row = e >> 5, c = e & 31;
+    const int kx = c < 16 ? 16 * p + c : ((p == 0 && c == 16) ? NX / 2 : NX - 16 * p - (c - 16));
+    db[(size_t)row * NX + kx] = T[row * kP + c];
+  }
+}
+
+// Stage (b) row IFFT + surface + argmax for npx = 256 (nx_corr/2 = 64, nxc = 128):
+// one block per patch, so the peak key and the sum are written directly.
+__global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const float2* __restrict__ D,
+                                                              float* __restrict__ surf,
+                                                              unsigned long long* __restrict__ peak,
+                                                              double* __restrict__ partial,
+                                                              const float2* __restrict__ tabs) {
+  __shared__ float2 tiles[16 * kT256Tile];
+  __shared__ unsigned long long redk[kThreads / 64];
+  __shared__ double redd[kThreads / 64];
+  const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+  const int b = blockIdx.x;
+  const float2* db = D + (size_t)b * P.nyc * 256;
+  float* sb = surf + (size_t)b * P.nyc * 128;
+  float2* tile = tiles + g * kT256Tile;
+  float2 t[15];
+  load_tw256(tabs + P.tw_nx, j, t);
+  const float sc = 1.0f / (float)P.N;
+  unsigned long long best = 0ull;
+  double sum = 0.0;
+  const int iters = (P.nyc + 15) >> 4;          // uniform trip count: the transform has block barriers
+  for (int it = 0; it < iters; it++) {
+    const int row = it * 16 + g;
+    const bool active = row < P.nyc;
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = active ? db[(size_t)row * 256 + j + 16 * r] : make_float2(0.0f, 0.0f);
+    fft256_group<+1>(v, j, tile, t);
+    if (!active) continue;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const float re = v[q + 4].x * sc, im = v[q + 4].y * sc;
+      const double dr = re, di = im;
+      const float cv = (float)sqrt(dr * dr + di * di);
+      const unsigned idx = (unsigned)(row * 128 + j + 16 * q);
+      sb[idx] = cv;
+      const unsigned long long k = peak_key(cv, idx);
+      best = k > best ? k : best;
+      sum += (double)cv;
+    }
+  }
+  best = block_max_u64(best, redk);
+  sum = block_sum_f64(sum, redd);
+  if (threadIdx.x == 0) {
+    peak[b] = best;
+    partial[b] = sum;
+  }
+}
+
+// --------------------------------------------------------------------------
 // stage (b): demean, mask, pack z = c1 + i c2, FFT along x
 // conv_ampl_demean.cu:111-139, xcorr.cu:385-401, do_freq_xcorr.cu:90-101
 // --------------------------------------------------------------------------
@@ -358,8 +691,9 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
                                                      xcorr_hip_result* __restrict__ out,
                                                      const float2* __restrict__ tabs) {
   extern __shared__ float2 smem[];
-  __shared__ long long redl[kThreads / 64];
+  __shared__ double redd[kThreads / 64];
   __shared__ unsigned long long redk[kThreads / 64];
+  __shared__ float2 sE[512];                 // e^{+2 pi i m/Lx} (m < Lx), then e^{+2 pi i m/Ly}
   __shared__ float md[64];
   __shared__ float2 Xr[64];
   __shared__ double s_msc;
@@ -384,21 +718,25 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
     const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
     const float* a1 = amp + (size_t)(2 * b) * P.N;
     const float* a2 = a1 + P.N;
-    long long num = 0, d1 = 0, d2 = 0;
-    const int nt = P.nx_corr * P.ny_corr;
+    // exact integer sums in FP64: |products| < 2^35 and every partial sum < 2^53,
+    // so the accumulation order cannot change them
+    double num = 0.0, d1 = 0.0, d2 = 0.0;
+    const int ncx = P.nx_corr, nt = P.nx_corr * P.ny_corr;
+    const float* arow = a1 + (size_t)(P.ys + ipeak) * NX + P.xs + jpeak;
+    const float* brow = a2 + (size_t)P.ys * NX + P.xs;
     for (int e = threadIdx.x; e < nt; e += kThreads) {
-      const int r = e / P.nx_corr, c = e - r * P.nx_corr;
-      const long long av = (int)(a1[(size_t)(P.ys + ipeak + r) * NX + P.xs + jpeak + c] - m1f);
-      const long long bv = (int)(a2[(size_t)(P.ys + r) * NX + P.xs + c] - m2f);
-      num += av * bv;
-      d1 += av * av;
-      d2 += bv * bv;
+      const int r = e / ncx, c = e - r * ncx;
+      const double av = (double)(int)(arow[(size_t)r * NX + c] - m1f);
+      const double bv = (double)(int)(brow[(size_t)r * NX + c] - m2f);
+      num = __builtin_fma(av, bv, num);
+      d1 = __builtin_fma(av, av, d1);
+      d2 = __builtin_fma(bv, bv, d2);
     }
-    num = block_sum_i64(num, redl);
-    d1 = block_sum_i64(d1, redl);
-    d2 = block_sum_i64(d2, redl);
-    const double den = sqrt(1.0 * (double)d1 * (double)d2);
-    const double gamma = (den == 0.0) ? 0.0 : 100.0 * fabs((double)num / den);
+    num = block_sum_f64(num, redd);
+    d1 = block_sum_f64(d1, redd);
+    d2 = block_sum_f64(d2, redd);
+    const double den = sqrt(1.0 * d1 * d2);
+    const double gamma = (den == 0.0) ? 0.0 : 100.0 * fabs(num / den);
     if (threadIdx.x == 0) s_msc = (double)cmax / gamma;
     res.xoff = (float)(-jpeak);
     res.yoff = (float)(-ipeak);
@@ -446,8 +784,11 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
       for (int k = 0; k < 8; k++) Xr[threadIdx.x * 8 + k] = v[k];
     }
     __syncthreads();
-    const float2* twx = tabs + P.tw_lx;
-    const float2* twy = tabs + P.tw_ly;
+    const float2* sEx = sE;
+    const float2* sEy = sE + Lx;
+    for (int m = threadIdx.x; m < Lx; m += kThreads) sE[m] = tabs[P.tw_lx + m];
+    for (int m = threadIdx.x; m < Ly; m += kThreads) sE[Lx + m] = tabs[P.tw_ly + m];
+    __syncthreads();
     const float sx = 1.0f / 8.0f, sy = 1.0f / 8.0f;
     for (int e = threadIdx.x; e < 8 * Lx; e += kThreads) {
       const int iy = e / Lx, t = e - iy * Lx;
@@ -455,9 +796,8 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
 #pragma unroll
       for (int k = 0; k < 8; k++) {
         const int kt = k < 4 ? k : k - 8;
-        int m = (kt * t) % Lx;
-        if (m < 0) m += Lx;
-        const float2 ex = twx[m];
+        const int m = (kt * t) & (Lx - 1);
+        const float2 ex = sEx[m];
         const float2 X = Xr[iy * 8 + k];
         acc.x = __builtin_fmaf(X.x, ex.x, __builtin_fmaf(-X.y, ex.y, acc.x));
         acc.y = __builtin_fmaf(X.x, ex.y, __builtin_fmaf(X.y, ex.x, acc.y));
@@ -474,21 +814,28 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
       for (int k = 0; k < 8; k++) Xc[t * 8 + k] = v[k];
     }
     __syncthreads();
+    // each thread owns one column t (its 8 spectrum values in registers) and
+    // walks the rows s; lanes of a wave share s, so the sEy reads broadcast
     unsigned long long best = 0ull;
-    for (int e = threadIdx.x; e < Lx * Ly; e += kThreads) {
-      const int s = e / Lx, t = e - s * Lx;
-      float acc = 0.0f;
+    const int per = kThreads >= Lx ? kThreads / Lx : 1;
+    for (int t0 = 0; t0 < Lx; t0 += kThreads) {
+      const int t = t0 + (per > 1 ? (int)threadIdx.x % Lx : (int)threadIdx.x);
+      const int s0 = per > 1 ? (int)threadIdx.x / Lx : 0;
+      if (t >= Lx) continue;
+      float2 X[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const int kt = k < 4 ? k : k - 8;
-        int m = (kt * s) % Ly;
-        if (m < 0) m += Ly;
-        const float2 ey = twy[m];
-        const float2 X = Xc[t * 8 + k];
-        acc = __builtin_fmaf(X.x, ey.x, __builtin_fmaf(-X.y, ey.y, acc));
+      for (int k = 0; k < 8; k++) X[k] = Xc[t * 8 + k];
+      for (int s = s0; s < Ly; s += per) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const int kt = k < 4 ? k : k - 8;
+          const float2 ey = sEy[(kt * s) & (Ly - 1)];
+          acc = __builtin_fmaf(X[k].x, ey.x, __builtin_fmaf(-X[k].y, ey.y, acc));
+        }
+        const unsigned long long kk = peak_key(acc * sy, (unsigned)(s * Lx + t));
+        best = kk > best ? kk : best;
       }
-      const unsigned long long kk = peak_key(acc * sy, (unsigned)e);
-      best = kk > best ? kk : best;
     }
     best = block_max_u64(best, redk);
     float bv;
@@ -863,7 +1210,7 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   d.G1 = std::min(d.NY, std::max(1, kTile / d.NX));
   d.G4 = std::min(d.nyc, std::max(1, kTile / d.NX));
   d.C3 = std::min(d.NX, std::max(1, kTile / d.NY));
-  d.groups4 = d.nyc / d.G4;
+  d.groups4 = (d.NX == 256) ? 1 : d.nyc / d.G4;   // the 256-sample row path sums each surface in one block
   // time search: LR surface rows x nxc lags per block (<= 2 lags per thread),
   // template rows staged RC at a time within 64 KB of LDS
   d.tLR = std::min(d.nyc, std::max(1, kThreads / d.nxc));
@@ -1060,33 +1407,72 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
   if (rc0) return rc0;
   const size_t nel1 = (size_t)d.G1 * d.NX;
   const size_t sh1 = nel1 * sizeof(float2) * (d.ri > 2 ? 3 : 2) + nel1 * sizeof(float);
-  int rc = launch_timed(c, timing, "k_range_ampl", [&] {
-    hipLaunchKernelGGL(k_range_ampl, dim3(nb * 2 * (d.NY / d.G1)), dim3(kThreads), sh1, c->stream, d, c->d_master,
-                       c->d_slave, d_win, c->amp, c->sums, (const float2*)c->tabs);
-  });
+  const bool row256 = d.NX == 256, col256 = d.NY == 256;
+  int rc;
+  if (row256 && d.ri == 2)
+    rc = launch_timed(c, timing, "k_range256", [&] {
+      hipLaunchKernelGGL(k_range256, dim3(nb * 2), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_win,
+                         c->amp, c->sums, (const float2*)c->tabs);
+    });
+  else
+    rc = launch_timed(c, timing, "k_range_ampl", [&] {
+      hipLaunchKernelGGL(k_range_ampl, dim3(nb * 2 * (d.NY / d.G1)), dim3(kThreads), sh1, c->stream, d, c->d_master,
+                         c->d_slave, d_win, c->amp, c->sums, (const float2*)c->tabs);
+    });
   if (rc) return rc;
   if (d.corr_flag == XCORR_HIP_FREQ) {
-    rc = launch_timed(c, timing, "k_rowfft", [&] {
-      hipLaunchKernelGGL(k_rowfft, dim3(nb * (d.NY / d.G1)), dim3(kThreads), nel1 * sizeof(float2) * 2, c->stream, d,
-                         (const float*)c->amp, (const unsigned long long*)c->sums, c->Z, (const float2*)c->tabs);
-    });
+    if (row256)
+      rc = launch_timed(c, timing, "k_rowfft256", [&] {
+        hipLaunchKernelGGL(k_rowfft256, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float*)c->amp,
+                           (const unsigned long long*)c->sums, c->Z, (const float2*)c->tabs);
+      });
+    else
+      rc = launch_timed(c, timing, "k_rowfft", [&] {
+        hipLaunchKernelGGL(k_rowfft, dim3(nb * (d.NY / d.G1)), dim3(kThreads), nel1 * sizeof(float2) * 2, c->stream,
+                           d, (const float*)c->amp, (const unsigned long long*)c->sums, c->Z, (const float2*)c->tabs);
+      });
     if (rc) return rc;
     const size_t nel3 = (size_t)d.C3 * d.NY;
-    rc = launch_timed(c, timing, "k_colfft", [&] {
-      hipLaunchKernelGGL(k_colfft, dim3(nb * (d.NX / d.C3)), dim3(kThreads), nel3 * sizeof(float2) * 2, c->stream, d,
-                         c->Z, (const float2*)c->tabs);
-    });
-    if (rc) return rc;
-    rc = launch_timed(c, timing, "k_product_colifft", [&] {
-      hipLaunchKernelGGL(k_product_colifft, dim3(nb * (d.NX / d.C3)), dim3(kThreads), nel3 * sizeof(float2) * 3,
-                         c->stream, d, (const float2*)c->Z, c->D, (const float2*)c->tabs);
-    });
+    if (col256 && d.NX >= 32)
+      rc = launch_timed(c, timing, "k_colpass256", [&] {
+        hipLaunchKernelGGL(k_colpass256, dim3(nb * (d.NX / 32)), dim3(kThreads), 0, c->stream, d,
+                           (const float2*)c->Z, c->D, (const float2*)c->tabs);
+      });
+    else {
+      if (col256)
+        rc = launch_timed(c, timing, "k_colfft256", [&] {
+          hipLaunchKernelGGL(k_colfft256, dim3(nb * (d.NX / 16)), dim3(kThreads), 0, c->stream, d, c->Z,
+                             (const float2*)c->tabs);
+        });
+      else
+        rc = launch_timed(c, timing, "k_colfft", [&] {
+          hipLaunchKernelGGL(k_colfft, dim3(nb * (d.NX / d.C3)), dim3(kThreads), nel3 * sizeof(float2) * 2,
+                             c->stream, d, c->Z, (const float2*)c->tabs);
+        });
+      if (rc) return rc;
+      if (col256)
+        rc = launch_timed(c, timing, "k_product_colifft256", [&] {
+          hipLaunchKernelGGL(k_product_colifft256, dim3(nb * (d.NX / 16)), dim3(kThreads), 0, c->stream, d,
+                             (const float2*)c->Z, c->D, (const float2*)c->tabs);
+        });
+      else
+        rc = launch_timed(c, timing, "k_product_colifft", [&] {
+          hipLaunchKernelGGL(k_product_colifft, dim3(nb * (d.NX / d.C3)), dim3(kThreads), nel3 * sizeof(float2) * 3,
+                             c->stream, d, (const float2*)c->Z, c->D, (const float2*)c->tabs);
+        });
+    }
     if (rc) return rc;
     const size_t nel4 = (size_t)d.G4 * d.NX;
-    rc = launch_timed(c, timing, "k_rowifft_surf", [&] {
-      hipLaunchKernelGGL(k_rowifft_surf, dim3(nb * d.groups4), dim3(kThreads), nel4 * sizeof(float2) * 2, c->stream, d,
-                         (const float2*)c->D, c->surf, c->peak, c->partial, (const float2*)c->tabs);
-    });
+    if (row256)
+      rc = launch_timed(c, timing, "k_rowifft_surf256", [&] {
+        hipLaunchKernelGGL(k_rowifft_surf256, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)c->D,
+                           c->surf, c->peak, c->partial, (const float2*)c->tabs);
+      });
+    else
+      rc = launch_timed(c, timing, "k_rowifft_surf", [&] {
+        hipLaunchKernelGGL(k_rowifft_surf, dim3(nb * d.groups4), dim3(kThreads), nel4 * sizeof(float2) * 2,
+                           c->stream, d, (const float2*)c->D, c->surf, c->peak, c->partial, (const float2*)c->tabs);
+      });
     if (rc) return rc;
   } else {
     rc = time_search(c, d_win, nb, timing);

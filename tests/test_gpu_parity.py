@@ -60,6 +60,11 @@ CASES = [
     (16, 32, 4, 4, 1),
     (32, 32, 2, 16, 0),
     (128, 128, 2, 8, 1),
+    (64, 8, 2, 16, 1),
+    (64, 4, 2, 8, 1),
+    (8, 64, 2, 16, 1),
+    (64, 64, 4, 16, 1),
+    (64, 64, 1, 16, 0),
 ]
 
 
