@@ -451,37 +451,36 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
     const float2 zm = self ? sx[my] : T[my * kT256Pitch + g];
     pT[r] = c3_bin(vT[r], zm, kxT, ky);
   }
-  __syncthreads();
+  // Only columns 0..npx/2 are needed downstream (the rest are conjugate
+  // mirrors); of the M columns that is npx/2 alone, held by group 0 of block 0.
+  if (p == 0) {
+    __syncthreads();
+    if (self)
 #pragma unroll
-  for (int r = 0; r < 16; r++) T[(j + 16 * r) * kT256Pitch + g] = vT[r];
-  if (self)
+      for (int r = 0; r < 16; r++) sx[j + 16 * r] = vM[r];
+    __syncthreads();
+    if (self)
 #pragma unroll
-    for (int r = 0; r < 16; r++) sx[j + 16 * r] = vM[r];
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < 16; r++) {
-    const int ky = j + 16 * r;
-    const int my = (256 - ky) & 255;
-    const float2 zm = self ? sx[my] : T[my * kT256Pitch + g];
-    vM[r] = c3_bin(vM[r], zm, kxM, ky);
+      for (int r = 0; r < 16; r++) {
+        const int ky = j + 16 * r;
+        vM[r] = c3_bin(vM[r], sx[(256 - ky) & 255], kxM, ky);
+      }
   }
   __syncthreads();
   fft256_group<+1>(pT, j, T + g * kT256Tile, t);
-  fft256_group<+1>(vM, j, T + g * kT256Tile, t);
-  // lag rows 64..191 (r = 4..11) of both columns: staged as [row][32 columns]
-  constexpr int kP = 33;
+  if (p == 0) fft256_group<+1>(vM, j, T + g * kT256Tile, t);
+  // lag rows 64..191 (r = 4..11)
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    T[(j + 16 * q) * kP + g] = pT[q + 4];
-    T[(j + 16 * q) * kP + 16 + g] = vM[q + 4];
-  }
+  for (int q = 0; q < 8; q++) T[(j + 16 * q) * kT256Pitch + g] = pT[q + 4];
+  if (self)
+#pragma unroll
+    for (int q = 0; q < 8; q++) sx[j + 16 * q] = vM[q + 4];
   __syncthreads();
   float2* db = D + (size_t)b * 128 * NX;
-  for (int e = threadIdx.x; e < 128 * 32; e += kThreads) {
-    const int row = e >> 5, c = e & 31;
-    const int kx = c < 16 ? 16 * p + c : ((p == 0 && c == 16) ? NX / 2 : NX - 16 * p - (c - 16));
-    db[(size_t)row * NX + kx] = T[row * kP + c];
-  }
+  for (int e = threadIdx.x; e < 128 * 16; e += kThreads)
+    db[(size_t)(e >> 4) * NX + 16 * p + (e & 15)] = T[(e >> 4) * kT256Pitch + (e & 15)];
+  if (p == 0)
+    for (int e = threadIdx.x; e < 128; e += kThreads) db[(size_t)e * NX + NX / 2] = sx[e];
 }
 
 // Stage (b) row IFFT + surface + argmax for npx = 256 (nx_corr/2 = 64, nxc = 128):
@@ -510,7 +509,15 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
     const bool active = row < P.nyc;
     float2 v[16];
 #pragma unroll
-    for (int r = 0; r < 16; r++) v[r] = active ? db[(size_t)row * 256 + j + 16 * r] : make_float2(0.0f, 0.0f);
+    for (int r = 0; r < 16; r++) {
+      const int x = j + 16 * r;
+      float2 u = make_float2(0.0f, 0.0f);
+      if (active) {
+        u = db[(size_t)row * 256 + (x <= 128 ? x : 256 - x)];   // upper half: conjugate mirror
+        if (x > 128) u.y = -u.y;
+      }
+      v[r] = u;
+    }
     fft256_group<+1>(v, j, tile, t);
     if (!active) continue;
 #pragma unroll
@@ -647,7 +654,18 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf(DevParams P, const fl
   const float2* db = D + ((size_t)b * P.nyc + (size_t)g * G) * NX;
   float2* A = smem;
   float2* Bf = A + nel;
-  for (int e = threadIdx.x; e < nel; e += kThreads) A[e] = db[e];
+  // columns above npx/2 are the conjugate mirrors of the lower half (C3 is Hermitian)
+  for (int e = threadIdx.x; e < nel; e += kThreads) {
+    const int row = e / NX, x = e - row * NX;
+    float2 v;
+    if (x <= NX / 2) {
+      v = db[e];
+    } else {
+      v = db[row * NX + NX - x];
+      v.y = -v.y;
+    }
+    A[e] = v;
+  }
   __syncthreads();
   float2* R = lds_fft_rows<+1>(A, Bf, NX, G, tabs + P.tw_nx);
   const int q0 = P.nx_corr / 2;

@@ -400,11 +400,20 @@ int xo32_patch(const xo_params *p, const c32 *mwin, const c32 *swin, xo_result *
       }
     }
     const int r0 = p->ny_corr / 2, q0 = p->nx_corr / 2;
-    for (int x = 0; x < NX; x++) {
+    /* C3 is exactly Hermitian (C3[-k] == conj(C3[k]) in this arithmetic), so
+     * only columns 0..NX/2 are inverse-transformed along y; the others are
+     * their conjugate mirrors */
+    for (int x = 0; x <= NX / 2; x++) {
       for (int y = 0; y < NY; y++) col[y] = c3[(size_t)y * NX + x];
       xo32_fft(col, NY, +1);
       for (int y = 0; y < NY; y++) c3[(size_t)y * NX + x] = col[y];
     }
+    for (int x = NX / 2 + 1; x < NX; x++)
+      for (int y = 0; y < NY; y++) {
+        c32 v = c3[(size_t)y * NX + (NX - x)];
+        v.im = -v.im;
+        c3[(size_t)y * NX + x] = v;
+      }
     const float sc = 1.0f / (float)N;
     for (int iy = 0; iy < nyc; iy++) {
       c32 *row = c3 + (size_t)(iy + r0) * NX;
