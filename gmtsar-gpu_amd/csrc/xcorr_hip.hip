@@ -40,6 +40,7 @@ struct DevParams {
   int groups4;                    // k_rowifft_surf blocks per patch
   int tLR, tRC;                   // k_time_search: surface rows per block, template rows per LDS chunk
   int tw_nx, tw_ny, tw_lx, tw_ly, phi;   // offsets into the table buffer
+  int mk_x, mk_y;                 // spectra of the 1-D mask boxes (npx, npy entries)
   int m_nx, m_ny, s_nx, s_ny;
 };
 
@@ -208,9 +209,51 @@ __global__ void __launch_bounds__(kThreads) k_range_ampl(DevParams P, const void
 // per-lane twiddles and phase factors are loaded once.
 // --------------------------------------------------------------------------
 
-// Stage (a) for npx = 256, ri = 2: one block per (patch, image), 16 groups of
-// 16 lanes, one row per group and iteration.  Output columns 2q-128, 2q-127
-// are the original sample q and the half-sample interpolant at q + 1/2.
+// One line of stage (a) for npx = 256, ri = 2, held by a 16-lane group: gather,
+// range interpolation, amplitudes.  On return ea[q] / oa[q] are the amplitudes of
+// output columns 2q' - 128 and 2q' - 127, q' = j + 16 (q + 4): the original
+// sample q' and the half-sample interpolant at q' + 1/2.
+__device__ __forceinline__ void range_row256(const void* __restrict__ src, int format, int inx, int iny, bool inside,
+                                             int gy, int x0, int j, float2* tile, const float2 (&t)[15],
+                                             const float2 (&ph)[16], float (&ea)[8], float (&oa)[8]) {
+  float2 v[16];
+  if (format == XCORR_HIP_CINT16) {
+    const unsigned* im = (const unsigned*)src;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int gx = x0 + j + 16 * r;
+      unsigned u = 0u;
+      if (inside || (gy >= 0 && gy < iny && gx >= 0 && gx < inx)) u = im[(size_t)gy * inx + gx];
+      v[r] = make_float2((float)(short)(u & 0xffffu), (float)(short)(u >> 16));
+    }
+  } else {
+    const float* im = (const float*)src;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int gx = x0 + j + 16 * r;
+      float u = 0.0f;
+      if (inside || (gy >= 0 && gy < iny && gx >= 0 && gx < inx)) u = im[(size_t)gy * inx + gx];
+      v[r] = make_float2(u, 0.0f);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) ea[q] = ampl(v[q + 4]);
+  fft256_group<-1>(v, j, tile, t);
+#pragma unroll
+  for (int r = 0; r < 16; r++) v[r] = cmul(v[r], ph[r]);
+  fft256_group<+1>(v, j, tile, t);
+  const float sc = 1.0f / 256.0f;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    float2 o = v[q + 4];
+    o.x = o.x * sc;
+    o.y = o.y * sc;
+    oa[q] = ampl(o);
+  }
+}
+
+// Stage (a) for npx = 256, ri = 2 (time modes): one block per (patch, image),
+// 16 groups of 16 lanes, one line per group and iteration.
 __global__ void __launch_bounds__(kThreads) k_range256(DevParams P, const void* __restrict__ master,
                                                        const void* __restrict__ slave,
                                                        const xcorr_hip_window* __restrict__ win,
@@ -235,47 +278,95 @@ __global__ void __launch_bounds__(kThreads) k_range256(DevParams P, const void* 
   float2* outb = (float2*)(amp + (size_t)(2 * b + img) * P.N);
   long long s = 0;
   for (int row = g; row < P.NY; row += 16) {
-    const int gy = y0 + row;
-    float2 v[16];
-    if (P.format == XCORR_HIP_CINT16) {
-      const unsigned* im = (const unsigned*)src;
-#pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const int gx = x0 + j + 16 * r;
-        unsigned u = 0u;
-        if (inside || (gy >= 0 && gy < iny && gx >= 0 && gx < inx)) u = im[(size_t)gy * inx + gx];
-        v[r] = make_float2((float)(short)(u & 0xffffu), (float)(short)(u >> 16));
-      }
-    } else {
-      const float* im = (const float*)src;
-#pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const int gx = x0 + j + 16 * r;
-        float u = 0.0f;
-        if (inside || (gy >= 0 && gy < iny && gx >= 0 && gx < inx)) u = im[(size_t)gy * inx + gx];
-        v[r] = make_float2(u, 0.0f);
-      }
-    }
-    float ea[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) ea[q] = ampl(v[q + 4]);
-    fft256_group<-1>(v, j, tile, t);
-#pragma unroll
-    for (int r = 0; r < 16; r++) v[r] = cmul(v[r], ph[r]);
-    fft256_group<+1>(v, j, tile, t);
-    const float sc = 1.0f / 256.0f;
+    float ea[8], oa[8];
+    range_row256(src, P.format, inx, iny, inside, y0 + row, x0, j, tile, t, ph, ea, oa);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      float2 o = v[q + 4];
-      o.x = o.x * sc;
-      o.y = o.y * sc;
-      const float oa = ampl(o);
-      outb[(size_t)row * 128 + j + 16 * q] = make_float2(ea[q], oa);
-      s += (long long)(ea[q] * 1048576.0f) + (long long)(oa * 1048576.0f);
+      outb[(size_t)row * 128 + j + 16 * q] = make_float2(ea[q], oa[q]);
+      s += (long long)(ea[q] * 1048576.0f) + (long long)(oa[q] * 1048576.0f);
     }
   }
   s = block_sum_i64(s, red);
   if (threadIdx.x == 0) sums[2 * b + img] = (unsigned long long)s;
+}
+
+// Stage (a) and the row transform of stage (b) for npx = 256, ri = 2, frequency
+// mode: one block per patch.  For each line: both images' range interpolation
+// and amplitudes, then z = amp1 + i amp2 mask and its FFT along x (the means
+// come off later, in the spectrum).  Amplitudes are kept for the NCC: the
+// master's everywhere, the slave's on the template.
+__global__ void __launch_bounds__(kThreads, 3) k_stageA256(DevParams P, const void* __restrict__ master,
+                                                        const void* __restrict__ slave,
+                                                        const xcorr_hip_window* __restrict__ win,
+                                                        float* __restrict__ amp,
+                                                        unsigned long long* __restrict__ sums,
+                                                        float2* __restrict__ Z, const float2* __restrict__ tabs) {
+  __shared__ float2 tiles[16 * kT256Tile];
+  __shared__ float2 mrow[16 * 128];                // each group's master amplitude line (column pairs)
+  __shared__ long long red[kThreads / 64];
+  const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+  const int b = blockIdx.x;
+  const xcorr_hip_window w = win[b];
+  float2* tile = tiles + g * kT256Tile;
+  float2* mr = mrow + g * 128;
+  float2 t[15];
+  load_tw256(tabs + P.tw_nx, j, t);
+  float2 ph[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++) ph[r] = tabs[P.phi + j + 16 * r];
+  const int NY = P.NY;
+  const bool in_m = w.mx0 >= 0 && w.mx0 + 256 <= P.m_nx && w.my0 >= 0 && w.my0 + NY <= P.m_ny;
+  const bool in_s = w.sx0 >= 0 && w.sx0 + 256 <= P.s_nx && w.sy0 >= 0 && w.sy0 + NY <= P.s_ny;
+  float2* out1 = (float2*)(amp + (size_t)(2 * b) * P.N);
+  float2* out2 = (float2*)(amp + (size_t)(2 * b + 1) * P.N);
+  float2* zb = Z + (size_t)b * P.N;
+  long long s1 = 0, s2 = 0;
+  for (int row = g; row < NY; row += 16) {
+    {
+      float ea[8], oa[8];
+      range_row256(master, P.format, P.m_nx, P.m_ny, in_m, w.my0 + row, w.mx0, j, tile, t, ph, ea, oa);
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const float2 a = make_float2(ea[q], oa[q]);
+        out1[(size_t)row * 128 + j + 16 * q] = a;
+        mr[j + 16 * q] = a;
+        s1 += (long long)(ea[q] * 1048576.0f) + (long long)(oa[q] * 1048576.0f);
+      }
+    }
+    {
+      float ea[8], oa[8];
+      range_row256(slave, P.format, P.s_nx, P.s_ny, in_s, w.sy0 + row, w.sx0, j, tile, t, ph, ea, oa);
+      const bool trow = row >= P.ys && row < P.ys + P.ny_corr;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const float2 a = make_float2(ea[q], oa[q]);
+        if (trow && q >= 2 && q < 6) out2[(size_t)row * 128 + j + 16 * q] = a;
+        tile[j + 16 * q] = a;                      // the tile doubles as the slave line buffer
+        s2 += (long long)(ea[q] * 1048576.0f) + (long long)(oa[q] * 1048576.0f);
+      }
+    }
+    __syncthreads();
+    const float* m1 = (const float*)mr;
+    const float* a2 = (const float*)tile;
+    const bool in_y = row >= P.ys && row < NY - P.ys;
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int x = j + 16 * r;
+      const bool in = in_y && x >= P.xs && x < 256 - P.xs;
+      v[r] = make_float2(m1[x], in ? a2[x] : 0.0f);
+    }
+    __syncthreads();
+    fft256_group<-1>(v, j, tile, t);
+#pragma unroll
+    for (int r = 0; r < 16; r++) zb[(size_t)row * 256 + j + 16 * r] = v[r];
+  }
+  s1 = block_sum_i64(s1, red);
+  s2 = block_sum_i64(s2, red);
+  if (threadIdx.x == 0) {
+    sums[2 * b] = (unsigned long long)s1;
+    sums[2 * b + 1] = (unsigned long long)s2;
+  }
 }
 
 // Stage (b) rows for npx = 256: one block per patch.
@@ -285,8 +376,6 @@ __global__ void __launch_bounds__(kThreads) k_rowfft256(DevParams P, const float
   __shared__ float2 tiles[16 * kT256Tile];
   const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
   const int b = blockIdx.x;
-  const float m1f = (float)patch_mean(sums[2 * b], P.N);
-  const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
   const float* a1 = amp + (size_t)(2 * b) * P.N;
   const float* a2 = a1 + P.N;
   float2* zb = Z + (size_t)b * P.N;
@@ -299,10 +388,8 @@ __global__ void __launch_bounds__(kThreads) k_rowfft256(DevParams P, const float
 #pragma unroll
     for (int r = 0; r < 16; r++) {
       const int x = j + 16 * r;
-      const float c1 = a1[row * 256 + x] - m1f;
-      float c2 = a2[row * 256 + x] - m2f;
-      if (!(in_y && x >= P.xs && x < 256 - P.xs)) c2 = 0.0f;
-      v[r] = make_float2(c1, c2);
+      const bool in = in_y && x >= P.xs && x < 256 - P.xs;
+      v[r] = make_float2(a1[row * 256 + x], in ? a2[row * 256 + x] : 0.0f);
     }
     fft256_group<-1>(v, j, tile, t);
 #pragma unroll
@@ -310,84 +397,7 @@ __global__ void __launch_bounds__(kThreads) k_rowfft256(DevParams P, const float
   }
 }
 
-// Stage (b) columns for npy = 256: FFT along y of a 16-column tile, in place.
-// The tile is staged through LDS (coalesced 128-byte rows) and reused as the
-// groups' transpose scratch.
-__global__ void __launch_bounds__(kThreads) k_colfft256(DevParams P, float2* __restrict__ Z,
-                                                        const float2* __restrict__ tabs) {
-  __shared__ float2 T[256 * kT256Pitch];
-  const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
-  const int NX = P.NX;
-  const int tiles = NX >> 4;
-  const int ct = blockIdx.x % tiles, b = blockIdx.x / tiles;
-  float2* zb = Z + (size_t)b * P.N + ct * 16;
-  float2 t[15];
-  load_tw256(tabs + P.tw_ny, j, t);
-  for (int e = threadIdx.x; e < 256 * 16; e += kThreads) T[(e >> 4) * kT256Pitch + (e & 15)] = zb[(size_t)(e >> 4) * NX + (e & 15)];
-  __syncthreads();
-  float2 v[16];
-#pragma unroll
-  for (int r = 0; r < 16; r++) v[r] = T[(j + 16 * r) * kT256Pitch + g];
-  __syncthreads();
-  fft256_group<-1>(v, j, T + g * kT256Tile, t);
-#pragma unroll
-  for (int r = 0; r < 16; r++) T[(j + 16 * r) * kT256Pitch + g] = v[r];
-  __syncthreads();
-  for (int e = threadIdx.x; e < 256 * 16; e += kThreads) zb[(size_t)(e >> 4) * NX + (e & 15)] = T[(e >> 4) * kT256Pitch + (e & 15)];
-}
-
-// Stage (b) product + column IFFT for npy = 256 (ny_corr/2 = 64, nyc = 128).
-__global__ void __launch_bounds__(kThreads) k_product_colifft256(DevParams P, const float2* __restrict__ Z,
-                                                                 float2* __restrict__ D,
-                                                                 const float2* __restrict__ tabs) {
-  __shared__ float2 T[256 * kT256Pitch];
-  const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
-  const int NX = P.NX;
-  const int tiles = NX >> 4;
-  const int ct = blockIdx.x % tiles, b = blockIdx.x / tiles;
-  const float2* zb = Z + (size_t)b * P.N;
-  float2 t[15];
-  load_tw256(tabs + P.tw_ny, j, t);
-  // this block's columns
-  for (int e = threadIdx.x; e < 256 * 16; e += kThreads)
-    T[(e >> 4) * kT256Pitch + (e & 15)] = zb[(size_t)(e >> 4) * NX + ct * 16 + (e & 15)];
-  __syncthreads();
-  const int kx = ct * 16 + g;
-  float2 v[16];
-#pragma unroll
-  for (int r = 0; r < 16; r++) v[r] = T[(j + 16 * r) * kT256Pitch + g];
-  __syncthreads();
-  // mirror columns (NX - kx) mod NX of the tile: staged in tile order c' = g
-  for (int e = threadIdx.x; e < 256 * 16; e += kThreads) {
-    const int c = e & 15;
-    const int mx = (NX - (ct * 16 + c)) & (NX - 1);
-    T[(e >> 4) * kT256Pitch + c] = zb[(size_t)(e >> 4) * NX + mx];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < 16; r++) {
-    const int ky = j + 16 * r;
-    const float2 a = v[r];
-    const float2 zm = T[((256 - ky) & 255) * kT256Pitch + g];
-    const float2 bb = make_float2(zm.x, -zm.y);
-    const float2 c1 = make_float2((a.x + bb.x) * 0.5f, (a.y + bb.y) * 0.5f);
-    const float2 d = csub(a, bb);
-    const float2 c2c = make_float2(0.5f * d.y, 0.5f * d.x);
-    float2 p = cmul(c1, c2c);
-    if ((kx + ky) & 1) { p.x = -p.x; p.y = -p.y; }
-    v[r] = p;
-  }
-  __syncthreads();
-  fft256_group<+1>(v, j, T + g * kT256Tile, t);
-  // keep rows 64..191 (ny_corr/2 + [0, nyc)): r = 4..11
-#pragma unroll
-  for (int q = 0; q < 8; q++) T[(j + 16 * q) * kT256Pitch + g] = v[q + 4];
-  __syncthreads();
-  float2* db = D + (size_t)b * 128 * NX + ct * 16;
-  for (int e = threadIdx.x; e < 128 * 16; e += kThreads) db[(size_t)(e >> 4) * NX + (e & 15)] = T[(e >> 4) * kT256Pitch + (e & 15)];
-}
-
-// C3 at one bin from A = Z[ky][kx] and Zm = Z[-ky][-kx] (spec of k_product_colifft).
+// C3 at one bin from A = Z[ky][kx] and Zm = Z[-ky][-kx] (Z = FFT2(c1 + i c2)).
 __device__ __forceinline__ float2 c3_bin(float2 a, float2 zm, int kx, int ky) {
   const float2 bb = make_float2(zm.x, -zm.y);
   const float2 c1 = make_float2((a.x + bb.x) * 0.5f, (a.y + bb.y) * 0.5f);
@@ -398,12 +408,25 @@ __device__ __forceinline__ float2 c3_bin(float2 a, float2 zm, int kx, int ky) {
   return p;
 }
 
+// Demean in the row spectrum: the row transform of z = amp1 + i amp2 mask minus
+// that of m1 + i m2 mask, i.e. Zr - m1 npx delta(kx) - i m2 [y in mask] Mx[kx].
+__device__ __forceinline__ float2 demean_row_bin(float2 v, int kx, bool ym, float m1NX, float m2f, float2 mx) {
+  float re = v.x, im = v.y;
+  if (kx == 0) re = re - m1NX;
+  if (ym) {
+    re = __builtin_fmaf(m2f, mx.y, re);
+    im = __builtin_fmaf(-m2f, mx.x, im);
+  }
+  return make_float2(re, im);
+}
+
 // Stage (b) columns, fused, for npy = 256 (ny_corr/2 = 64, nyc = 128): FFT along y,
 // C3 = C1 conj(C2) (-1)^(kx+ky), IFFT along y, keep lag rows 64..191.  Block p
 // owns columns T = [16p, 16p+16) and their mirrors M = {npx - 16p - g} (with
 // npx/2 in place of the duplicate column 0 for p = 0), so every column is
 // transformed once and C3 needs no second pass over Z.  Requires npx >= 32.
 __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const float2* __restrict__ Z,
+                                                         const unsigned long long* __restrict__ sums,
                                                          float2* __restrict__ D, const float2* __restrict__ tabs) {
   __shared__ float2 T[256 * kT256Pitch];
   __shared__ float2 sx[256];                                 // spectrum of a self-mirrored column
@@ -417,18 +440,24 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
   const bool self = (p == 0 && g == 0);                      // columns 0 and npx/2 mirror onto themselves
   float2 t[15];
   load_tw256(tabs + P.tw_ny, j, t);
+  const float m1NX = (float)patch_mean(sums[2 * b], P.N) * (float)NX;
+  const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
   float2 vT[16], vM[16];
-  // stage and load the T columns, then the M columns
-  for (int e = threadIdx.x; e < 256 * 16; e += kThreads)
-    T[(e >> 4) * kT256Pitch + (e & 15)] = zb[(size_t)(e >> 4) * NX + 16 * p + (e & 15)];
+  // stage and load the T columns, then the M columns (demeaned on the way in)
+  for (int e = threadIdx.x; e < 256 * 16; e += kThreads) {
+    const int y = e >> 4, kx = 16 * p + (e & 15);
+    T[y * kT256Pitch + (e & 15)] = demean_row_bin(zb[(size_t)y * NX + kx], kx, y >= P.ys && y < 256 - P.ys, m1NX,
+                                                  m2f, tabs[P.mk_x + kx]);
+  }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < 16; r++) vT[r] = T[(j + 16 * r) * kT256Pitch + g];
   __syncthreads();
   for (int e = threadIdx.x; e < 256 * 16; e += kThreads) {
-    const int c = e & 15;
+    const int c = e & 15, y = e >> 4;
     const int mx = (p == 0 && c == 0) ? NX / 2 : NX - 16 * p - c;
-    T[(e >> 4) * kT256Pitch + c] = zb[(size_t)(e >> 4) * NX + mx];
+    T[y * kT256Pitch + c] = demean_row_bin(zb[(size_t)y * NX + mx], mx, y >= P.ys && y < 256 - P.ys, m1NX, m2f,
+                                           tabs[P.mk_x + mx]);
   }
   __syncthreads();
 #pragma unroll
@@ -551,20 +580,16 @@ __global__ void __launch_bounds__(kThreads) k_rowfft(DevParams P, const float* _
   const int NX = P.NX, NY = P.NY, G = P.G1, nel = G * NX;
   const int groups = NY / G;
   const int g = blockIdx.x % groups, b = blockIdx.x / groups;
-  const float m1f = (float)patch_mean(sums[2 * b], P.N);
-  const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
   const float* a1 = amp + (size_t)(2 * b) * P.N + (size_t)g * G * NX;
   const float* a2 = a1 + P.N;
   float2* A = smem;
   float2* Bf = A + nel;
+  // raw amplitudes; the means come off in the row spectrum (demean_row_bin)
   for (int e = threadIdx.x; e < nel; e += kThreads) {
     const int row = e / NX, x = e - row * NX;
     const int y = g * G + row;
-    const float c1 = a1[e] - m1f;
-    float c2 = a2[e] - m2f;
     const bool in = y >= P.ys && y < NY - P.ys && x >= P.xs && x < NX - P.xs;
-    if (!in) c2 = 0.0f;
-    A[e] = make_float2(c1, c2);
+    A[e] = make_float2(a1[e], in ? a2[e] : 0.0f);
   }
   __syncthreads();
   float2* R = lds_fft_rows<-1>(A, Bf, NX, G, tabs + P.tw_nx);
@@ -574,6 +599,7 @@ __global__ void __launch_bounds__(kThreads) k_rowfft(DevParams P, const float* _
 
 // FFT along y, in place on Z (columns tile per block).
 __global__ void __launch_bounds__(kThreads) k_colfft(DevParams P, float2* __restrict__ Z,
+                                                     const unsigned long long* __restrict__ sums,
                                                      const float2* __restrict__ tabs) {
   extern __shared__ float2 smem[];
   const int NX = P.NX, NY = P.NY, C = P.C3, nel = C * NY;
@@ -582,9 +608,13 @@ __global__ void __launch_bounds__(kThreads) k_colfft(DevParams P, float2* __rest
   float2* zb = Z + (size_t)b * P.N + ct * C;
   float2* A = smem;
   float2* Bf = A + nel;
+  const float m1NX = (float)patch_mean(sums[2 * b], P.N) * (float)NX;
+  const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
   for (int e = threadIdx.x; e < nel; e += kThreads) {
     const int y = e / C, c = e - y * C;
-    A[c * NY + y] = zb[(size_t)y * NX + c];
+    const int kx = ct * C + c;
+    A[c * NY + y] = demean_row_bin(zb[(size_t)y * NX + c], kx, y >= P.ys && y < NY - P.ys, m1NX, m2f,
+                                   tabs[P.mk_x + kx]);
   }
   __syncthreads();
   float2* R = lds_fft_rows<-1>(A, Bf, NY, C, tabs + P.tw_ny);
@@ -618,15 +648,8 @@ __global__ void __launch_bounds__(kThreads) k_product_colifft(DevParams P, const
   for (int e = threadIdx.x; e < nel; e += kThreads) {
     const int c = e / NY, ky = e - c * NY;
     const int kx = ct * C + c;
-    const float2 a = A[e];
     const float2 zm = M[c * NY + ((NY - ky) & (NY - 1))];
-    const float2 bb = make_float2(zm.x, -zm.y);
-    const float2 c1 = make_float2((a.x + bb.x) * 0.5f, (a.y + bb.y) * 0.5f);
-    const float2 d = csub(a, bb);
-    const float2 c2c = make_float2(0.5f * d.y, 0.5f * d.x);
-    float2 p = cmul(c1, c2c);
-    if ((kx + ky) & 1) { p.x = -p.x; p.y = -p.y; }
-    O[e] = p;
+    O[e] = c3_bin(A[e], zm, kx, ky);
   }
   __syncthreads();
   float2* R = lds_fft_rows<+1>(O, A, NY, C, tabs + P.tw_ny);
@@ -1248,6 +1271,22 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   d.tw_ny = add_table(d.NY);
   d.tw_lx = add_table(d.Lx);
   d.tw_ly = add_table(d.Ly);
+  auto add_mask = [&](int n, int sx) {   // spectrum of the box [sx, n - sx), summed in double
+    int off = (int)tab.size();
+    for (int k = 0; k < n; k++) {
+      double re = 0.0, im = 0.0;
+      for (int x = sx; x < n - sx; x++) {
+        double cc, ss;
+        twiddle64(n, (long)k * x, &cc, &ss);
+        re += cc;
+        im -= ss;
+      }
+      tab.push_back(make_float2((float)re, (float)im));
+    }
+    return off;
+  };
+  d.mk_x = add_mask(d.NX, d.xs);
+  d.mk_y = add_mask(d.NY, d.ys);
   d.phi = (int)tab.size();
   const int M = d.ri * d.NX;
   for (int r = 1; r < d.ri; r++)
@@ -1426,8 +1465,15 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
   const size_t nel1 = (size_t)d.G1 * d.NX;
   const size_t sh1 = nel1 * sizeof(float2) * (d.ri > 2 ? 3 : 2) + nel1 * sizeof(float);
   const bool row256 = d.NX == 256, col256 = d.NY == 256;
+  const bool freq = d.corr_flag == XCORR_HIP_FREQ;
+  const bool fusedA = row256 && d.ri == 2 && freq;
   int rc;
-  if (row256 && d.ri == 2)
+  if (fusedA)
+    rc = launch_timed(c, timing, "k_stageA256", [&] {
+      hipLaunchKernelGGL(k_stageA256, dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_win,
+                         c->amp, c->sums, c->Z, (const float2*)c->tabs);
+    });
+  else if (row256 && d.ri == 2)
     rc = launch_timed(c, timing, "k_range256", [&] {
       hipLaunchKernelGGL(k_range256, dim3(nb * 2), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_win,
                          c->amp, c->sums, (const float2*)c->tabs);
@@ -1438,8 +1484,10 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
                          c->d_slave, d_win, c->amp, c->sums, (const float2*)c->tabs);
     });
   if (rc) return rc;
-  if (d.corr_flag == XCORR_HIP_FREQ) {
-    if (row256)
+  if (freq) {
+    if (fusedA)
+      rc = 0;
+    else if (row256)
       rc = launch_timed(c, timing, "k_rowfft256", [&] {
         hipLaunchKernelGGL(k_rowfft256, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float*)c->amp,
                            (const unsigned long long*)c->sums, c->Z, (const float2*)c->tabs);
@@ -1451,33 +1499,21 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
       });
     if (rc) return rc;
     const size_t nel3 = (size_t)d.C3 * d.NY;
-    if (col256 && d.NX >= 32)
+    if (col256 && d.NX >= 32) {
       rc = launch_timed(c, timing, "k_colpass256", [&] {
         hipLaunchKernelGGL(k_colpass256, dim3(nb * (d.NX / 32)), dim3(kThreads), 0, c->stream, d,
-                           (const float2*)c->Z, c->D, (const float2*)c->tabs);
+                           (const float2*)c->Z, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs);
       });
-    else {
-      if (col256)
-        rc = launch_timed(c, timing, "k_colfft256", [&] {
-          hipLaunchKernelGGL(k_colfft256, dim3(nb * (d.NX / 16)), dim3(kThreads), 0, c->stream, d, c->Z,
-                             (const float2*)c->tabs);
-        });
-      else
-        rc = launch_timed(c, timing, "k_colfft", [&] {
-          hipLaunchKernelGGL(k_colfft, dim3(nb * (d.NX / d.C3)), dim3(kThreads), nel3 * sizeof(float2) * 2,
-                             c->stream, d, c->Z, (const float2*)c->tabs);
-        });
+    } else {
+      rc = launch_timed(c, timing, "k_colfft", [&] {
+        hipLaunchKernelGGL(k_colfft, dim3(nb * (d.NX / d.C3)), dim3(kThreads), nel3 * sizeof(float2) * 2,
+                           c->stream, d, c->Z, (const unsigned long long*)c->sums, (const float2*)c->tabs);
+      });
       if (rc) return rc;
-      if (col256)
-        rc = launch_timed(c, timing, "k_product_colifft256", [&] {
-          hipLaunchKernelGGL(k_product_colifft256, dim3(nb * (d.NX / 16)), dim3(kThreads), 0, c->stream, d,
-                             (const float2*)c->Z, c->D, (const float2*)c->tabs);
-        });
-      else
-        rc = launch_timed(c, timing, "k_product_colifft", [&] {
-          hipLaunchKernelGGL(k_product_colifft, dim3(nb * (d.NX / d.C3)), dim3(kThreads), nel3 * sizeof(float2) * 3,
-                             c->stream, d, (const float2*)c->Z, c->D, (const float2*)c->tabs);
-        });
+      rc = launch_timed(c, timing, "k_product_colifft", [&] {
+        hipLaunchKernelGGL(k_product_colifft, dim3(nb * (d.NX / d.C3)), dim3(kThreads), nel3 * sizeof(float2) * 3,
+                           c->stream, d, (const float2*)c->Z, c->D, (const float2*)c->tabs);
+      });
     }
     if (rc) return rc;
     const size_t nel4 = (size_t)d.G4 * d.NX;

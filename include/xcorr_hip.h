@@ -122,7 +122,8 @@ int xcorr_hip_kernel_times(xcorr_hip_ctx *ctx, const char **names, float *ms, in
 
 /* Copy an internal stage buffer of patch `slot` of the last batch to host
  * memory, for stage-by-stage parity tests.  what: 0 amp master (npy*npx f32),
- * 1 amp slave, 2 correlation surface (nyc*nxc f32; f64 in time modes),
+ * 1 amp slave (valid on the template window; elsewhere only in some modes),
+ * 2 correlation surface (nyc*nxc f32; f64 in time modes),
  * 3 patch means (2 f64).  Returns bytes copied or a negative code. */
 long xcorr_hip_dump(xcorr_hip_ctx *ctx, int what, int slot, void *host, size_t bytes);
 

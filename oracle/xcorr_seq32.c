@@ -270,6 +270,23 @@ int xo32_range_interp_row(const c32 *x, int n, int ri, float *amp) {
   return 0;
 }
 
+/* Spectrum of the 1-D mask box [s, n - s) (make_mask, xcorr.cu:385-401):
+ * M[k] = sum_{x=s}^{n-s-1} e^{-2 pi i k x/n}, summed in double in increasing x,
+ * rounded to float.  M[-k] == conj(M[k]) exactly. */
+void xo_mask_spectrum(int n, int s, c32 *m) {
+  for (int k = 0; k < n; k++) {
+    double re = 0.0, im = 0.0;
+    for (int x = s; x < n - s; x++) {
+      double c, sn;
+      xo_twiddle64(n, (long)k * x, &c, &sn);
+      re += c;
+      im -= sn;
+    }
+    m[k].re = (float)re;
+    m[k].im = (float)im;
+  }
+}
+
 /* exact, order-free patch sum: each amplitude truncated to 2^-20 units */
 static inline int64_t fxp20(float a) { return (int64_t)(a * 1048576.0f); }
 
@@ -368,8 +385,10 @@ int xo32_patch(const xo_params *p, const c32 *mwin, const c32 *swin, xo_result *
       if (!inmask) c2 = 0.0f;
       i1[o] = (int32_t)c1;
       i2[o] = (int32_t)c2;
-      z[o].re = c1;
-      z[o].im = c2;
+      /* the transform takes the raw amplitudes; the means come off in the
+       * spectrum (mask_spectrum / spectral split below) */
+      z[o].re = amp1[o];
+      z[o].im = inmask ? amp2[o] : 0.0f;
     }
   }
 
@@ -380,7 +399,29 @@ int xo32_patch(const xo_params *p, const c32 *mwin, const c32 *swin, xo_result *
 
   if (p->corr_flag == 2) {
     /* (b) frequency correlation: do_freq_xcorr.cu:90-200 with c1 + i*c2 packed */
+    c32 *mx = (c32 *)malloc(sizeof(c32) * NX);
+    if (!mx) { rc = -2; goto done; }
+    xo_mask_spectrum(NX, p->xsearch, mx);
     for (int y = 0; y < NY; y++) xo32_fft(z + (size_t)y * NX, NX, -1);
+    /* demean in the row spectrum: z - m1 - i m2 mask has row transform
+     * Zr - m1 NX delta(kx) - i m2 [y in mask] Mx[kx] */
+    {
+      const float m1NX = m1f * (float)NX;
+      for (int y = 0; y < NY; y++) {
+        const int ym = (y >= p->ysearch) && (y < NY - p->ysearch);
+        for (int kx = 0; kx < NX; kx++) {
+          c32 *v = &z[(size_t)y * NX + kx];
+          float re = v->re, im = v->im;
+          if (kx == 0) re = re - m1NX;
+          if (ym) {
+            re = fmaf(m2f, mx[kx].im, re);
+            im = fmaf(-m2f, mx[kx].re, im);
+          }
+          v->re = re;
+          v->im = im;
+        }
+      }
+    }
     for (int x = 0; x < NX; x++) {
       for (int y = 0; y < NY; y++) col[y] = z[(size_t)y * NX + x];
       xo32_fft(col, NY, -1);
@@ -399,6 +440,7 @@ int xo32_patch(const xo_params *p, const c32 *mwin, const c32 *swin, xo_result *
         c3[(size_t)ky * NX + kx] = P;
       }
     }
+    free(mx);
     const int r0 = p->ny_corr / 2, q0 = p->nx_corr / 2;
     /* C3 is exactly Hermitian (C3[-k] == conj(C3[k]) in this arithmetic), so
      * only columns 0..NX/2 are inverse-transformed along y; the others are
