@@ -7,8 +7,10 @@ dy = -7.6 + 3e-4 y, coherence per 512 x 512 block uniform in [0.3, 0.95]),
 locations from get_locations with -nx 100 -ny 400 -xsearch 64 -ysearch 64
 (40,000 patches, 256 x 256 windows, range interpolation x2, 16x high-res).
 
-One step = the whole hot path over all 40,000 locations with the images and the
-window list already resident in HBM and the results left in HBM.
+One step = the whole hot path over all 40,000 locations with the images already
+resident in HBM and the results left in HBM.  The window list is host memory,
+as in xcorr: each step builds the strip plan on the host (sort, merge, upload
+of a few hundred KB per batch) while the previous batches run on the GPU.
 
 Also reported:
   roofline      dominant kernel: algorithmic bytes per launch (524,308 B per
@@ -97,20 +99,19 @@ def main():
     x, y = xg.get_locations(n_range, n_az, nxl, nyl, 64, 64)
     w = xg.make_windows(x, y, p.npx, p.npy)
     n = len(w)
-    dw = torch.from_numpy(w.view(np.int32).reshape(-1, 4).copy()).to(dev)
     dout = torch.zeros(n * xg.RESULT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
     eng = xg.Engine(p, max_batch=min(args.batch, n))
     eng.set_images(m, s)
 
     for _ in range(args.warmup):
-        eng.correlate(dw, out=dout, flags=xg.NO_SYNC)
+        eng.correlate(w, out=dout, flags=xg.NO_SYNC)
     eng.sync()
     torch.cuda.synchronize()
     log("warmup done")
 
     t1 = time.perf_counter()
     for _ in range(args.steps):
-        eng.correlate(dw, out=dout, flags=xg.NO_SYNC)
+        eng.correlate(w, out=dout, flags=xg.NO_SYNC)
     eng.sync()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
@@ -119,7 +120,7 @@ def main():
     log("timed %d steps: %.3f ms/step, %.0f patches/s" % (args.steps, step_s * 1e3, value))
 
     # per-kernel HIP-event timing on the engine's stream (separate, untimed step)
-    eng.correlate(dw, out=dout, flags=xg.TIMING)
+    eng.correlate(w, out=dout, flags=xg.TIMING)
     kt = eng.kernel_times()
     dom = max(kt.items(), key=lambda kv: kv[1][0])
     dom_name, (dom_ms, dom_launches) = dom

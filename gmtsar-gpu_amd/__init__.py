@@ -22,7 +22,7 @@ HOST_LIB_PATH = os.path.join(HERE, "libxcorr_host.so")
 
 TIME, TIME4, FREQ = 0, 1, 2
 CINT16, FLOAT32 = 0, 1
-DEVICE_PTRS, NO_SYNC, TIMING = 1, 2, 4
+DEVICE_PTRS, NO_SYNC, TIMING, HOST_WINDOWS = 1, 2, 4, 8
 
 EXPORTED_SYMBOLS = (
     "xcorr_hip_abi_version", "xcorr_hip_default_params", "xcorr_hip_create", "xcorr_hip_set_images",
@@ -95,6 +95,13 @@ def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
+def _as_windows(windows):
+    w = np.ascontiguousarray(windows)
+    if w.dtype != WINDOW_DTYPE:
+        w = np.ascontiguousarray(w.astype(np.int32).reshape(-1, 4)).view(WINDOW_DTYPE).reshape(-1)
+    return w
+
+
 def _is_torch(x):
     return type(x).__module__.startswith("torch")
 
@@ -140,16 +147,22 @@ class Engine:
 
     def correlate(self, windows, out=None, flags=0):
         """windows: numpy WINDOW_DTYPE array (or (n,4) int32) -> numpy RESULT_DTYPE array.
-        With torch device tensors for windows/out the call stays on the device."""
+        With torch device tensors for windows/out the call stays on the device; host
+        windows with a device `out` tensor write the results on the device."""
+        if _is_torch(out) and not _is_torch(windows):
+            w = _as_windows(windows)
+            self._keep = w            # read until the queued work has consumed it
+            rc = self.lib.xcorr_hip_correlate(self.h, _ptr(w), w.shape[0], ctypes.c_void_p(out.data_ptr()),
+                                              flags | DEVICE_PTRS | HOST_WINDOWS)
+            self._check(rc, "xcorr_hip_correlate")
+            return out
         if _is_torch(windows):
             n = windows.shape[0]
             rc = self.lib.xcorr_hip_correlate(self.h, ctypes.c_void_p(windows.data_ptr()), n,
                                               ctypes.c_void_p(out.data_ptr()), flags | DEVICE_PTRS)
             self._check(rc, "xcorr_hip_correlate")
             return out
-        w = np.ascontiguousarray(windows)
-        if w.dtype != WINDOW_DTYPE:
-            w = np.ascontiguousarray(w.astype(np.int32).reshape(-1, 4)).view(WINDOW_DTYPE).reshape(-1)
+        w = _as_windows(windows)
         n = w.shape[0]
         res = np.zeros(n, RESULT_DTYPE) if out is None else out
         rc = self.lib.xcorr_hip_correlate(self.h, _ptr(w), n, _ptr(res), flags)

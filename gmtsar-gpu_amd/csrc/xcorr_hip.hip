@@ -105,6 +105,15 @@ __device__ __forceinline__ void key_decode(unsigned long long key, float* v, uns
   *idx = ~(unsigned)(key & 0xffffffffull);
 }
 
+// trunc(a * 2^20) for 0 <= a < 2^43, exactly: x = h 2^32 + l with h = trunc(x 2^-32),
+// l = x - h 2^32 (both steps exact), so trunc(x) = h 2^32 + trunc(l).
+__device__ __forceinline__ long long fxp20(float a) {
+  const float x = a * 1048576.0f;
+  const float h = truncf(x * 2.3283064365386963e-10f);
+  const float l = __builtin_fmaf(-h, 4294967296.0f, x);
+  return (long long)(((unsigned long long)(unsigned)h << 32) + (unsigned long long)(unsigned)l);
+}
+
 __device__ __forceinline__ double patch_mean(unsigned long long s, int N) {
   return (double)(long long)s * (1.0 / 1048576.0) / (double)N;
 }
@@ -196,7 +205,7 @@ __global__ void __launch_bounds__(kThreads) k_range_ampl(DevParams P, const void
   for (int e = threadIdx.x; e < nel; e += kThreads) {
     const float a = ab[e];
     out[e] = a;
-    s += (long long)(a * 1048576.0f);
+    s += fxp20(a);
   }
   s = block_sum_i64(s, red);
   if (threadIdx.x == 0) atomicAdd(&sums[b * 2 + img], (unsigned long long)s);
@@ -220,12 +229,12 @@ __device__ __forceinline__ void range_row256(const void* __restrict__ src, int f
   if (format == XCORR_HIP_CINT16) {
     const unsigned* im = (const unsigned*)src;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const int gx = x0 + j + 16 * r;
-      unsigned u = 0u;
-      if (inside || (gy >= 0 && gy < iny && gx >= 0 && gx < inx)) u = im[(size_t)gy * inx + gx];
-      v[r] = make_float2((float)(short)(u & 0xffffu), (float)(short)(u >> 16));
-    }
+      for (int r = 0; r < 16; r++) {
+        const int gx = x0 + j + 16 * r;
+        unsigned u = 0u;
+        if (inside || (gy >= 0 && gy < iny && gx >= 0 && gx < inx)) u = im[(size_t)gy * inx + gx];
+        v[r] = make_float2((float)(short)(u & 0xffffu), (float)(short)(u >> 16));
+      }
   } else {
     const float* im = (const float*)src;
 #pragma unroll
@@ -283,7 +292,7 @@ __global__ void __launch_bounds__(kThreads) k_range256(DevParams P, const void* 
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       outb[(size_t)row * 128 + j + 16 * q] = make_float2(ea[q], oa[q]);
-      s += (long long)(ea[q] * 1048576.0f) + (long long)(oa[q] * 1048576.0f);
+      s += fxp20(ea[q]) + fxp20(oa[q]);
     }
   }
   s = block_sum_i64(s, red);
@@ -330,7 +339,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_stageA256(DevParams P, const vo
         const float2 a = make_float2(ea[q], oa[q]);
         out1[(size_t)row * 128 + j + 16 * q] = a;
         mr[j + 16 * q] = a;
-        s1 += (long long)(ea[q] * 1048576.0f) + (long long)(oa[q] * 1048576.0f);
+        s1 += fxp20(ea[q]) + fxp20(oa[q]);
       }
     }
     {
@@ -342,7 +351,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_stageA256(DevParams P, const vo
         const float2 a = make_float2(ea[q], oa[q]);
         if (trow && q >= 2 && q < 6) out2[(size_t)row * 128 + j + 16 * q] = a;
         tile[j + 16 * q] = a;                      // the tile doubles as the slave line buffer
-        s2 += (long long)(ea[q] * 1048576.0f) + (long long)(oa[q] * 1048576.0f);
+        s2 += fxp20(ea[q]) + fxp20(oa[q]);
       }
     }
     __syncthreads();
@@ -364,6 +373,219 @@ __global__ void __launch_bounds__(kThreads, 3) k_stageA256(DevParams P, const vo
   s1 = block_sum_i64(s1, red);
   s2 = block_sum_i64(s2, red);
   if (threadIdx.x == 0) {
+    sums[2 * b] = (unsigned long long)s1;
+    sums[2 * b + 1] = (unsigned long long)s2;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Strips (npx = npy = 256, ri = 2, frequency mode).  Windows that share the
+// master column, the slave column and the slave line offset see the same
+// range-interpolated lines, so a strip computes each line once: amplitudes,
+// their exact fixed-point sum, and the row transforms of z = amp1 + i amp2
+// mask for the two mask states a line can have inside a patch (masked rows
+// [ys, npy - ys), unmasked rows elsewhere).  A patch is a run of npy strip
+// lines starting at its row base.  The arithmetic of every value is that of
+// the per-patch path.
+// --------------------------------------------------------------------------
+constexpr int kStripBlockLines = 64;      // strip lines per block (16 groups x 4)
+
+// Raw samples x0 + j + 16 r of line gy (one dword per sample in both formats),
+// zero outside the image.
+__device__ __forceinline__ void load_line256(const unsigned* __restrict__ im, int inx, int iny, bool inside, int gy,
+                                             int x0, int j, unsigned (&u)[16]) {
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int gx = x0 + j + 16 * r;
+    unsigned w = 0u;
+    if (inside || (gy >= 0 && gy < iny && gx >= 0 && gx < inx)) w = im[(size_t)gy * inx + gx];
+    u[r] = w;
+  }
+}
+
+// Range interpolation of one held line (range_row256 after the gather); the
+// phase factors come from LDS.
+__device__ __forceinline__ void interp_line256(int format, const unsigned (&u)[16], int j, float2* tile,
+                                               const float2 (&t)[15], const float2* sph, float (&ea)[8],
+                                               float (&oa)[8]) {
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++)
+    v[r] = format == XCORR_HIP_CINT16 ? make_float2((float)(short)(u[r] & 0xffffu), (float)(short)(u[r] >> 16))
+                                      : make_float2(__uint_as_float(u[r]), 0.0f);
+#pragma unroll
+  for (int q = 0; q < 8; q++) ea[q] = ampl(v[q + 4]);
+  fft256_group<-1>(v, j, tile, t);
+#pragma unroll
+  for (int r = 0; r < 16; r++) v[r] = cmul(v[r], sph[j + 16 * r]);
+  fft256_group<+1>(v, j, tile, t);
+  const float sc = 1.0f / 256.0f;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    float2 o = v[q + 4];
+    o.x = o.x * sc;
+    o.y = o.y * sc;
+    oa[q] = ampl(o);
+  }
+}
+
+struct StripDesc {
+  int mx0, my0, sx0, sy0;                 // master / slave column and line of strip line 0
+  int line0, nlines;                      // first global line (a multiple of kStripBlockLines), count
+  int pad0, pad1;
+};
+
+// needM[l] / needU[l] = 1 when some patch of the batch holds line l as a masked / unmasked row.
+__global__ void k_need_clear(unsigned char* __restrict__ needM, unsigned char* __restrict__ needU, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    needM[i] = 0;
+    needU[i] = 0;
+  }
+}
+__global__ void k_need_mark(DevParams P, const int* __restrict__ rb, unsigned char* __restrict__ needM,
+                            unsigned char* __restrict__ needU, int nb) {
+  const int b = blockIdx.x;
+  if (b >= nb) return;
+  for (int y = threadIdx.x; y < P.NY; y += blockDim.x) {
+    const size_t l = (size_t)rb[b] + y;
+    if (y >= P.ys && y < P.NY - P.ys) needM[l] = 1;
+    else needU[l] = 1;
+  }
+}
+
+__device__ __forceinline__ long long group16_sum_i64(long long v) {
+  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// One block per kStripBlockLines strip lines; 16 groups of 16 lanes, one line
+// per group and iteration.  Out: amp1 (all columns), amp2 (template columns of
+// masked lines), lsum (per-line fixed-point sums), Zm / Zu (row spectra of
+// masked / unmasked lines, where needed).
+__global__ void __launch_bounds__(kThreads, 3) k_stripA256(DevParams P, const void* __restrict__ master,
+                                                        const void* __restrict__ slave,
+                                                        const StripDesc* __restrict__ strips,
+                                                        const int* __restrict__ blk_strip,
+                                                        const unsigned char* __restrict__ needM,
+                                                        const unsigned char* __restrict__ needU,
+                                                        float* __restrict__ amp1, float* __restrict__ amp2,
+                                                        long long* __restrict__ lsum, float2* __restrict__ Zm,
+                                                        float2* __restrict__ Zu, const float2* __restrict__ tabs) {
+  __shared__ float2 tiles[16 * kT256Tile];
+  __shared__ float2 mrow[16 * 128];                // each group's master amplitude line (column pairs)
+  __shared__ float2 sph[256];                      // range-interpolation phase factors
+  const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+  const StripDesc S = strips[blk_strip[blockIdx.x]];
+  float2* tile = tiles + g * kT256Tile;
+  float2* mr = mrow + g * 128;
+  float2 t[15];
+  load_tw256(tabs + P.tw_nx, j, t);
+  for (int k = threadIdx.x; k < 256; k += kThreads) sph[k] = tabs[P.phi + k];
+  const unsigned* im_m = (const unsigned*)master;
+  const unsigned* im_s = (const unsigned*)slave;
+  const bool col_m = S.mx0 >= 0 && S.mx0 + 256 <= P.m_nx;
+  const bool col_s = S.sx0 >= 0 && S.sx0 + 256 <= P.s_nx;
+  const int first = blockIdx.x * kStripBlockLines - S.line0;     // strip-relative line of this block
+  const int nit = min(kStripBlockLines, S.nlines - first + 15) >> 4;
+  // the samples of a group's next line are loaded one iteration ahead
+  unsigned um[16], us[16];
+  {
+    const int gym = S.my0 + first + g, gys = S.sy0 + first + g;
+    load_line256(im_m, P.m_nx, P.m_ny, col_m && gym >= 0 && gym < P.m_ny, gym, S.mx0, j, um);
+    load_line256(im_s, P.s_nx, P.s_ny, col_s && gys >= 0 && gys < P.s_ny, gys, S.sx0, j, us);
+  }
+  __syncthreads();
+  for (int it = 0; it < nit; it++) {
+    const int i = first + 16 * it + g;                            // strip-relative line of this group
+    const bool act = i < S.nlines;
+    const bool more = it + 1 < nit;
+    const size_t l = (size_t)(S.line0 + i);
+    const bool wm = act && needM[l], wu = act && needU[l];
+    {
+      float ea[8], oa[8];
+      const unsigned cur[16] = {um[0], um[1], um[2], um[3], um[4], um[5], um[6], um[7],
+                                um[8], um[9], um[10], um[11], um[12], um[13], um[14], um[15]};
+      if (more) {
+        const int gy = S.my0 + i + 16;
+        load_line256(im_m, P.m_nx, P.m_ny, col_m && gy >= 0 && gy < P.m_ny, gy, S.mx0, j, um);
+      }
+      interp_line256(P.format, cur, j, tile, t, sph, ea, oa);
+      long long s = 0;
+      float2* o = (float2*)(amp1 + l * 256);
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const float2 a = make_float2(ea[q], oa[q]);
+        if (act) o[j + 16 * q] = a;
+        mr[j + 16 * q] = a;
+        s += fxp20(ea[q]) + fxp20(oa[q]);
+      }
+      s = group16_sum_i64(s);
+      if (act && j == 0) lsum[2 * l] = s;
+    }
+    {
+      float ea[8], oa[8];
+      const unsigned cur[16] = {us[0], us[1], us[2], us[3], us[4], us[5], us[6], us[7],
+                                us[8], us[9], us[10], us[11], us[12], us[13], us[14], us[15]};
+      if (more) {
+        const int gy = S.sy0 + i + 16;
+        load_line256(im_s, P.s_nx, P.s_ny, col_s && gy >= 0 && gy < P.s_ny, gy, S.sx0, j, us);
+      }
+      interp_line256(P.format, cur, j, tile, t, sph, ea, oa);
+      long long s = 0;
+      float2* o = (float2*)(amp2 + l * 256);
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const float2 a = make_float2(ea[q], oa[q]);
+        if (wm && q >= 2 && q < 6) o[j + 16 * q] = a;
+        tile[j + 16 * q] = a;                      // the tile doubles as the slave line buffer
+        s += fxp20(ea[q]) + fxp20(oa[q]);
+      }
+      s = group16_sum_i64(s);
+      if (act && j == 0) lsum[2 * l + 1] = s;
+    }
+    const bool anyM = __syncthreads_or(wm);
+    const float* m1 = (const float*)mr;
+    const float* a2 = (const float*)tile;
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int x = j + 16 * r;
+      const bool in = x >= P.xs && x < 256 - P.xs;
+      v[r] = make_float2(m1[x], in ? a2[x] : 0.0f);
+    }
+    const bool anyU = __syncthreads_or(wu);
+    if (anyM) {
+      fft256_group<-1>(v, j, tile, t);
+      if (wm)
+#pragma unroll
+        for (int r = 0; r < 16; r++) Zm[l * 256 + j + 16 * r] = v[r];
+    }
+    if (anyU) {
+#pragma unroll
+      for (int r = 0; r < 16; r++) v[r] = make_float2(m1[j + 16 * r], 0.0f);
+      fft256_group<-1>(v, j, tile, t);
+      if (wu)
+#pragma unroll
+        for (int r = 0; r < 16; r++) Zu[l * 256 + j + 16 * r] = v[r];
+    }
+  }
+}
+
+// Patch sums from the per-line sums: one wave per patch, exact in int64.
+__global__ void k_patch_sums(DevParams P, const int* __restrict__ rb, const long long* __restrict__ lsum,
+                             unsigned long long* __restrict__ sums, int nb) {
+  const int b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (b >= nb) return;
+  long long s1 = 0, s2 = 0;
+  for (int y = lane; y < P.NY; y += 64) {
+    const size_t l = (size_t)rb[b] + y;
+    s1 += lsum[2 * l];
+    s2 += lsum[2 * l + 1];
+  }
+  s1 = wave_sum_i64(s1);
+  s2 = wave_sum_i64(s2);
+  if (lane == 0) {
     sums[2 * b] = (unsigned long long)s1;
     sums[2 * b + 1] = (unsigned long long)s2;
   }
@@ -425,7 +647,10 @@ __device__ __forceinline__ float2 demean_row_bin(float2 v, int kx, bool ym, floa
 // owns columns T = [16p, 16p+16) and their mirrors M = {npx - 16p - g} (with
 // npx/2 in place of the duplicate column 0 for p = 0), so every column is
 // transformed once and C3 needs no second pass over Z.  Requires npx >= 32.
+// Row spectra come from Z[b] (rb null) or, for strips, from line rb[b] + y of
+// Zm (masked rows) or Zu.
 __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const float2* __restrict__ Z,
+                                                         const float2* __restrict__ Zu, const int* __restrict__ rb,
                                                          const unsigned long long* __restrict__ sums,
                                                          float2* __restrict__ D, const float2* __restrict__ tabs) {
   __shared__ float2 T[256 * kT256Pitch];
@@ -434,7 +659,11 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
   const int NX = P.NX;
   const int pairs = NX >> 5;
   const int p = blockIdx.x % pairs, b = blockIdx.x / pairs;
-  const float2* zb = Z + (size_t)b * P.N;
+  const size_t row0 = rb ? (size_t)rb[b] : (size_t)b * 256;
+  auto zrow = [&](int y) -> const float2* {
+    const float2* base = (rb && !(y >= P.ys && y < 256 - P.ys)) ? Zu : Z;
+    return base + (row0 + y) * NX;
+  };
   const int kxT = 16 * p + g;
   const int kxM = (p == 0 && g == 0) ? NX / 2 : NX - 16 * p - g;
   const bool self = (p == 0 && g == 0);                      // columns 0 and npx/2 mirror onto themselves
@@ -443,21 +672,34 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
   const float m1NX = (float)patch_mean(sums[2 * b], P.N) * (float)NX;
   const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
   float2 vT[16], vM[16];
-  // stage and load the T columns, then the M columns (demeaned on the way in)
-  for (int e = threadIdx.x; e < 256 * 16; e += kThreads) {
-    const int y = e >> 4, kx = 16 * p + (e & 15);
-    T[y * kT256Pitch + (e & 15)] = demean_row_bin(zb[(size_t)y * NX + kx], kx, y >= P.ys && y < 256 - P.ys, m1NX,
-                                                  m2f, tabs[P.mk_x + kx]);
-  }
-  __syncthreads();
+  // Thread (y0, c) loads column c of the T and M tiles at rows y0 + 16 k: all
+  // 32 loads are in flight together, then the demeaned values go through LDS.
+  {
+    const int c = threadIdx.x & 15, y0 = threadIdx.x >> 4;
+    const int kxt = 16 * p + c;
+    const int kxm = (p == 0 && c == 0) ? NX / 2 : NX - 16 * p - c;
+    const float2 mkt = tabs[P.mk_x + kxt], mkm = tabs[P.mk_x + kxm];
+    float2 rT[16], rM[16];
 #pragma unroll
-  for (int r = 0; r < 16; r++) vT[r] = T[(j + 16 * r) * kT256Pitch + g];
-  __syncthreads();
-  for (int e = threadIdx.x; e < 256 * 16; e += kThreads) {
-    const int c = e & 15, y = e >> 4;
-    const int mx = (p == 0 && c == 0) ? NX / 2 : NX - 16 * p - c;
-    T[y * kT256Pitch + c] = demean_row_bin(zb[(size_t)y * NX + mx], mx, y >= P.ys && y < 256 - P.ys, m1NX, m2f,
-                                           tabs[P.mk_x + mx]);
+    for (int k = 0; k < 16; k++) {
+      const float2* zr = zrow(y0 + 16 * k);
+      rT[k] = zr[kxt];
+      rM[k] = zr[kxm];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int y = y0 + 16 * k;
+      T[y * kT256Pitch + c] = demean_row_bin(rT[k], kxt, y >= P.ys && y < 256 - P.ys, m1NX, m2f, mkt);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) vT[r] = T[(j + 16 * r) * kT256Pitch + g];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int y = y0 + 16 * k;
+      T[y * kT256Pitch + c] = demean_row_bin(rM[k], kxm, y >= P.ys && y < 256 - P.ys, m1NX, m2f, mkm);
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -721,7 +963,11 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf(DevParams P, const fl
 // do_freq_xcorr.cu:150-200, do_time_int_xcorr.cu:71-100, highres_corr.cu:23-79,
 // conv_ampl_demean.cu:75-95, fft_interpolate_routines.cu:118-147
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* __restrict__ amp,
+// Amplitudes of patch b: rows y of A1 / A2 at (rb[b] + y) npx (strips) or
+// b pstride + y npx.  The result goes to out[perm[b]] (or out[b]).
+__global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* __restrict__ A1,
+                                                     const float* __restrict__ A2, const int* __restrict__ rb,
+                                                     long long pstride, const int* __restrict__ perm,
                                                      const unsigned long long* __restrict__ sums,
                                                      const float* __restrict__ surf,
                                                      const double* __restrict__ surfd,
@@ -757,21 +1003,39 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
     const int ipeak = ip - nyc / 2, jpeak = jp - nxc / 2;
     const float m1f = (float)patch_mean(sums[2 * b], P.N);
     const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
-    const float* a1 = amp + (size_t)(2 * b) * P.N;
-    const float* a2 = a1 + P.N;
+    const size_t pb = rb ? (size_t)rb[b] * NX : (size_t)b * pstride;
+    const float* a1 = A1 + pb;
+    const float* a2 = A2 + pb;
     // exact integer sums in FP64: |products| < 2^35 and every partial sum < 2^53,
     // so the accumulation order cannot change them
     double num = 0.0, d1 = 0.0, d2 = 0.0;
     const int ncx = P.nx_corr, nt = P.nx_corr * P.ny_corr;
     const float* arow = a1 + (size_t)(P.ys + ipeak) * NX + P.xs + jpeak;
     const float* brow = a2 + (size_t)P.ys * NX + P.xs;
-    for (int e = threadIdx.x; e < nt; e += kThreads) {
-      const int r = e / ncx, c = e - r * ncx;
-      const double av = (double)(int)(arow[(size_t)r * NX + c] - m1f);
-      const double bv = (double)(int)(brow[(size_t)r * NX + c] - m2f);
-      num = __builtin_fma(av, bv, num);
-      d1 = __builtin_fma(av, av, d1);
-      d2 = __builtin_fma(bv, bv, d2);
+    const int lg = __builtin_ctz(ncx);          // nx_corr is a power of two
+    for (int e0 = 0; e0 < nt; e0 += 8 * kThreads) {   // 8 independent loads per thread in flight
+      float ra[8], rb8[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int e = e0 + k * kThreads + threadIdx.x;
+        ra[k] = 0.0f;
+        rb8[k] = 0.0f;
+        if (e < nt) {
+          const size_t o = (size_t)(e >> lg) * NX + (e & (ncx - 1));
+          ra[k] = arow[o];
+          rb8[k] = brow[o];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        if (e0 + k * kThreads + (int)threadIdx.x < nt) {
+          const double av = (double)(int)(ra[k] - m1f);
+          const double bv = (double)(int)(rb8[k] - m2f);
+          num = __builtin_fma(av, bv, num);
+          d1 = __builtin_fma(av, av, d1);
+          d2 = __builtin_fma(bv, bv, d2);
+        }
+      }
     }
     num = block_sum_f64(num, redd);
     d1 = block_sum_f64(d1, redd);
@@ -889,7 +1153,7 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
     res.hip = hp;
     res.hjp = hq;
   }
-  if (threadIdx.x == 0) out[b] = res;
+  if (threadIdx.x == 0) out[perm ? perm[b] : b] = res;
 }
 
 // --------------------------------------------------------------------------
@@ -1145,10 +1409,27 @@ struct xcorr_hip_ctx {
   xcorr_hip_window* d_win = nullptr;
   xcorr_hip_result* d_out = nullptr;
   float2* tabs = nullptr;
+  // strip mode (npx = npy = 256, ri = 2, frequency): per-line buffers
+  bool strips = false;
+  int max_lines = 0;                      // line capacity of a batch (multiple of kStripBlockLines)
+  float* amp1s = nullptr;
+  float* amp2s = nullptr;
+  long long* lsum = nullptr;
+  float2* Zu = nullptr;                   // unmasked row spectra (masked ones in Z)
+  unsigned char* need = nullptr;          // needM [max_lines] then needU [max_lines]
+  int* d_plan = nullptr;                  // rb | perm | blk_strip | strips, per batch
+  void* h_stage[2] = {nullptr, nullptr};  // pinned staging of the plan, two batches in flight
+  hipEvent_t ev_stage[2] = {nullptr, nullptr};
+  bool stage_used[2] = {false, false};
+  int stage_next = 0;
+  size_t plan_bytes = 0;
+  std::vector<int> order;                 // windows sorted by strip key
+  std::vector<xcorr_hip_window> hwin;     // host copy of device windows
+  std::vector<xcorr_hip_result> hres;     // batch results before the host scatter
   std::string err;
   std::vector<KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int last_n = 0;
+  std::vector<int> last_perm, last_rb;    // last batch: location index and row base of each slot
 };
 
 static std::string g_create_err;
@@ -1209,12 +1490,25 @@ static int validate(const xcorr_hip_params* p, std::string* why) {
 
 static void free_scratch(xcorr_hip_ctx* c) {
   void* ptrs[] = {c->amp, c->sums, c->Z, c->D, c->surf, c->surfd, c->peak, c->partial, c->tpeak,
-                  c->tcave, c->d_win, c->d_out, c->tabs};
+                  c->tcave, c->d_win, c->d_out, c->tabs, c->amp1s, c->amp2s, c->lsum, c->Zu, c->need, c->d_plan};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  for (int i = 0; i < 2; i++) {
+    if (c->h_stage[i]) (void)hipHostFree(c->h_stage[i]);
+    if (c->ev_stage[i]) (void)hipEventDestroy(c->ev_stage[i]);
+    c->h_stage[i] = nullptr;
+    c->ev_stage[i] = nullptr;
+  }
   c->amp = nullptr; c->sums = nullptr; c->Z = nullptr; c->D = nullptr; c->surf = nullptr;
   c->surfd = nullptr; c->peak = nullptr; c->partial = nullptr; c->tpeak = nullptr; c->tcave = nullptr;
-  c->d_win = nullptr; c->d_out = nullptr; c->tabs = nullptr;
+  c->d_win = nullptr; c->d_out = nullptr; c->tabs = nullptr; c->amp1s = nullptr; c->amp2s = nullptr;
+  c->lsum = nullptr; c->Zu = nullptr; c->need = nullptr; c->d_plan = nullptr;
+}
+
+// Plan layout (ints): rb[max_batch] | perm[max_batch] | blk_strip[max_lines / 64] | StripDesc[max_batch]
+static size_t plan_ints(const xcorr_hip_ctx* c) {
+  return 2 * (size_t)c->max_batch + (size_t)c->max_lines / kStripBlockLines +
+         (size_t)c->max_batch * (sizeof(StripDesc) / sizeof(int));
 }
 
 extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int device, xcorr_hip_ctx** out) {
@@ -1305,10 +1599,29 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   auto alloc = [&](void** ptr, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(ptr, bytes ? bytes : 16);
   };
-  alloc((void**)&c->amp, B * 2 * N * sizeof(float));
+  c->strips = d.NX == 256 && d.NY == 256 && d.ri == 2 && p->corr_flag == XCORR_HIP_FREQ;
+  if (c->strips) {
+    // half the lines of unshared windows: overlapping windows need far fewer
+    c->max_lines = std::max(max_batch * (d.NY / 2), d.NY + kStripBlockLines);
+    c->max_lines = (c->max_lines + kStripBlockLines - 1) / kStripBlockLines * kStripBlockLines;
+    const size_t L = (size_t)c->max_lines;
+    alloc((void**)&c->amp1s, L * d.NX * sizeof(float));
+    alloc((void**)&c->amp2s, L * d.NX * sizeof(float));
+    alloc((void**)&c->lsum, L * 2 * sizeof(long long));
+    alloc((void**)&c->Zu, L * d.NX * sizeof(float2));
+    alloc((void**)&c->need, L * 2);
+    c->plan_bytes = plan_ints(c) * sizeof(int);
+    alloc((void**)&c->d_plan, c->plan_bytes);
+    for (int i = 0; i < 2; i++) {
+      if (e == hipSuccess) e = hipHostMalloc(&c->h_stage[i], c->plan_bytes, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_stage[i], hipEventDisableTiming);
+    }
+  } else {
+    alloc((void**)&c->amp, B * 2 * N * sizeof(float));
+  }
   alloc((void**)&c->sums, B * 2 * sizeof(unsigned long long));
   if (p->corr_flag == XCORR_HIP_FREQ) {
-    alloc((void**)&c->Z, B * N * sizeof(float2));
+    alloc((void**)&c->Z, (c->strips ? (size_t)c->max_lines * d.NX : B * N) * sizeof(float2));
     alloc((void**)&c->D, B * d.nyc * d.NX * sizeof(float2));
     alloc((void**)&c->surf, B * ns * sizeof(float));
     alloc((void**)&c->partial, B * d.groups4 * sizeof(double));
@@ -1426,6 +1739,8 @@ static int launch_timed(xcorr_hip_ctx* c, bool timing, const char* name, F&& f) 
 }
 
 static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xcorr_hip_result* d_out, bool timing);
+static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_hip_result* out, bool dev_out,
+                      bool timing);
 
 extern "C" int xcorr_hip_correlate(xcorr_hip_ctx* c, const xcorr_hip_window* win, int n, xcorr_hip_result* out,
                                    unsigned flags) {
@@ -1433,24 +1748,173 @@ extern "C" int xcorr_hip_correlate(xcorr_hip_ctx* c, const xcorr_hip_window* win
   if (n < 0 || (n > 0 && (!win || !out))) return fail(c, XCORR_HIP_EINVAL, "correlate: bad arguments");
   if (!c->images) return fail(c, XCORR_HIP_ESTATE, "correlate: no images set");
   HIPCK(c, hipSetDevice(c->device));
-  const bool dev = (flags & XCORR_HIP_DEVICE_PTRS) != 0;
+  const bool dev_out = (flags & XCORR_HIP_DEVICE_PTRS) != 0;
+  const bool dev_win = dev_out && !(flags & XCORR_HIP_HOST_WINDOWS);
   const bool timing = (flags & XCORR_HIP_TIMING) != 0;
   if (timing)
     for (auto& t : c->timers) { t.ms = 0.0f; t.launches = 0; }
-  for (int off = 0; off < n; off += c->max_batch) {
-    const int nb = std::min(c->max_batch, n - off);
-    const xcorr_hip_window* dw = dev ? win + off : c->d_win;
-    xcorr_hip_result* dout = dev ? out + off : c->d_out;
-    if (!dev) HIPCK(c, hipMemcpyAsync(c->d_win, win + off, nb * sizeof(xcorr_hip_window), hipMemcpyHostToDevice, c->stream));
-    int rc = run_batch(c, dw, nb, dout, timing);
-    if (rc) return rc;
-    if (!dev) {
-      HIPCK(c, hipMemcpyAsync(out + off, c->d_out, nb * sizeof(xcorr_hip_result), hipMemcpyDeviceToHost, c->stream));
+  if (c->strips) {
+    const xcorr_hip_window* hw = win;
+    if (dev_win && n > 0) {     // the plan is built on the host
+      c->hwin.resize(n);
+      HIPCK(c, hipMemcpyAsync(c->hwin.data(), win, (size_t)n * sizeof(xcorr_hip_window), hipMemcpyDeviceToHost,
+                              c->stream));
       HIPCK(c, hipStreamSynchronize(c->stream));
+      hw = c->hwin.data();
+    }
+    const int rc = run_strips(c, hw, n, out, dev_out, timing);
+    if (rc) return rc;
+  } else {
+    for (int off = 0; off < n; off += c->max_batch) {
+      const int nb = std::min(c->max_batch, n - off);
+      const xcorr_hip_window* dw = dev_win ? win + off : c->d_win;
+      xcorr_hip_result* dout = dev_out ? out + off : c->d_out;
+      if (!dev_win)
+        HIPCK(c, hipMemcpyAsync(c->d_win, win + off, nb * sizeof(xcorr_hip_window), hipMemcpyHostToDevice,
+                                c->stream));
+      int rc = run_batch(c, dw, nb, dout, timing);
+      if (rc) return rc;
+      if (!dev_out) {
+        HIPCK(c, hipMemcpyAsync(out + off, c->d_out, nb * sizeof(xcorr_hip_result), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(c, hipStreamSynchronize(c->stream));
+      }
+      c->last_perm.resize(nb);
+      for (int i = 0; i < nb; i++) c->last_perm[i] = off + i;
+      c->last_rb.clear();
     }
   }
-  c->last_n = std::min(n, c->max_batch);
   if (!(flags & XCORR_HIP_NO_SYNC)) HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
+
+// Strip mode.  Windows are sorted by (master column, slave column, slave line
+// offset, master line); a batch takes them in that order and merges each run
+// of overlapping windows with one key into a strip.  Per batch the host
+// uploads the row base and location index of every patch, the strip of every
+// line block and the strips themselves.
+static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_hip_result* out, bool dev_out,
+                      bool timing) {
+  const DevParams& d = c->dp;
+  const int NY = d.NY, mb = c->max_batch, ML = c->max_lines, blkcap = ML / kStripBlockLines;
+  std::vector<int>& ord = c->order;
+  ord.resize(n);
+  for (int i = 0; i < n; i++) ord[i] = i;
+  std::sort(ord.begin(), ord.end(), [w](int a, int b) {
+    const xcorr_hip_window &A = w[a], &B = w[b];
+    if (A.mx0 != B.mx0) return A.mx0 < B.mx0;
+    if (A.sx0 != B.sx0) return A.sx0 < B.sx0;
+    const long long da = (long long)A.sy0 - A.my0, db = (long long)B.sy0 - B.my0;
+    if (da != db) return da < db;
+    if (A.my0 != B.my0) return A.my0 < B.my0;
+    return a < b;
+  });
+  const long long pad_ny = round_up(NY, kStripBlockLines);
+  for (int pos = 0; pos < n;) {
+    const int si = c->stage_next;
+    c->stage_next ^= 1;
+    if (c->stage_used[si]) HIPCK(c, hipEventSynchronize(c->ev_stage[si]));
+    int* h = (int*)c->h_stage[si];
+    int* rb = h;
+    int* perm = h + mb;
+    int* bs = h + 2 * mb;
+    StripDesc* st = (StripDesc*)(h + 2 * mb + blkcap);
+    int nb = 0, ns = 0;
+    long long L = 0, cur_end = 0;
+    while (pos < n && nb < mb) {
+      const xcorr_hip_window& q = w[ord[pos]];
+      bool ext = false;
+      if (ns > 0) {
+        const StripDesc& S = st[ns - 1];
+        ext = S.mx0 == q.mx0 && S.sx0 == q.sx0 && (long long)S.sy0 - S.my0 == (long long)q.sy0 - q.my0 &&
+              (long long)q.my0 <= cur_end;
+      }
+      if (ext) {
+        StripDesc& S = st[ns - 1];
+        const long long end = std::max(cur_end, (long long)q.my0 + NY);
+        const long long lines = S.line0 + round_up(end - S.my0, kStripBlockLines);
+        if (lines > ML) break;
+        cur_end = end;
+        S.nlines = (int)(end - S.my0);
+        L = lines;
+        rb[nb] = S.line0 + (q.my0 - S.my0);
+      } else {
+        if (L + pad_ny > ML) break;
+        StripDesc& S = st[ns++];
+        S.mx0 = q.mx0; S.my0 = q.my0; S.sx0 = q.sx0; S.sy0 = q.sy0;
+        S.line0 = (int)L; S.nlines = NY; S.pad0 = S.pad1 = 0;
+        cur_end = (long long)q.my0 + NY;
+        rb[nb] = (int)L;
+        L += pad_ny;
+      }
+      perm[nb] = ord[pos];
+      nb++;
+      pos++;
+    }
+    if (nb == 0) return fail(c, XCORR_HIP_EINVAL, "correlate: a window does not fit the line capacity");
+    for (int k = 0; k < ns; k++) {
+      const int b0 = st[k].line0 / kStripBlockLines;
+      const int b1 = (int)((st[k].line0 + round_up(st[k].nlines, kStripBlockLines)) / kStripBlockLines);
+      for (int blk = b0; blk < b1; blk++) bs[blk] = k;
+    }
+    const int nl = (int)L, nblk = nl / kStripBlockLines;
+    const size_t bytes = (size_t)(2 * mb + blkcap) * sizeof(int) + (size_t)ns * sizeof(StripDesc);
+    HIPCK(c, hipMemcpyAsync(c->d_plan, h, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipEventRecord(c->ev_stage[si], c->stream));
+    c->stage_used[si] = true;
+    c->last_perm.assign(perm, perm + nb);
+    c->last_rb.assign(rb, rb + nb);
+    const int* d_rb = c->d_plan;
+    const int* d_perm = c->d_plan + mb;
+    const int* d_bs = c->d_plan + 2 * mb;
+    const StripDesc* d_st = (const StripDesc*)(c->d_plan + 2 * mb + blkcap);
+    unsigned char* needM = c->need;
+    unsigned char* needU = c->need + ML;
+    int rc = launch_timed(c, timing, "k_need", [&] {
+      hipLaunchKernelGGL(k_need_clear, dim3((nl + 255) / 256), dim3(256), 0, c->stream, needM, needU, nl);
+      hipLaunchKernelGGL(k_need_mark, dim3(nb), dim3(256), 0, c->stream, d, d_rb, needM, needU, nb);
+    });
+    if (rc) return rc;
+    rc = launch_timed(c, timing, "k_stripA256", [&] {
+      hipLaunchKernelGGL(k_stripA256, dim3(nblk), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_st,
+                         d_bs, (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s, c->amp2s, c->lsum,
+                         c->Z, c->Zu, (const float2*)c->tabs);
+    });
+    if (rc) return rc;
+    rc = launch_timed(c, timing, "k_patch_sums", [&] {
+      hipLaunchKernelGGL(k_patch_sums, dim3((nb + 3) / 4), dim3(256), 0, c->stream, d, d_rb,
+                         (const long long*)c->lsum, c->sums, nb);
+    });
+    if (rc) return rc;
+    rc = launch_timed(c, timing, "k_colpass256", [&] {
+      hipLaunchKernelGGL(k_colpass256, dim3(nb * (d.NX / 32)), dim3(kThreads), 0, c->stream, d, (const float2*)c->Z,
+                         (const float2*)c->Zu, d_rb, (const unsigned long long*)c->sums, c->D,
+                         (const float2*)c->tabs);
+    });
+    if (rc) return rc;
+    rc = launch_timed(c, timing, "k_rowifft_surf256", [&] {
+      hipLaunchKernelGGL(k_rowifft_surf256, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)c->D, c->surf,
+                         c->peak, c->partial, (const float2*)c->tabs);
+    });
+    if (rc) return rc;
+    const size_t sh5 = d.interp_flag ? (size_t)d.Lx * 16 * sizeof(float2) : 16;
+    rc = launch_timed(c, timing, "k_finish", [&] {
+      hipLaunchKernelGGL(k_finish, dim3(nb), dim3(kThreads), sh5, c->stream, d, (const float*)c->amp1s,
+                         (const float*)c->amp2s, d_rb, 0LL, dev_out ? d_perm : (const int*)nullptr,
+                         (const unsigned long long*)c->sums, (const float*)c->surf, (const double*)nullptr,
+                         (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)nullptr,
+                         (const double*)nullptr, dev_out ? out : c->d_out, (const float2*)c->tabs);
+    });
+    if (rc) return rc;
+    if (!dev_out) {
+      c->hres.resize(nb);
+      HIPCK(c, hipMemcpyAsync(c->hres.data(), c->d_out, nb * sizeof(xcorr_hip_result), hipMemcpyDeviceToHost,
+                              c->stream));
+      HIPCK(c, hipStreamSynchronize(c->stream));
+      for (int i = 0; i < nb; i++) out[c->last_perm[i]] = c->hres[i];
+    }
+  }
   return 0;
 }
 
@@ -1502,7 +1966,8 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
     if (col256 && d.NX >= 32) {
       rc = launch_timed(c, timing, "k_colpass256", [&] {
         hipLaunchKernelGGL(k_colpass256, dim3(nb * (d.NX / 32)), dim3(kThreads), 0, c->stream, d,
-                           (const float2*)c->Z, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs);
+                           (const float2*)c->Z, (const float2*)nullptr, (const int*)nullptr,
+                           (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs);
       });
     } else {
       rc = launch_timed(c, timing, "k_colfft", [&] {
@@ -1535,6 +2000,7 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
   const size_t sh5 = d.interp_flag ? (size_t)d.Lx * 16 * sizeof(float2) : 16;
   rc = launch_timed(c, timing, "k_finish", [&] {
     hipLaunchKernelGGL(k_finish, dim3(nb), dim3(kThreads), sh5, c->stream, d, (const float*)c->amp,
+                       (const float*)c->amp + d.N, (const int*)nullptr, (long long)2 * d.N, (const int*)nullptr,
                        (const unsigned long long*)c->sums, (const float*)c->surf, (const double*)c->surfd,
                        (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)c->tpeak,
                        (const double*)c->tcave, d_out, (const float2*)c->tabs);
@@ -1577,7 +2043,11 @@ extern "C" int xcorr_hip_kernel_times(xcorr_hip_ctx* c, const char** names, floa
 
 extern "C" long xcorr_hip_dump(xcorr_hip_ctx* c, int what, int slot, void* host, size_t bytes) {
   if (!c || !host) return XCORR_HIP_EINVAL;
-  if (slot < 0 || slot >= c->max_batch) return fail(c, XCORR_HIP_EINVAL, "dump: slot out of range");
+  const int loc = slot;
+  slot = -1;
+  for (size_t i = 0; i < c->last_perm.size(); i++)
+    if (c->last_perm[i] == loc) slot = (int)i;
+  if (slot < 0) return fail(c, XCORR_HIP_EINVAL, "dump: location not in the last batch");
   HIPCK(c, hipSetDevice(c->device));
   HIPCK(c, hipStreamSynchronize(c->stream));
   const DevParams& d = c->dp;
@@ -1588,7 +2058,10 @@ extern "C" long xcorr_hip_dump(xcorr_hip_ctx* c, int what, int slot, void* host,
   switch (what) {
     case 0:
     case 1:
-      src = c->amp + ((size_t)slot * 2 + what) * d.N;
+      if (c->strips)   // npy consecutive strip lines from the row base
+        src = (what ? c->amp2s : c->amp1s) + (size_t)c->last_rb[slot] * d.NX;
+      else
+        src = c->amp + ((size_t)slot * 2 + what) * d.N;
       nbytes = (size_t)d.N * sizeof(float);
       break;
     case 2:

@@ -48,6 +48,7 @@ extern "C" {
 #define XCORR_HIP_DEVICE_PTRS 1u   /* image (set_images) or window/result (correlate) pointers are device memory */
 #define XCORR_HIP_NO_SYNC 2u       /* correlate: return without waiting for the stream (device results only) */
 #define XCORR_HIP_TIMING 4u        /* correlate: record per-kernel HIP events (xcorr_hip_kernel_times) */
+#define XCORR_HIP_HOST_WINDOWS 8u  /* correlate with DEVICE_PTRS: windows are host memory, results device memory */
 
 /* Run parameters, named after struct xcorr (xcorr_d.h:23-77).  Sizes follow
  * parse_xcorr_input.c:139-158: nx_corr = nxc = 2*xsearch, npx = 4*xsearch. */
@@ -105,7 +106,12 @@ int xcorr_hip_set_images(xcorr_hip_ctx *ctx, const void *master, int m_nx, int m
                          const void *slave, int s_nx, int s_ny, unsigned flags);
 
 /* Correlate n locations (any n >= 0; batches of max_batch internally).
- * win[n] in, out[n] out (host memory unless XCORR_HIP_DEVICE_PTRS). */
+ * win[n] in, out[n] out (host memory unless XCORR_HIP_DEVICE_PTRS; with
+ * XCORR_HIP_HOST_WINDOWS as well, windows are host memory and results device
+ * memory).  out[i] always belongs to win[i].  For xsearch = ysearch = 64, ri 2,
+ * frequency mode, windows that share mx0, sx0 and sy0 - my0 and overlap in
+ * lines are computed from shared range-interpolated lines; the windows are then
+ * read on the host (device windows are copied back first). */
 int xcorr_hip_correlate(xcorr_hip_ctx *ctx, const xcorr_hip_window *win, int n,
                         xcorr_hip_result *out, unsigned flags);
 
@@ -120,8 +126,9 @@ void *xcorr_hip_stream(xcorr_hip_ctx *ctx);
  * kernels; returns the number of kernels. */
 int xcorr_hip_kernel_times(xcorr_hip_ctx *ctx, const char **names, float *ms, int *launches, int cap);
 
-/* Copy an internal stage buffer of patch `slot` of the last batch to host
- * memory, for stage-by-stage parity tests.  what: 0 amp master (npy*npx f32),
+/* Copy an internal stage buffer of location `slot` (its index in the last
+ * correlate call; it must belong to that call's last batch) to host memory,
+ * for stage-by-stage parity tests.  what: 0 amp master (npy*npx f32),
  * 1 amp slave (valid on the template window; elsewhere only in some modes),
  * 2 correlation surface (nyc*nxc f32; f64 in time modes),
  * 3 patch means (2 f64).  Returns bytes copied or a negative code. */

@@ -1,0 +1,98 @@
+"""GPU parity of the strip path (-xsearch 64 -ysearch 64, ri 2, frequency mode).
+
+Windows that share the master column, the slave column and the slave line
+offset are computed from shared strip lines.  These cases exercise the
+planner: overlapping, touching, duplicate and isolated windows in one column,
+mixed slave offsets, unsorted input, windows across the image edges, batches
+that split a strip, and the three pointer modes.  Every result must equal the
+FP32 CPU restatement bit for bit, in input order.
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_present
+from test_gpu_parity import _assert_bit_exact, _locs, _oracle_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pair(synth):
+    if not gpu_present():
+        pytest.skip("no GPU")
+    return synth.constant_shift_pair(768, 1536, -2.63, 5.41, 0.8)
+
+
+def _strip_windows(xg):
+    rng = np.random.default_rng(7)
+    rows = []
+    # column A: dense overlapping run, a duplicate, a touching window, a gap
+    for my0 in (100, 137, 137, 150, 300, 356, 612, 868, 1100):
+        rows.append((200, my0, 203, my0 - 5))
+    # column A with a second slave offset (separate strips, same master lines)
+    for my0 in (120, 180, 240):
+        rows.append((200, my0, 203, my0 + 9))
+    # column B: random overlapping lines
+    for my0 in sorted(rng.integers(0, 1200, 10)):
+        rows.append((420, int(my0), 418, int(my0) + 2))
+    # edges: left/right/top/bottom overhang, fully outside
+    rows += [(-60, 20, -57, 18), (600, 1400, 603, 1395), (-100, -200, -98, -190), (700, -30, 705, -25),
+             (-300, 500, -297, 495), (900, 900, 905, 900)]
+    w = np.array(rows, np.int32)
+    perm = rng.permutation(len(w))                 # unsorted input order
+    return np.ascontiguousarray(w[perm]).view(xg.WINDOW_DTYPE).reshape(-1)
+
+
+def test_strip_planner_bit_exact(xg, orc, pair):
+    m, s = pair
+    p = xg.make_params(64, 64)
+    w = _strip_windows(xg)
+    ref = orc.run(_oracle_params(orc, p), m, s, _locs(w), precision=32)
+    for mb in (7, 64):                              # 7: strips split across batches
+        with xg.Engine(p, max_batch=mb) as eng:
+            eng.set_images(m, s)
+            got = eng.correlate(w)
+        _assert_bit_exact(got, ref, "max_batch %d" % mb)
+
+
+def test_strip_pointer_modes(xg, pair):
+    import torch
+    m, s = pair
+    p = xg.make_params(64, 64)
+    w = _strip_windows(xg)
+    dm, ds = torch.from_numpy(m).cuda(), torch.from_numpy(s).cuda()
+    dw = torch.from_numpy(w.view(np.int32).reshape(-1, 4).copy()).cuda()
+    nbytes = len(w) * xg.RESULT_DTYPE.itemsize
+    with xg.Engine(p, max_batch=9) as eng:
+        eng.set_images(dm, ds)
+        host = eng.correlate(w)
+        d1 = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
+        eng.correlate(dw, out=d1)                   # device windows and results
+        d2 = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
+        eng.correlate(w, out=d2, flags=xg.NO_SYNC)  # host windows, device results
+        eng.sync()
+    for d, label in ((d1, "device windows"), (d2, "host windows, device results")):
+        got = d.cpu().numpy().view(xg.RESULT_DTYPE)
+        _assert_bit_exact(got, host, label)
+
+
+def test_strip_dumps_in_shared_strip(xg, orc, pair):
+    m, s = pair
+    p = xg.make_params(64, 64)
+    w = np.array([(300, my0, 297, my0 + 3) for my0 in (400, 450, 520, 560)], np.int32).view(
+        xg.WINDOW_DTYPE).reshape(-1)
+    with xg.Engine(p) as eng:
+        eng.set_images(m, s)
+        eng.correlate(w)
+        for slot in (1, 2):
+            amp1, amp2 = eng.dump(0, slot), eng.dump(1, slot)
+            means, surf = eng.dump(3, slot), eng.dump(2, slot)
+            mw = m[w["my0"][slot]:w["my0"][slot] + 256, w["mx0"][slot]:w["mx0"][slot] + 256].astype(np.float32)
+            sw = s[w["sy0"][slot]:w["sy0"][slot] + 256, w["sx0"][slot]:w["sx0"][slot] + 256].astype(np.float32)
+            _, d = orc.patch(_oracle_params(orc, p), mw[..., 0] + 1j * mw[..., 1], sw[..., 0] + 1j * sw[..., 1],
+                             precision=32, dump=True)
+            t = (slice(64, 192), slice(64, 192))
+            assert np.array_equal(amp1.view(np.uint32), d["amp1"].view(np.uint32)), "amp master, slot %d" % slot
+            assert np.array_equal(amp2[t].view(np.uint32), d["amp2"][t].view(np.uint32)), "amp slave, slot %d" % slot
+            assert np.array_equal(means, d["mean"]), "means, slot %d" % slot
+            assert np.array_equal(surf.view(np.uint32), d["surf"].view(np.uint32)), "surface, slot %d" % slot
