@@ -41,4 +41,55 @@ __device__ __forceinline__ void fft256_group(float2 (&v)[16], int j, float2* til
   Dft<16, DIR>::run(v);                       // pass 2 -> X[j + 16 r] = v[r]
 }
 
+// Hermitian completion of a row spectrum held as v[r] = X[j + 16 r]: when
+// `apply`, X[k] = conj(X[256 - k]) for k > 128.  All 256 threads of the block
+// call this together (block barriers); `tile` is the group's 16 x 17 scratch.
+__device__ __forceinline__ void hermitian_upper256(float2 (&v)[16], int j, float2* tile, bool apply) {
+#pragma unroll
+  for (int r = 0; r < 16; r++) tile[r * kT256Pitch + j] = v[r];
+  __syncthreads();
+  if (apply) {
+#pragma unroll
+    for (int r = 8; r < 16; r++) {
+      const int k = j + 16 * r;
+      if (k > 128) {
+        const int m = 256 - k;
+        const float2 w = tile[(m >> 4) * kT256Pitch + (m & 15)];
+        v[r] = make_float2(w.x, -w.y);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// An offset of 0 the compiler cannot see through (an empty asm statement, no
+// instruction): LDS reads indexed with it stay where they are written instead
+// of being hoisted out of loops into registers.
+__device__ __forceinline__ int opaque_zero() {
+  int z = 0;
+  asm volatile("" : "+v"(z));
+  return z;
+}
+
+// As fft256_group, with the pass-2 twiddles read from LDS at the point of use:
+// twl[15 j + r - 1] = w256^(r j).
+template <int DIR>
+__device__ __forceinline__ void fft256_group_lds(float2 (&v)[16], int j, float2* tile, const float2* twl) {
+  Dft<16, DIR>::run(v);
+#pragma unroll
+  for (int r = 0; r < 16; r++) tile[j * kT256Pitch + r] = v[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; r++) v[r] = tile[r * kT256Pitch + j];
+  __syncthreads();
+  const float2* tj = twl + 15 * j + opaque_zero();
+#pragma unroll
+  for (int r = 1; r < 16; r++) {
+    float2 w = tj[r - 1];
+    if (DIR < 0) w.y = -w.y;
+    v[r] = cmul(v[r], w);
+  }
+  Dft<16, DIR>::run(v);
+}
+
 }  // namespace xc

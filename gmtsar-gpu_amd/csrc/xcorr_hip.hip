@@ -367,6 +367,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_stageA256(DevParams P, const vo
     }
     __syncthreads();
     fft256_group<-1>(v, j, tile, t);
+    hermitian_upper256(v, j, tile, !in_y);        // unmasked rows: real input
 #pragma unroll
     for (int r = 0; r < 16; r++) zb[(size_t)row * 256 + j + 16 * r] = v[r];
   }
@@ -391,34 +392,63 @@ __global__ void __launch_bounds__(kThreads, 3) k_stageA256(DevParams P, const vo
 constexpr int kStripBlockLines = 64;      // strip lines per block (16 groups x 4)
 
 // Raw samples x0 + j + 16 r of line gy (one dword per sample in both formats),
-// zero outside the image.
-__device__ __forceinline__ void load_line256(const unsigned* __restrict__ im, int inx, int iny, bool inside, int gy,
-                                             int x0, int j, unsigned (&u)[16]) {
+// zero outside the image.  Branch-free: the address is clamped into the image
+// (so every load is in bounds) and the value is zeroed afterwards, which lets
+// all 16 loads issue back to back.
+// EDGE false: columns [x0, x0 + 256) lie in the image.
+template <bool EDGE>
+__device__ __forceinline__ void load_line256(const unsigned* __restrict__ im, int inx, int iny, int gy, int x0, int j,
+                                             unsigned (&u)[16]) {
+  const bool oky = (unsigned)gy < (unsigned)iny;
+  const unsigned* line = im + (size_t)min(max(gy, 0), iny - 1) * inx;
+  if (!EDGE) {
+    const unsigned* p = line + x0 + j;
 #pragma unroll
-  for (int r = 0; r < 16; r++) {
-    const int gx = x0 + j + 16 * r;
-    unsigned w = 0u;
-    if (inside || (gy >= 0 && gy < iny && gx >= 0 && gx < inx)) w = im[(size_t)gy * inx + gx];
-    u[r] = w;
+    for (int r = 0; r < 16; r++) {
+      const unsigned w = p[16 * r];
+      u[r] = oky ? w : 0u;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int gx = x0 + j + 16 * r;
+      const unsigned w = line[min(max(gx, 0), inx - 1)];
+      u[r] = (oky && (unsigned)gx < (unsigned)inx) ? w : 0u;
+    }
   }
 }
 
-// Range interpolation of one held line (range_row256 after the gather); the
-// phase factors come from LDS.
-__device__ __forceinline__ void interp_line256(int format, const unsigned (&u)[16], int j, float2* tile,
-                                               const float2 (&t)[15], const float2* sph, float (&ea)[8],
-                                               float (&oa)[8]) {
+// Range interpolation of one held line (range_row256 after the gather), with
+// twiddles and phase factors from LDS.  sph holds phi[0..128]; the rest follow
+// from phi[k] = conj(phi[256 - k]) (exact: the table entries are built from
+// the same angles).
+template <int FMT>
+__device__ __forceinline__ void interp_line256(const unsigned (&u)[16], int j, float2* tile, const float2* twl,
+                                               const float2* sph, float (&ea)[8], float (&oa)[8]) {
   float2 v[16];
 #pragma unroll
   for (int r = 0; r < 16; r++)
-    v[r] = format == XCORR_HIP_CINT16 ? make_float2((float)(short)(u[r] & 0xffffu), (float)(short)(u[r] >> 16))
-                                      : make_float2(__uint_as_float(u[r]), 0.0f);
+    v[r] = FMT == XCORR_HIP_CINT16 ? make_float2((float)(short)(u[r] & 0xffffu), (float)(short)(u[r] >> 16))
+                                   : make_float2(__uint_as_float(u[r]), 0.0f);
 #pragma unroll
   for (int q = 0; q < 8; q++) ea[q] = ampl(v[q + 4]);
-  fft256_group<-1>(v, j, tile, t);
+  fft256_group_lds<-1>(v, j, tile, twl);
+  sph += opaque_zero();
 #pragma unroll
-  for (int r = 0; r < 16; r++) v[r] = cmul(v[r], sph[j + 16 * r]);
-  fft256_group<+1>(v, j, tile, t);
+  for (int r = 0; r < 16; r++) {
+    float2 ph;
+    if (r < 8) {
+      ph = sph[j + 16 * r];
+    } else if (r == 8) {
+      ph = sph[128 - j];
+      if (j != 0) ph.y = -ph.y;
+    } else {
+      ph = sph[256 - j - 16 * r];
+      ph.y = -ph.y;
+    }
+    v[r] = cmul(v[r], ph);
+  }
+  fft256_group_lds<+1>(v, j, tile, twl);
   const float sc = 1.0f / 256.0f;
 #pragma unroll
   for (int q = 0; q < 8; q++) {
@@ -458,43 +488,67 @@ __device__ __forceinline__ long long group16_sum_i64(long long v) {
   for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
+__device__ __forceinline__ double group16_sum_f64(double v) {
+  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Exact fixed-point sum of the 16 amplitudes a lane holds for one line, as
+// sum_k trunc(a_k 2^20).  Complex int16 samples give a < 2^16, so every term
+// is an integer below 2^36 and a 256-term line sum stays below 2^53: FP64
+// accumulation is exact in any order.  Real float samples are unbounded and
+// take the int64 route.
+template <int FMT>
+struct LineSum {
+  double d = 0.0;
+  long long i = 0;
+  __device__ __forceinline__ void add(float a) {
+    if (FMT == XCORR_HIP_CINT16) d += (double)truncf(a * 1048576.0f);
+    else i += fxp20(a);
+  }
+  __device__ __forceinline__ long long group_total() {
+    if (FMT == XCORR_HIP_CINT16) return (long long)group16_sum_f64(d);
+    return group16_sum_i64(i);
+  }
+};
 
 // One block per kStripBlockLines strip lines; 16 groups of 16 lanes, one line
 // per group and iteration.  Out: amp1 (all columns), amp2 (template columns of
 // masked lines), lsum (per-line fixed-point sums), Zm / Zu (row spectra of
 // masked / unmasked lines, where needed).
-__global__ void __launch_bounds__(kThreads, 3) k_stripA256(DevParams P, const void* __restrict__ master,
+// Block b + blk_off covers kStripBlockLines lines from global line blk[2b + 1]
+// of strip blk[2b].  EDGE: the strips' windows may cross the image sides.
+template <int FMT, bool EDGE>
+__global__ void __launch_bounds__(kThreads, 2) k_stripA256(DevParams P, const void* __restrict__ master,
                                                         const void* __restrict__ slave,
                                                         const StripDesc* __restrict__ strips,
-                                                        const int* __restrict__ blk_strip,
+                                                        const int* __restrict__ blk, int blk_off,
                                                         const unsigned char* __restrict__ needM,
                                                         const unsigned char* __restrict__ needU,
                                                         float* __restrict__ amp1, float* __restrict__ amp2,
                                                         long long* __restrict__ lsum, float2* __restrict__ Zm,
                                                         float2* __restrict__ Zu, const float2* __restrict__ tabs) {
-  __shared__ float2 tiles[16 * kT256Tile];
-  __shared__ float2 mrow[16 * 128];                // each group's master amplitude line (column pairs)
-  __shared__ float2 sph[256];                      // range-interpolation phase factors
+  // one shared array: 16 transpose tiles | 16 master lines (column pairs) |
+  // pass-2 twiddles w256^(r j) [16][15] | phase factors phi[0..128]
+  __shared__ float2 lds[16 * kT256Tile + 16 * 128 + 240 + 129];
   const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
-  const StripDesc S = strips[blk_strip[blockIdx.x]];
-  float2* tile = tiles + g * kT256Tile;
-  float2* mr = mrow + g * 128;
-  float2 t[15];
-  load_tw256(tabs + P.tw_nx, j, t);
-  for (int k = threadIdx.x; k < 256; k += kThreads) sph[k] = tabs[P.phi + k];
+  const int bb = blockIdx.x + blk_off;
+  const StripDesc S = strips[blk[2 * bb]];
+  float2* tile = lds + g * kT256Tile;
+  float2* mr = lds + 16 * kT256Tile + g * 128;
+  float2* twl = lds + 16 * kT256Tile + 16 * 128;
+  float2* sph = twl + 240;
+  for (int k = threadIdx.x; k < 240; k += kThreads) twl[k] = tabs[P.tw_nx + (k % 15 + 1) * (k / 15)];
+  for (int k = threadIdx.x; k <= 128; k += kThreads) sph[k] = tabs[P.phi + k];
   const unsigned* im_m = (const unsigned*)master;
   const unsigned* im_s = (const unsigned*)slave;
-  const bool col_m = S.mx0 >= 0 && S.mx0 + 256 <= P.m_nx;
-  const bool col_s = S.sx0 >= 0 && S.sx0 + 256 <= P.s_nx;
-  const int first = blockIdx.x * kStripBlockLines - S.line0;     // strip-relative line of this block
+  const int first = blk[2 * bb + 1] - S.line0;                   // strip-relative line of this block
   const int nit = min(kStripBlockLines, S.nlines - first + 15) >> 4;
-  // the samples of a group's next line are loaded one iteration ahead
-  unsigned um[16], us[16];
-  {
-    const int gym = S.my0 + first + g, gys = S.sy0 + first + g;
-    load_line256(im_m, P.m_nx, P.m_ny, col_m && gym >= 0 && gym < P.m_ny, gym, S.mx0, j, um);
-    load_line256(im_s, P.s_nx, P.s_ny, col_s && gys >= 0 && gys < P.s_ny, gys, S.sx0, j, us);
-  }
+  // Loads run ahead of the transforms with one line in flight at a time: the
+  // slave line is loaded before the master transforms, the next master line
+  // before the slave and z transforms.
+  unsigned um[16];
+  load_line256<EDGE>(im_m, P.m_nx, P.m_ny, S.my0 + first + g, S.mx0, j, um);
   __syncthreads();
   for (int it = 0; it < nit; it++) {
     const int i = first + 16 * it + g;                            // strip-relative line of this group
@@ -502,61 +556,53 @@ __global__ void __launch_bounds__(kThreads, 3) k_stripA256(DevParams P, const vo
     const bool more = it + 1 < nit;
     const size_t l = (size_t)(S.line0 + i);
     const bool wm = act && needM[l], wu = act && needU[l];
+    unsigned us[16];
+    load_line256<EDGE>(im_s, P.s_nx, P.s_ny, S.sy0 + i, S.sx0, j, us);
     {
       float ea[8], oa[8];
-      const unsigned cur[16] = {um[0], um[1], um[2], um[3], um[4], um[5], um[6], um[7],
-                                um[8], um[9], um[10], um[11], um[12], um[13], um[14], um[15]};
-      if (more) {
-        const int gy = S.my0 + i + 16;
-        load_line256(im_m, P.m_nx, P.m_ny, col_m && gy >= 0 && gy < P.m_ny, gy, S.mx0, j, um);
-      }
-      interp_line256(P.format, cur, j, tile, t, sph, ea, oa);
-      long long s = 0;
+      interp_line256<FMT>(um, j, tile, twl, sph, ea, oa);
+      if (more) load_line256<EDGE>(im_m, P.m_nx, P.m_ny, S.my0 + i + 16, S.mx0, j, um);
+      LineSum<FMT> s;
       float2* o = (float2*)(amp1 + l * 256);
 #pragma unroll
       for (int q = 0; q < 8; q++) {
         const float2 a = make_float2(ea[q], oa[q]);
         if (act) o[j + 16 * q] = a;
         mr[j + 16 * q] = a;
-        s += fxp20(ea[q]) + fxp20(oa[q]);
+        s.add(ea[q]);
+        s.add(oa[q]);
       }
-      s = group16_sum_i64(s);
-      if (act && j == 0) lsum[2 * l] = s;
+      const long long st = s.group_total();
+      if (act && j == 0) lsum[2 * l] = st;
     }
     {
       float ea[8], oa[8];
-      const unsigned cur[16] = {us[0], us[1], us[2], us[3], us[4], us[5], us[6], us[7],
-                                us[8], us[9], us[10], us[11], us[12], us[13], us[14], us[15]};
-      if (more) {
-        const int gy = S.sy0 + i + 16;
-        load_line256(im_s, P.s_nx, P.s_ny, col_s && gy >= 0 && gy < P.s_ny, gy, S.sx0, j, us);
-      }
-      interp_line256(P.format, cur, j, tile, t, sph, ea, oa);
-      long long s = 0;
+      interp_line256<FMT>(us, j, tile, twl, sph, ea, oa);
+      LineSum<FMT> s;
       float2* o = (float2*)(amp2 + l * 256);
 #pragma unroll
       for (int q = 0; q < 8; q++) {
         const float2 a = make_float2(ea[q], oa[q]);
         if (wm && q >= 2 && q < 6) o[j + 16 * q] = a;
         tile[j + 16 * q] = a;                      // the tile doubles as the slave line buffer
-        s += fxp20(ea[q]) + fxp20(oa[q]);
+        s.add(ea[q]);
+        s.add(oa[q]);
       }
-      s = group16_sum_i64(s);
-      if (act && j == 0) lsum[2 * l + 1] = s;
+      const long long st = s.group_total();
+      if (act && j == 0) lsum[2 * l + 1] = st;
     }
     const bool anyM = __syncthreads_or(wm);
     const float* m1 = (const float*)mr;
     const float* a2 = (const float*)tile;
     float2 v[16];
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
+    for (int r = 0; r < 16; r++) {              // npx = 256, xs = 64: x in [xs, npx - xs) iff r in [4, 12)
       const int x = j + 16 * r;
-      const bool in = x >= P.xs && x < 256 - P.xs;
-      v[r] = make_float2(m1[x], in ? a2[x] : 0.0f);
+      v[r] = make_float2(m1[x], (r >= 4 && r < 12) ? a2[x] : 0.0f);
     }
     const bool anyU = __syncthreads_or(wu);
     if (anyM) {
-      fft256_group<-1>(v, j, tile, t);
+      fft256_group_lds<-1>(v, j, tile, twl);
       if (wm)
 #pragma unroll
         for (int r = 0; r < 16; r++) Zm[l * 256 + j + 16 * r] = v[r];
@@ -564,10 +610,11 @@ __global__ void __launch_bounds__(kThreads, 3) k_stripA256(DevParams P, const vo
     if (anyU) {
 #pragma unroll
       for (int r = 0; r < 16; r++) v[r] = make_float2(m1[j + 16 * r], 0.0f);
-      fft256_group<-1>(v, j, tile, t);
-      if (wu)
+      fft256_group_lds<-1>(v, j, tile, twl);
+      if (wu)                                   // bins 0..128; the rest are their conjugate mirrors
 #pragma unroll
-        for (int r = 0; r < 16; r++) Zu[l * 256 + j + 16 * r] = v[r];
+        for (int r = 0; r <= 8; r++)
+          if (r < 8 || j == 0) Zu[l * 256 + j + 16 * r] = v[r];
     }
   }
 }
@@ -614,6 +661,7 @@ __global__ void __launch_bounds__(kThreads) k_rowfft256(DevParams P, const float
       v[r] = make_float2(a1[row * 256 + x], in ? a2[row * 256 + x] : 0.0f);
     }
     fft256_group<-1>(v, j, tile, t);
+    hermitian_upper256(v, j, tile, !in_y);        // unmasked rows: real input
 #pragma unroll
     for (int r = 0; r < 16; r++) zb[(size_t)row * 256 + j + 16 * r] = v[r];
   }
@@ -679,12 +727,20 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
     const int kxt = 16 * p + c;
     const int kxm = (p == 0 && c == 0) ? NX / 2 : NX - 16 * p - c;
     const float2 mkt = tabs[P.mk_x + kxt], mkm = tabs[P.mk_x + kxm];
+    const bool self = p == 0 && c == 0;
     float2 rT[16], rM[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
       const float2* zr = zrow(y0 + 16 * k);
       rT[k] = zr[kxt];
-      rM[k] = zr[kxm];
+      // rows y0 + 16 k outside [64, 192) are unmasked: their spectrum is
+      // Hermitian, so column npx - kx is conj(column kx) (npx/2 is stored)
+      if (k >= 4 && k < 12) {
+        rM[k] = zr[kxm];
+      } else {
+        const float2 w = zr[self ? kxm : kxt];   // branch-free; a cache hit for kxt
+        rM[k] = self ? w : make_float2(w.x, -w.y);
+      }
     }
 #pragma unroll
     for (int k = 0; k < 16; k++) {
@@ -775,20 +831,27 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
   unsigned long long best = 0ull;
   double sum = 0.0;
   const int iters = (P.nyc + 15) >> 4;          // uniform trip count: the transform has block barriers
+  // row loads run one iteration ahead of the transform
+  auto load_row = [&](int row, float2 (&u)[16]) {
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int x = j + 16 * r;
+      u[r] = make_float2(0.0f, 0.0f);
+      if (row < P.nyc) u[r] = db[(size_t)row * 256 + (x <= 128 ? x : 256 - x)];
+    }
+  };
+  float2 nx[16];
+  load_row(g, nx);
   for (int it = 0; it < iters; it++) {
     const int row = it * 16 + g;
     const bool active = row < P.nyc;
     float2 v[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-      const int x = j + 16 * r;
-      float2 u = make_float2(0.0f, 0.0f);
-      if (active) {
-        u = db[(size_t)row * 256 + (x <= 128 ? x : 256 - x)];   // upper half: conjugate mirror
-        if (x > 128) u.y = -u.y;
-      }
-      v[r] = u;
+      v[r] = nx[r];
+      if (j + 16 * r > 128) v[r].y = -v[r].y;      // upper half: conjugate mirror
     }
+    if (it + 1 < iters) load_row(row + 16, nx);
     fft256_group<+1>(v, j, tile, t);
     if (!active) continue;
 #pragma unroll
@@ -836,7 +899,16 @@ __global__ void __launch_bounds__(kThreads) k_rowfft(DevParams P, const float* _
   __syncthreads();
   float2* R = lds_fft_rows<-1>(A, Bf, NX, G, tabs + P.tw_nx);
   float2* out = Z + (size_t)b * P.N + (size_t)g * G * NX;
-  for (int e = threadIdx.x; e < nel; e += kThreads) out[e] = R[e];
+  for (int e = threadIdx.x; e < nel; e += kThreads) {
+    const int row = e / NX, x = e - row * NX;
+    const int y = g * G + row;
+    float2 v = R[e];
+    if (!(y >= P.ys && y < NY - P.ys) && x > NX / 2) {   // unmasked rows: Hermitian completion
+      const float2 w = R[row * NX + NX - x];
+      v = make_float2(w.x, -w.y);
+    }
+    out[e] = v;
+  }
 }
 
 // FFT along y, in place on Z (columns tile per block).
@@ -1505,9 +1577,9 @@ static void free_scratch(xcorr_hip_ctx* c) {
   c->lsum = nullptr; c->Zu = nullptr; c->need = nullptr; c->d_plan = nullptr;
 }
 
-// Plan layout (ints): rb[max_batch] | perm[max_batch] | blk_strip[max_lines / 64] | StripDesc[max_batch]
+// Plan layout (ints): rb[max_batch] | perm[max_batch] | blk[2 max_lines / 64] | StripDesc[max_batch]
 static size_t plan_ints(const xcorr_hip_ctx* c) {
-  return 2 * (size_t)c->max_batch + (size_t)c->max_lines / kStripBlockLines +
+  return 2 * (size_t)c->max_batch + 2 * (size_t)c->max_lines / kStripBlockLines +
          (size_t)c->max_batch * (sizeof(StripDesc) / sizeof(int));
 }
 
@@ -1583,6 +1655,12 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   d.mk_y = add_mask(d.NY, d.ys);
   d.phi = (int)tab.size();
   const int M = d.ri * d.NX;
+  // the strip kernel stores phi[0..npx/2] only and rebuilds phi[k] = conj(phi[npx - k])
+  bool phi_sym = true;
+  for (int k = d.NX / 2 + 1; k < d.NX && d.ri == 2; k++) {
+    const float2 a = twf(M, k - d.NX), b = twf(M, d.NX - k);
+    phi_sym = phi_sym && a.x == b.x && a.y == -b.y;
+  }
   for (int r = 1; r < d.ri; r++)
     for (int k = 0; k < d.NX; k++) {
       const long kt = (k < d.NX / 2) ? k : k - d.NX;
@@ -1599,7 +1677,7 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   auto alloc = [&](void** ptr, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(ptr, bytes ? bytes : 16);
   };
-  c->strips = d.NX == 256 && d.NY == 256 && d.ri == 2 && p->corr_flag == XCORR_HIP_FREQ;
+  c->strips = d.NX == 256 && d.NY == 256 && d.ri == 2 && p->corr_flag == XCORR_HIP_FREQ && phi_sym;
   if (c->strips) {
     // half the lines of unshared windows: overlapping windows need far fewer
     c->max_lines = std::max(max_batch * (d.NY / 2), d.NY + kStripBlockLines);
@@ -1797,7 +1875,7 @@ static long long round_up(long long v, long long m) { return (v + m - 1) / m * m
 static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_hip_result* out, bool dev_out,
                       bool timing) {
   const DevParams& d = c->dp;
-  const int NY = d.NY, mb = c->max_batch, ML = c->max_lines, blkcap = ML / kStripBlockLines;
+  const int NY = d.NY, mb = c->max_batch, ML = c->max_lines, blkcap = 2 * (ML / kStripBlockLines);
   std::vector<int>& ord = c->order;
   ord.resize(n);
   for (int i = 0; i < n; i++) ord[i] = i;
@@ -1853,12 +1931,22 @@ static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_
       pos++;
     }
     if (nb == 0) return fail(c, XCORR_HIP_EINVAL, "correlate: a window does not fit the line capacity");
-    for (int k = 0; k < ns; k++) {
-      const int b0 = st[k].line0 / kStripBlockLines;
-      const int b1 = (int)((st[k].line0 + round_up(st[k].nlines, kStripBlockLines)) / kStripBlockLines);
-      for (int blk = b0; blk < b1; blk++) bs[blk] = k;
+    // line blocks: strips whose windows lie inside both images first, then the rest
+    int nblk = 0, nblk_in = 0;
+    for (int pass = 0; pass < 2; pass++) {
+      for (int k = 0; k < ns; k++) {
+        const StripDesc& S = st[k];
+        const bool in = S.mx0 >= 0 && S.mx0 + d.NX <= d.m_nx && S.sx0 >= 0 && S.sx0 + d.NX <= d.s_nx;
+        if (in != (pass == 0)) continue;
+        for (int l0 = S.line0; l0 < S.line0 + S.nlines; l0 += kStripBlockLines) {
+          bs[2 * nblk] = k;
+          bs[2 * nblk + 1] = l0;
+          nblk++;
+        }
+      }
+      if (pass == 0) nblk_in = nblk;
     }
-    const int nl = (int)L, nblk = nl / kStripBlockLines;
+    const int nl = (int)L;
     const size_t bytes = (size_t)(2 * mb + blkcap) * sizeof(int) + (size_t)ns * sizeof(StripDesc);
     HIPCK(c, hipMemcpyAsync(c->d_plan, h, bytes, hipMemcpyHostToDevice, c->stream));
     HIPCK(c, hipEventRecord(c->ev_stage[si], c->stream));
@@ -1877,9 +1965,17 @@ static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_
     });
     if (rc) return rc;
     rc = launch_timed(c, timing, "k_stripA256", [&] {
-      hipLaunchKernelGGL(k_stripA256, dim3(nblk), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_st,
-                         d_bs, (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s, c->amp2s, c->lsum,
-                         c->Z, c->Zu, (const float2*)c->tabs);
+      const bool ci = d.format == XCORR_HIP_CINT16;
+      auto kin = ci ? k_stripA256<XCORR_HIP_CINT16, false> : k_stripA256<XCORR_HIP_FLOAT32, false>;
+      auto kedge = ci ? k_stripA256<XCORR_HIP_CINT16, true> : k_stripA256<XCORR_HIP_FLOAT32, true>;
+      if (nblk_in > 0)
+        hipLaunchKernelGGL(kin, dim3(nblk_in), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_st, d_bs,
+                           0, (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s, c->amp2s, c->lsum,
+                           c->Z, c->Zu, (const float2*)c->tabs);
+      if (nblk > nblk_in)
+        hipLaunchKernelGGL(kedge, dim3(nblk - nblk_in), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave,
+                           d_st, d_bs, nblk_in, (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s,
+                           c->amp2s, c->lsum, c->Z, c->Zu, (const float2*)c->tabs);
     });
     if (rc) return rc;
     rc = launch_timed(c, timing, "k_patch_sums", [&] {

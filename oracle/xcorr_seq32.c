@@ -10,6 +10,10 @@
  *   amplitude          sqrtf(fma(re, re, im*im))
  *   patch mean         exact fixed-point sum of amplitudes (2^-20 units)
  *   |c3|               (float)sqrt((double)re*re + (double)im*im)
+ *   row spectra        z = amp1 + i amp2 mask transformed along x; rows
+ *                      outside the mask (real input) keep bins 0..NX/2 and
+ *                      complete the rest as X[k] = conj(X[NX - k]); the means
+ *                      come off in this spectrum
  *
  * Compile with -ffp-contract=off so that only the explicit fmaf calls fuse.
  */
@@ -402,7 +406,19 @@ int xo32_patch(const xo_params *p, const c32 *mwin, const c32 *swin, xo_result *
     c32 *mx = (c32 *)malloc(sizeof(c32) * NX);
     if (!mx) { rc = -2; goto done; }
     xo_mask_spectrum(NX, p->xsearch, mx);
-    for (int y = 0; y < NY; y++) xo32_fft(z + (size_t)y * NX, NX, -1);
+    for (int y = 0; y < NY; y++) {
+      c32 *row = z + (size_t)y * NX;
+      xo32_fft(row, NX, -1);
+      /* rows outside the mask carry a real input (amp1 + 0i): their spectrum is
+       * taken as bins 0..NX/2 with the upper half completed by conjugate
+       * symmetry, X[k] = conj(X[NX - k]) (the transform alone is Hermitian only
+       * to rounding) */
+      if (!((y >= p->ysearch) && (y < NY - p->ysearch)))
+        for (int kx = NX / 2 + 1; kx < NX; kx++) {
+          row[kx].re = row[NX - kx].re;
+          row[kx].im = -row[NX - kx].im;
+        }
+    }
     /* demean in the row spectrum: z - m1 - i m2 mask has row transform
      * Zr - m1 NX delta(kx) - i m2 [y in mask] Mx[kx] */
     {
