@@ -42,6 +42,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def committed_traffic():
+    """Per-launch HBM bytes per kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (the JSON line under "Per-launch
+    traffic"), or {} when it is absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_xcorr_cfg2.md")
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return {}
+    for i, line in enumerate(lines):
+        if line.startswith("Per-launch traffic"):
+            for nxt in lines[i + 1:]:
+                if nxt.strip().startswith("{"):
+                    return json.loads(nxt.strip())
+    return {}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,12 +144,16 @@ def main():
     per_launch_ms = dom_ms / dom_launches
     patches_per_launch = n / dom_launches
     achieved = BYTES_PER_PATCH * patches_per_launch / (per_launch_ms * 1e-3) / 1e9
-    traffic = None
+    tr = {}
     if args.traffic:
         tr = json.loads(args.traffic) if args.traffic.lstrip().startswith("{") else json.load(open(args.traffic))
-        traffic = tr.get(dom_name)
+    elif not args.small:
+        tr = committed_traffic()
+    traffic = tr.get(dom_name)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/hbm_traffic_xcorr_cfg2.md"
+                                   if traffic and not args.traffic else None),
                 "kernel": dom_name, "launch_ms": round(per_launch_ms, 4),
                 "patches_per_launch": patches_per_launch}
     kernels = {k: {"ms_total": round(v[0], 4), "launches": v[1]} for k, v in kt.items()}
