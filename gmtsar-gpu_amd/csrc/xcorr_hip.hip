@@ -1191,9 +1191,19 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
       for (int k = 0; k < 8; k++) Xc[t * 8 + k] = v[k];
     }
     __syncthreads();
+    // E8[s][k] = e^{2 pi i kt s / Ly}: the row factors of output row s
+    float2* E8 = Xc + Lx * 8;
+    for (int e = threadIdx.x; e < Ly * 8; e += kThreads) {
+      const int s = e >> 3, k = e & 7;
+      E8[e] = sEy[((k < 4 ? k : k - 8) * s) & (Ly - 1)];
+    }
+    __syncthreads();
     // each thread owns one column t (its 8 spectrum values in registers) and
-    // walks the rows s; lanes of a wave share s, so the sEy reads broadcast
-    unsigned long long best = 0ull;
+    // walks the rows s in increasing order; lanes of a wave share s, so the E8
+    // reads broadcast.  The running maximum keeps the first index (strict >)
+    // and skips values <= 0 and NaN, as the peak key does.
+    float bestv = 0.0f;
+    unsigned besti = 0u;
     const int per = kThreads >= Lx ? kThreads / Lx : 1;
     for (int t0 = 0; t0 < Lx; t0 += kThreads) {
       const int t = t0 + (per > 1 ? (int)threadIdx.x % Lx : (int)threadIdx.x);
@@ -1203,18 +1213,21 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
 #pragma unroll
       for (int k = 0; k < 8; k++) X[k] = Xc[t * 8 + k];
       for (int s = s0; s < Ly; s += per) {
+        const float2* es = E8 + s * 8;
         float acc = 0.0f;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-          const int kt = k < 4 ? k : k - 8;
-          const float2 ey = sEy[(kt * s) & (Ly - 1)];
+          const float2 ey = es[k];
           acc = __builtin_fmaf(X[k].x, ey.x, __builtin_fmaf(-X[k].y, ey.y, acc));
         }
-        const unsigned long long kk = peak_key(acc * sy, (unsigned)(s * Lx + t));
-        best = kk > best ? kk : best;
+        const float v = acc * sy;
+        if (v > bestv) {
+          bestv = v;
+          besti = (unsigned)(s * Lx + t);
+        }
       }
     }
-    best = block_max_u64(best, redk);
+    unsigned long long best = block_max_u64(peak_key(bestv, besti), redk);
     float bv;
     unsigned bidx;
     key_decode(best, &bv, &bidx);
@@ -1994,7 +2007,7 @@ static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_
                          c->peak, c->partial, (const float2*)c->tabs);
     });
     if (rc) return rc;
-    const size_t sh5 = d.interp_flag ? (size_t)d.Lx * 16 * sizeof(float2) : 16;
+    const size_t sh5 = d.interp_flag ? ((size_t)d.Lx * 16 + (size_t)d.Ly * 8) * sizeof(float2) : 16;
     rc = launch_timed(c, timing, "k_finish", [&] {
       hipLaunchKernelGGL(k_finish, dim3(nb), dim3(kThreads), sh5, c->stream, d, (const float*)c->amp1s,
                          (const float*)c->amp2s, d_rb, 0LL, dev_out ? d_perm : (const int*)nullptr,
@@ -2093,7 +2106,7 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
     rc = time_search(c, d_win, nb, timing);
     if (rc) return rc;
   }
-  const size_t sh5 = d.interp_flag ? (size_t)d.Lx * 16 * sizeof(float2) : 16;
+  const size_t sh5 = d.interp_flag ? ((size_t)d.Lx * 16 + (size_t)d.Ly * 8) * sizeof(float2) : 16;
   rc = launch_timed(c, timing, "k_finish", [&] {
     hipLaunchKernelGGL(k_finish, dim3(nb), dim3(kThreads), sh5, c->stream, d, (const float*)c->amp,
                        (const float*)c->amp + d.N, (const int*)nullptr, (long long)2 * d.N, (const int*)nullptr,
