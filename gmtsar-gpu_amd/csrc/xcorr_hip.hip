@@ -1681,7 +1681,15 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
     }
   if (tab.empty()) tab.push_back(make_float2(0.f, 0.f));
 
-  if (max_batch <= 0) max_batch = 2048;
+  if (max_batch <= 0) {
+    // scratch per location (both paths allocate at most about this much):
+    // amplitudes / strip lines, spectra, lag rows, surface
+    const size_t per = (size_t)d.N * 4 * (2 + 4) + (size_t)d.nyc * d.NX * 8 + (size_t)d.nxc * d.nyc * 8 + 256;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
+    const size_t budget = std::min(free_b / 4, (size_t)24 << 30);
+    max_batch = (int)std::max<size_t>(64, std::min<size_t>(8192, budget / per));
+  }
   c->max_batch = max_batch;
   const size_t B = (size_t)max_batch;
   const size_t N = (size_t)d.N;

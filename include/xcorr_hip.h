@@ -93,9 +93,9 @@ typedef struct xcorr_hip_ctx xcorr_hip_ctx;
  * set_defaults (parse_xcorr_input.c:38-88): ri 2, interp 16, n2 8, freq, cint16. */
 void xcorr_hip_default_params(xcorr_hip_params *p, int xsearch, int ysearch);
 
-/* Allocate a context for up to max_batch locations per correlate call
- * (max_batch <= 0 picks a size from device memory).  device < 0 uses the
- * current device. */
+/* Allocate a context for up to max_batch locations per batch (max_batch <= 0
+ * picks up to 8192 from the window size and a quarter of the free device
+ * memory).  device < 0 uses the current device. */
 int xcorr_hip_create(const xcorr_hip_params *p, int max_batch, int device, xcorr_hip_ctx **out);
 
 /* Make the master/slave images available to the context.  With
