@@ -27,8 +27,12 @@ DEVICE_PTRS, NO_SYNC, TIMING, HOST_WINDOWS = 1, 2, 4, 8
 EXPORTED_SYMBOLS = (
     "xcorr_hip_abi_version", "xcorr_hip_default_params", "xcorr_hip_create", "xcorr_hip_set_images",
     "xcorr_hip_correlate", "xcorr_hip_sync", "xcorr_hip_stream", "xcorr_hip_kernel_times",
-    "xcorr_hip_dump", "xcorr_hip_destroy", "xcorr_hip_last_error",
+    "xcorr_hip_dump", "xcorr_hip_destroy", "xcorr_hip_last_error", "xcorr_hip_correlate_stream",
 )
+
+# int reader(void *user, int image, int first, int count, void *dst)
+LINE_READER = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_void_p)
 
 
 class Params(ctypes.Structure):
@@ -74,6 +78,8 @@ def load_library():
     lib.xcorr_hip_dump.restype = ctypes.c_long
     lib.xcorr_hip_destroy.argtypes = [P]
     lib.xcorr_hip_destroy.restype = None
+    lib.xcorr_hip_correlate_stream.argtypes = [P, LINE_READER, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int, P, ctypes.c_int, P]
     lib.xcorr_hip_last_error.argtypes = [P]
     lib.xcorr_hip_last_error.restype = ctypes.c_char_p
     _lib = lib
@@ -144,6 +150,36 @@ class Engine:
         rc = self.lib.xcorr_hip_set_images(self.h, mp, shape_m[1], shape_m[0], sp, shape_s[1], shape_s[0],
                                            DEVICE_PTRS if dev else 0)
         self._check(rc, "xcorr_hip_set_images")
+
+    def correlate_stream(self, master, slave, windows, band_lines=0):
+        """Correlate with the images streamed in bands of lines.  master / slave
+        are host arrays (numpy, e.g. np.memmap of the SLC files) of shape
+        (lines, samples[, 2]) with 4-byte samples; they are read line band by
+        line band through a callback."""
+        imgs = [np.ascontiguousarray(a).reshape(np.shape(a)[0], -1).view(np.uint32) for a in (master, slave)]
+        err = []
+
+        def reader(user, image, first, count, dst):
+            try:
+                src = imgs[image]
+                nx = src.shape[1]
+                out = np.ctypeslib.as_array(ctypes.cast(dst, ctypes.POINTER(ctypes.c_uint32)), (count, nx))
+                out[:] = src[first:first + count]
+                return 0
+            except Exception as e:          # reported through the engine's error path
+                err.append(e)
+                return 1
+
+        cb = LINE_READER(reader)
+        w = _as_windows(windows)
+        res = np.zeros(w.shape[0], RESULT_DTYPE)
+        (m_ny, m_nx), (s_ny, s_nx) = imgs[0].shape, imgs[1].shape
+        rc = self.lib.xcorr_hip_correlate_stream(self.h, cb, None, m_nx, m_ny, s_nx, s_ny, int(band_lines), _ptr(w),
+                                                 w.shape[0], _ptr(res))
+        if err:
+            raise err[0]
+        self._check(rc, "xcorr_hip_correlate_stream")
+        return res
 
     def correlate(self, windows, out=None, flags=0):
         """windows: numpy WINDOW_DTYPE array (or (n,4) int32) -> numpy RESULT_DTYPE array.

@@ -23,6 +23,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <climits>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,9 @@ struct DevParams {
   int tw_nx, tw_ny, tw_lx, tw_ly, phi;   // offsets into the table buffer
   int mk_x, mk_y;                 // spectra of the 1-D mask boxes (npx, npy entries)
   int m_nx, m_ny, s_nx, s_ny;
+  // lines held in device memory (all of each image, or a streamed band):
+  // addresses are only ever formed for lines in [lo, hi)
+  int m_lo, m_hi, s_lo, s_hi;
 };
 
 // --------------------------------------------------------------------------
@@ -395,12 +399,13 @@ constexpr int kStripBlockLines = 64;      // strip lines per block (16 groups x 
 // zero outside the image.  Branch-free: the address is clamped into the image
 // (so every load is in bounds) and the value is zeroed afterwards, which lets
 // all 16 loads issue back to back.
-// EDGE false: columns [x0, x0 + 256) lie in the image.
+// EDGE false: columns [x0, x0 + 256) lie in the image.  Lines [lo, hi) are
+// held in device memory; the clamped address always falls among them.
 template <bool EDGE>
-__device__ __forceinline__ void load_line256(const unsigned* __restrict__ im, int inx, int iny, int gy, int x0, int j,
-                                             unsigned (&u)[16]) {
+__device__ __forceinline__ void load_line256(const unsigned* __restrict__ im, int inx, int iny, int lo, int hi, int gy,
+                                             int x0, int j, unsigned (&u)[16]) {
   const bool oky = (unsigned)gy < (unsigned)iny;
-  const unsigned* line = im + (size_t)min(max(gy, 0), iny - 1) * inx;
+  const unsigned* line = im + (ptrdiff_t)min(max(gy, lo), hi - 1) * inx;
   if (!EDGE) {
     const unsigned* p = line + x0 + j;
 #pragma unroll
@@ -548,7 +553,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_stripA256(DevParams P, const vo
   // slave line is loaded before the master transforms, the next master line
   // before the slave and z transforms.
   unsigned um[16];
-  load_line256<EDGE>(im_m, P.m_nx, P.m_ny, S.my0 + first + g, S.mx0, j, um);
+  load_line256<EDGE>(im_m, P.m_nx, P.m_ny, P.m_lo, P.m_hi, S.my0 + first + g, S.mx0, j, um);
   __syncthreads();
   for (int it = 0; it < nit; it++) {
     const int i = first + 16 * it + g;                            // strip-relative line of this group
@@ -557,11 +562,11 @@ __global__ void __launch_bounds__(kThreads, 2) k_stripA256(DevParams P, const vo
     const size_t l = (size_t)(S.line0 + i);
     const bool wm = act && needM[l], wu = act && needU[l];
     unsigned us[16];
-    load_line256<EDGE>(im_s, P.s_nx, P.s_ny, S.sy0 + i, S.sx0, j, us);
+    load_line256<EDGE>(im_s, P.s_nx, P.s_ny, P.s_lo, P.s_hi, S.sy0 + i, S.sx0, j, us);
     {
       float ea[8], oa[8];
       interp_line256<FMT>(um, j, tile, twl, sph, ea, oa);
-      if (more) load_line256<EDGE>(im_m, P.m_nx, P.m_ny, S.my0 + i + 16, S.mx0, j, um);
+      if (more) load_line256<EDGE>(im_m, P.m_nx, P.m_ny, P.m_lo, P.m_hi, S.my0 + i + 16, S.mx0, j, um);
       LineSum<FMT> s;
       float2* o = (float2*)(amp1 + l * 256);
 #pragma unroll
@@ -1515,6 +1520,17 @@ struct xcorr_hip_ctx {
   std::vector<KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<int> last_perm, last_rb;    // last batch: location index and row base of each slot
+  // streamed images: copy stream, two band buffers per image, pinned chunks
+  hipStream_t cstream = nullptr;
+  void* band[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [image][buffer]
+  size_t band_bytes[2] = {0, 0};
+  void* chunk[2] = {nullptr, nullptr};
+  size_t chunk_bytes = 0;
+  hipEvent_t ev_chunk[2] = {nullptr, nullptr}, ev_loaded[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
+  bool chunk_used[2] = {false, false};
+  int chunk_next = 0;
+  xcorr_hip_result* d_res = nullptr;
+  size_t d_res_n = 0;
 };
 
 static std::string g_create_err;
@@ -1747,10 +1763,25 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   return 0;
 }
 
+static void free_stream(xcorr_hip_ctx* c) {
+  if (c->cstream) (void)hipStreamSynchronize(c->cstream);
+  for (int i = 0; i < 2; i++) {
+    for (int b = 0; b < 2; b++)
+      if (c->band[i][b]) (void)hipFree(c->band[i][b]);
+    if (c->chunk[i]) (void)hipHostFree(c->chunk[i]);
+    hipEvent_t* evs[3] = {&c->ev_chunk[i], &c->ev_loaded[i], &c->ev_done[i]};
+    for (hipEvent_t* e : evs)
+      if (*e) (void)hipEventDestroy(*e);
+  }
+  if (c->d_res) (void)hipFree(c->d_res);
+  if (c->cstream) (void)hipStreamDestroy(c->cstream);
+}
+
 extern "C" void xcorr_hip_destroy(xcorr_hip_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_stream(c);
   free_scratch(c);
   if (c->own_master) (void)hipFree(c->own_master);
   if (c->own_slave) (void)hipFree(c->own_slave);
@@ -1795,6 +1826,7 @@ extern "C" int xcorr_hip_set_images(xcorr_hip_ctx* c, const void* master, int m_
     c->d_slave = c->own_slave;
   }
   c->dp.m_nx = m_nx; c->dp.m_ny = m_ny; c->dp.s_nx = s_nx; c->dp.s_ny = s_ny;
+  c->dp.m_lo = 0; c->dp.m_hi = m_ny; c->dp.s_lo = 0; c->dp.s_hi = s_ny;
   c->images = true;
   return 0;
 }
@@ -2145,6 +2177,161 @@ static int time_search(xcorr_hip_ctx* c, const xcorr_hip_window*, int nb, bool t
     hipLaunchKernelGGL(k_time_peak, dim3(nb), dim3(kThreads), 0, c->stream, d, (const double*)c->surfd, c->tpeak,
                        c->tcave);
   });
+}
+
+// --------------------------------------------------------------------------
+// Streamed images: windows are grouped into bands of lines; the lines of band
+// k + 1 are read (on this thread, through the caller's reader) and copied on
+// a second stream while band k is correlated.
+// --------------------------------------------------------------------------
+extern "C" int xcorr_hip_correlate_stream(xcorr_hip_ctx* c, xcorr_hip_line_reader reader, void* user, int m_nx,
+                                          int m_ny, int s_nx, int s_ny, int band_lines,
+                                          const xcorr_hip_window* win, int n, xcorr_hip_result* out) {
+  if (!c) return XCORR_HIP_EINVAL;
+  if (!reader || m_nx <= 0 || m_ny <= 0 || s_nx <= 0 || s_ny <= 0 || n < 0 || (n > 0 && (!win || !out)))
+    return fail(c, XCORR_HIP_EINVAL, "correlate_stream: bad arguments");
+  if (n == 0) return 0;
+  HIPCK(c, hipSetDevice(c->device));
+  const int NY = c->dp.NY;
+  long long C = band_lines;
+  if (C <= 0) {                                   // about 1.5 GB of band buffers per image
+    const long long per_line = 4LL * std::max(m_nx, s_nx);
+    C = std::max<long long>(2LL * NY, std::min<long long>(16384, ((long long)768 << 20) / per_line));
+  }
+  // lines each window needs (master and slave lines that exist)
+  std::vector<int> lo(n), hi(n), ord(n);
+  for (int i = 0; i < n; i++) {
+    long long a = LLONG_MAX, b = LLONG_MIN;
+    const long long ma = std::max<long long>(win[i].my0, 0), mb = std::min<long long>((long long)win[i].my0 + NY, m_ny);
+    const long long sa = std::max<long long>(win[i].sy0, 0), sb = std::min<long long>((long long)win[i].sy0 + NY, s_ny);
+    if (ma < mb) { a = std::min(a, ma); b = std::max(b, mb); }
+    if (sa < sb) { a = std::min(a, sa); b = std::max(b, sb); }
+    if (a > b) a = b = 0;                         // no lines in either image
+    if (b - a > C) return fail(c, XCORR_HIP_EINVAL, "correlate_stream: a window spans more lines than a band");
+    lo[i] = (int)a;
+    hi[i] = (int)b;
+    ord[i] = i;
+  }
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return lo[x] < lo[y]; });
+  struct Band { int b0, b1, begin, end; };
+  std::vector<Band> bands;
+  for (int i = 0; i < n;) {
+    Band B{lo[ord[i]], lo[ord[i]], i, i};
+    while (i < n && (long long)hi[ord[i]] <= (long long)B.b0 + C) {
+      B.b1 = std::max(B.b1, hi[ord[i]]);
+      i++;
+    }
+    B.end = i;
+    bands.push_back(B);
+  }
+  // buffers (kept for later calls)
+  const size_t need[2] = {(size_t)C * m_nx * 4, (size_t)C * s_nx * 4};
+  for (int img = 0; img < 2; img++)
+    if (c->band_bytes[img] < need[img]) {
+      for (int b = 0; b < 2; b++) {
+        if (c->band[img][b]) (void)hipFree(c->band[img][b]);
+        c->band[img][b] = nullptr;
+      }
+      c->band_bytes[img] = 0;
+      for (int b = 0; b < 2; b++)
+        if (hipMalloc(&c->band[img][b], need[img]) != hipSuccess)
+          return fail(c, XCORR_HIP_ENOMEM, "correlate_stream: band buffer allocation failed");
+      c->band_bytes[img] = need[img];
+    }
+  const int K = (int)std::min<long long>(C, 256);          // lines per pinned chunk
+  const size_t cb = (size_t)K * std::max(m_nx, s_nx) * 4;
+  if (c->chunk_bytes < cb) {
+    if (c->cstream) HIPCK(c, hipStreamSynchronize(c->cstream));
+    for (int i = 0; i < 2; i++) {
+      if (c->chunk[i]) (void)hipHostFree(c->chunk[i]);
+      c->chunk[i] = nullptr;
+      c->chunk_used[i] = false;
+    }
+    c->chunk_bytes = 0;
+    for (int i = 0; i < 2; i++)
+      if (hipHostMalloc(&c->chunk[i], cb, hipHostMallocDefault) != hipSuccess)
+        return fail(c, XCORR_HIP_ENOMEM, "correlate_stream: pinned staging allocation failed");
+    c->chunk_bytes = cb;
+  }
+  if (!c->cstream) HIPCK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
+  for (int i = 0; i < 2; i++) {
+    if (!c->ev_chunk[i]) HIPCK(c, hipEventCreateWithFlags(&c->ev_chunk[i], hipEventDisableTiming));
+    if (!c->ev_loaded[i]) HIPCK(c, hipEventCreateWithFlags(&c->ev_loaded[i], hipEventDisableTiming));
+    if (!c->ev_done[i]) HIPCK(c, hipEventCreateWithFlags(&c->ev_done[i], hipEventDisableTiming));
+  }
+  if (c->d_res_n < (size_t)n) {
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if (c->d_res) (void)hipFree(c->d_res);
+    c->d_res = nullptr;
+    c->d_res_n = 0;
+    if (hipMalloc((void**)&c->d_res, (size_t)n * sizeof(xcorr_hip_result)) != hipSuccess)
+      return fail(c, XCORR_HIP_ENOMEM, "correlate_stream: result buffer allocation failed");
+    c->d_res_n = n;
+  }
+  const int nx_of[2] = {m_nx, s_nx}, ny_of[2] = {m_ny, s_ny};
+  auto load = [&](int k) -> int {
+    const Band& B = bands[k];
+    const int buf = k & 1;
+    if (k >= 2) HIPCK(c, hipStreamWaitEvent(c->cstream, c->ev_done[buf], 0));   // band k - 2 is done with it
+    for (int img = 0; img < 2; img++) {
+      const int nx = nx_of[img];
+      const int a = std::max(B.b0, 0), b = std::min(B.b1, ny_of[img]);
+      for (int l = a; l < b; l += K) {
+        const int cnt = std::min(K, b - l);
+        const int slot = c->chunk_next;                 // the chunk copied two chunks ago
+        c->chunk_next ^= 1;
+        if (c->chunk_used[slot]) HIPCK(c, hipEventSynchronize(c->ev_chunk[slot]));
+        if (reader(user, img, l, cnt, c->chunk[slot]))
+          return fail(c, XCORR_HIP_EINVAL, "correlate_stream: the line reader failed");
+        char* dst = (char*)c->band[img][buf] + (size_t)(l - B.b0) * nx * 4;
+        HIPCK(c, hipMemcpyAsync(dst, c->chunk[slot], (size_t)cnt * nx * 4, hipMemcpyHostToDevice, c->cstream));
+        HIPCK(c, hipEventRecord(c->ev_chunk[slot], c->cstream));
+        c->chunk_used[slot] = true;
+      }
+    }
+    HIPCK(c, hipEventRecord(c->ev_loaded[buf], c->cstream));
+    return 0;
+  };
+  // the images set with set_images come back afterwards
+  const void* keep_m = c->d_master;
+  const void* keep_s = c->d_slave;
+  const bool keep_images = c->images;
+  const DevParams keep_dp = c->dp;
+  std::vector<xcorr_hip_window> sub(n);
+  for (int i = 0; i < n; i++) sub[i] = win[ord[i]];
+  int rc = load(0);
+  for (size_t k = 0; !rc && k < bands.size(); k++) {
+    const Band& B = bands[k];
+    const int buf = (int)(k & 1);
+    rc = -1;
+    HIPCK(c, hipStreamWaitEvent(c->stream, c->ev_loaded[buf], 0));
+    // device pointers at line 0 of each image, so that line y sits at im + y nx
+    c->d_master = (const char*)c->band[0][buf] - (ptrdiff_t)B.b0 * m_nx * 4;
+    c->d_slave = (const char*)c->band[1][buf] - (ptrdiff_t)B.b0 * s_nx * 4;
+    c->dp.m_nx = m_nx; c->dp.m_ny = m_ny; c->dp.s_nx = s_nx; c->dp.s_ny = s_ny;
+    c->dp.m_lo = c->dp.s_lo = B.b0;
+    c->dp.m_hi = c->dp.s_hi = (int)(B.b0 + C);
+    c->images = true;
+    rc = xcorr_hip_correlate(c, sub.data() + B.begin, B.end - B.begin, c->d_res + B.begin,
+                             XCORR_HIP_DEVICE_PTRS | XCORR_HIP_HOST_WINDOWS | XCORR_HIP_NO_SYNC);
+    if (rc) break;
+    HIPCK(c, hipEventRecord(c->ev_done[buf], c->stream));
+    if (k + 1 < bands.size()) rc = load((int)k + 1);
+  }
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->cstream);
+  c->d_master = keep_m;
+  c->d_slave = keep_s;
+  c->images = keep_images;
+  c->dp = keep_dp;
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(c, XCORR_HIP_EDEVICE, std::string("correlate_stream: ") + hipGetErrorString(e));
+  std::vector<xcorr_hip_result> tmp(n);
+  HIPCK(c, hipMemcpy(tmp.data(), c->d_res, (size_t)n * sizeof(xcorr_hip_result), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; i++) out[ord[i]] = tmp[i];
+  // dumps refer to indices of this call
+  for (int& q : c->last_perm) q = ord[q + (int)bands.back().begin];
+  return 0;
 }
 
 extern "C" int xcorr_hip_kernel_times(xcorr_hip_ctx* c, const char** names, float* ms, int* launches, int cap) {

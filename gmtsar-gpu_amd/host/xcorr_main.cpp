@@ -1,9 +1,11 @@
 // Drop-in `xcorr` for MI355X: same command line, PRM/SLC inputs and output file
 // as GMTSAR's xcorr (xcorr.cu:437-490), with the correlation work done by the
 // HIP engine (include/xcorr_hip.h).
+#include <fcntl.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include <chrono>
 #include <vector>
@@ -34,6 +36,28 @@ static const char* USAGE =
     "\nuse fitoffset.csh to convert output to PRM format\n"
     "\nExample:\n"
     "xcorr IMG-HH-ALPSRP075880660-H1.0__A.PRM IMG-HH-ALPSRP129560660-H1.0__A.PRM -nx 20 -ny 50 \n";
+
+// The SLC pair, read line band by line band (read_complex_short2 /
+// read_real_float2, read_xcorr_data.c:11-49, read whole lines the same way).
+struct SlcPair {
+  int fd[2];
+  int nx[2];
+};
+
+static int slc_reader(void* user, int image, int first, int count, void* dst) {
+  const SlcPair* p = (const SlcPair*)user;
+  const size_t bytes = (size_t)count * p->nx[image] * 4;
+  off_t off = (off_t)first * p->nx[image] * 4;
+  size_t got = 0;
+  while (got < bytes) {
+    const ssize_t r = pread(p->fd[image], (char*)dst + got, bytes - got, off + (off_t)got);
+    if (r < 0) return -1;
+    if (r == 0) break;                 // past the end of the file: zeros
+    got += (size_t)r;
+  }
+  if (got < bytes) memset((char*)dst + got, 0, bytes - got);
+  return 0;
+}
 
 int main(int argc, char** argv) {
   xh_config xc;
@@ -75,18 +99,26 @@ int main(int argc, char** argv) {
     fprintf(stderr, " %s %s \n", "xcorr engine:", xcorr_hip_last_error(nullptr));
     return 1;
   }
-  std::vector<char> m((size_t)xc.m_nx * xc.m_ny * 4 + 4), s((size_t)xc.s_nx * xc.s_ny * 4 + 4);
-  if (xh_read_slc(xc.data1_name, xc.m_nx, xc.m_ny, m.data()) ||
-      xh_read_slc(xc.data2_name, xc.s_nx, xc.s_ny, s.data())) {
-    fprintf(stderr, " %s %s \n", "Cannot open SLC_file", xc.data1_name);
+  SlcPair pair;
+  pair.fd[0] = open(xc.data1_name, O_RDONLY);
+  pair.fd[1] = open(xc.data2_name, O_RDONLY);
+  pair.nx[0] = xc.m_nx;
+  pair.nx[1] = xc.s_nx;
+  if (pair.fd[0] < 0 || pair.fd[1] < 0) {
+    fprintf(stderr, " %s %s \n", "Cannot open SLC_file", pair.fd[0] < 0 ? xc.data1_name : xc.data2_name);
     return 1;
   }
-  rc = xcorr_hip_set_images(ctx, m.data(), xc.m_nx, xc.m_ny, s.data(), xc.s_nx, xc.s_ny, 0);
   std::vector<xcorr_hip_window> win((size_t)(nlocs > 0 ? nlocs : 1));
   for (int i = 0; i < nlocs; i++)
     xh_window(&xc, xy[2 * i], xy[2 * i + 1], &win[i].mx0, &win[i].my0, &win[i].sx0, &win[i].sy0);
   std::vector<xcorr_hip_result> res((size_t)(nlocs > 0 ? nlocs : 1));
-  if (!rc) rc = xcorr_hip_correlate(ctx, win.data(), nlocs, res.data(), 0);
+  // lines per band (tuning; the default sizes bands from the line length)
+  const char* bl = getenv("XCORR_HIP_BAND_LINES");
+  const int band_lines = bl ? atoi(bl) : 0;
+  rc = xcorr_hip_correlate_stream(ctx, slc_reader, &pair, xc.m_nx, xc.m_ny, xc.s_nx, xc.s_ny, band_lines, win.data(),
+                                  nlocs, res.data());
+  close(pair.fd[0]);
+  close(pair.fd[1]);
   if (rc) {
     fprintf(stderr, " %s %s \n", "xcorr engine:", xcorr_hip_last_error(ctx));
     xcorr_hip_destroy(ctx);

@@ -115,6 +115,25 @@ int xcorr_hip_set_images(xcorr_hip_ctx *ctx, const void *master, int m_nx, int m
 int xcorr_hip_correlate(xcorr_hip_ctx *ctx, const xcorr_hip_window *win, int n,
                         xcorr_hip_result *out, unsigned flags);
 
+/* Streamed images (pairs larger than device memory, or file reads overlapped
+ * with the correlation).  reader(user, image, first, count, dst) copies lines
+ * [first, first + count) of image 0 (master, m_nx samples per line) or image 1
+ * (slave, s_nx samples per line) into dst: row-major, 4 bytes per sample, in
+ * the context's format; lines past the end of a file read as zero.  It returns
+ * 0 on success.  Lines are requested in increasing order per band. */
+typedef int (*xcorr_hip_line_reader)(void *user, int image, int first, int count, void *dst);
+
+/* Correlate n locations (host windows, host results) of a pair read through
+ * `reader`.  Windows are grouped into bands of band_lines lines (<= 0: about
+ * 768 MB of lines per image); while one band is correlated the next one is
+ * read and copied to the device on a second stream.  A window whose master and
+ * slave lines together span more than band_lines lines is rejected.  Replaces
+ * the per-row reads read_xcorr_data.c:52-85 and the patch H2D copies
+ * xcorr.cu:326-340.  Images set with xcorr_hip_set_images are kept. */
+int xcorr_hip_correlate_stream(xcorr_hip_ctx *ctx, xcorr_hip_line_reader reader, void *user,
+                               int m_nx, int m_ny, int s_nx, int s_ny, int band_lines,
+                               const xcorr_hip_window *win, int n, xcorr_hip_result *out);
+
 /* Wait for all work queued on the context's stream. */
 int xcorr_hip_sync(xcorr_hip_ctx *ctx);
 
