@@ -23,11 +23,15 @@ HOST_LIB_PATH = os.path.join(HERE, "libxcorr_host.so")
 TIME, TIME4, FREQ = 0, 1, 2
 CINT16, FLOAT32 = 0, 1
 DEVICE_PTRS, NO_SYNC, TIMING, HOST_WINDOWS = 1, 2, 4, 8
+TUNE, TUNE_RADIX = 16, 32
 
 EXPORTED_SYMBOLS = (
     "xcorr_hip_abi_version", "xcorr_hip_default_params", "xcorr_hip_create", "xcorr_hip_set_images",
     "xcorr_hip_correlate", "xcorr_hip_sync", "xcorr_hip_stream", "xcorr_hip_kernel_times",
     "xcorr_hip_dump", "xcorr_hip_destroy", "xcorr_hip_last_error", "xcorr_hip_correlate_stream",
+    "xcorr_hip_get_tuning", "xcorr_hip_set_tuning", "xcorr_hip_autotune", "xcorr_hip_multi_create",
+    "xcorr_hip_multi_run", "xcorr_hip_multi_sync", "xcorr_hip_multi_stream", "xcorr_hip_multi_classes",
+    "xcorr_hip_multi_destroy", "xcorr_hip_multi_last_error",
 )
 
 # int reader(void *user, int image, int first, int count, void *dst)
@@ -40,6 +44,27 @@ class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "xsearch", "ysearch", "nx_corr", "ny_corr", "npx", "npy", "nxc", "nyc", "ri",
         "interp_flag", "interp_factor", "n2x", "n2y", "corr_flag", "format")]
+
+
+class Tuning(ctypes.Structure):
+    """``xcorr_hip_tuning``: launch settings of a context."""
+    _fields_ = [("batch", ctypes.c_int32), ("rows_per_block", ctypes.c_int32), ("cols_per_block", ctypes.c_int32),
+                ("threads", ctypes.c_int32), ("radix_x", ctypes.c_int32 * 4), ("radix_y", ctypes.c_int32 * 4),
+                ("ms_per_batch", ctypes.c_float), ("candidates", ctypes.c_int32)]
+
+    def as_dict(self):
+        rx = [int(v) for v in self.radix_x if v]
+        ry = [int(v) for v in self.radix_y if v]
+        return {"batch": self.batch, "rows_per_block": self.rows_per_block, "cols_per_block": self.cols_per_block,
+                "threads": self.threads, "radix_x": rx, "radix_y": ry,
+                "ms_per_batch": round(float(self.ms_per_batch), 4), "candidates": self.candidates}
+
+
+class Pair(ctypes.Structure):
+    """``xcorr_hip_pair``: one SLC pair of a multi-pair workload."""
+    _fields_ = [("params", Params), ("master", ctypes.c_void_p), ("m_nx", ctypes.c_int32), ("m_ny", ctypes.c_int32),
+                ("slave", ctypes.c_void_p), ("s_nx", ctypes.c_int32), ("s_ny", ctypes.c_int32),
+                ("win", ctypes.c_void_p), ("n", ctypes.c_int32), ("out", ctypes.c_void_p)]
 
 
 WINDOW_DTYPE = np.dtype([("mx0", "<i4"), ("my0", "<i4"), ("sx0", "<i4"), ("sy0", "<i4")])
@@ -82,6 +107,21 @@ def load_library():
                                                ctypes.c_int, ctypes.c_int, P, ctypes.c_int, P]
     lib.xcorr_hip_last_error.argtypes = [P]
     lib.xcorr_hip_last_error.restype = ctypes.c_char_p
+    T = ctypes.POINTER(Tuning)
+    lib.xcorr_hip_get_tuning.argtypes = [P, T]
+    lib.xcorr_hip_set_tuning.argtypes = [P, T]
+    lib.xcorr_hip_autotune.argtypes = [P, P, ctypes.c_int, ctypes.c_uint, T]
+    lib.xcorr_hip_multi_create.argtypes = [ctypes.POINTER(Pair), ctypes.c_int, ctypes.c_int, ctypes.c_uint,
+                                           ctypes.POINTER(P)]
+    lib.xcorr_hip_multi_run.argtypes = [P, ctypes.c_uint]
+    lib.xcorr_hip_multi_sync.argtypes = [P]
+    lib.xcorr_hip_multi_stream.argtypes = [P]
+    lib.xcorr_hip_multi_stream.restype = P
+    lib.xcorr_hip_multi_classes.argtypes = [P, ctypes.POINTER(Params), T, P, ctypes.c_int]
+    lib.xcorr_hip_multi_destroy.argtypes = [P]
+    lib.xcorr_hip_multi_destroy.restype = None
+    lib.xcorr_hip_multi_last_error.argtypes = [P]
+    lib.xcorr_hip_multi_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 
@@ -208,6 +248,32 @@ class Engine:
     def sync(self):
         self._check(self.lib.xcorr_hip_sync(self.h), "xcorr_hip_sync")
 
+    def get_tuning(self):
+        t = Tuning()
+        self._check(self.lib.xcorr_hip_get_tuning(self.h, ctypes.byref(t)), "xcorr_hip_get_tuning")
+        return t
+
+    def set_tuning(self, **kw):
+        """Change launch settings (batch, rows_per_block, cols_per_block, threads,
+        radix_x / radix_y as lists of radices); unnamed ones are kept."""
+        t = self.get_tuning()
+        for k, v in kw.items():
+            if k in ("radix_x", "radix_y"):
+                arr = getattr(t, k)
+                for i in range(4):
+                    arr[i] = v[i] if v is not None and i < len(v) else 0
+            else:
+                setattr(t, k, int(v))
+        self._check(self.lib.xcorr_hip_set_tuning(self.h, ctypes.byref(t)), "xcorr_hip_set_tuning")
+        return t
+
+    def autotune(self, windows, radix=False):
+        w = _as_windows(windows)
+        t = Tuning()
+        rc = self.lib.xcorr_hip_autotune(self.h, _ptr(w), w.shape[0], TUNE_RADIX if radix else 0, ctypes.byref(t))
+        self._check(rc, "xcorr_hip_autotune")
+        return t
+
     def stream(self):
         return self.lib.xcorr_hip_stream(self.h)
 
@@ -234,6 +300,80 @@ class Engine:
     def close(self):
         if getattr(self, "h", None):
             self.lib.xcorr_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Multi:
+    """Multi-pair workload run as one HIP graph launch (xcorr_hip_multi_*).
+
+    pairs: list of dicts with keys params (Params), master, slave (torch device
+    tensors, (lines, samples, 2) int16 or (lines, samples) float32), windows
+    (WINDOW_DTYPE array) and out (torch device uint8 tensor of
+    n * RESULT_DTYPE.itemsize bytes)."""
+
+    def __init__(self, pairs, device=-1, tune=False, tune_radix=False):
+        self.lib = load_library()
+        arr = (Pair * max(1, len(pairs)))()
+        self._keep = []
+        for i, q in enumerate(pairs):
+            w = _as_windows(q["windows"])
+            m, s = q["master"], q["slave"]
+            self._keep += [w, m, s, q["out"]]
+            a = arr[i]
+            a.params = q["params"]
+            a.master, a.m_nx, a.m_ny = m.data_ptr(), m.shape[1], m.shape[0]
+            a.slave, a.s_nx, a.s_ny = s.data_ptr(), s.shape[1], s.shape[0]
+            a.win, a.n = w.ctypes.data, w.shape[0]
+            a.out = q["out"].data_ptr()
+            if q["out"].numel() < w.shape[0] * RESULT_DTYPE.itemsize:
+                raise ValueError("pair %d: out holds fewer than n results" % i)
+        flags = (TUNE if tune or tune_radix else 0) | (TUNE_RADIX if tune_radix else 0)
+        h = ctypes.c_void_p()
+        rc = self.lib.xcorr_hip_multi_create(arr, len(pairs), int(device), flags, ctypes.byref(h))
+        if rc != 0:
+            raise XcorrError("xcorr_hip_multi_create failed (%d): %s"
+                             % (rc, self.lib.xcorr_hip_multi_last_error(None).decode()))
+        self.h = h
+        self.npairs = len(pairs)
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise XcorrError("%s failed (%d): %s" % (what, rc, self.lib.xcorr_hip_multi_last_error(self.h).decode()))
+        return rc
+
+    def run(self, flags=0):
+        self._check(self.lib.xcorr_hip_multi_run(self.h, flags), "xcorr_hip_multi_run")
+
+    def sync(self):
+        self._check(self.lib.xcorr_hip_multi_sync(self.h), "xcorr_hip_multi_sync")
+
+    def stream(self):
+        return self.lib.xcorr_hip_multi_stream(self.h)
+
+    def classes(self):
+        """[(params, tuning dict)] per class and the class of every pair."""
+        cap = 64
+        ps = (Params * cap)()
+        ts = (Tuning * cap)()
+        pc = np.zeros(max(1, self.npairs), np.int32)
+        n = self._check(self.lib.xcorr_hip_multi_classes(self.h, ps, ts, _ptr(pc), cap), "xcorr_hip_multi_classes")
+        return [(ps[k], ts[k].as_dict()) for k in range(min(n, cap))], pc[:self.npairs]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.xcorr_hip_multi_destroy(self.h)
             self.h = None
 
     def __del__(self):

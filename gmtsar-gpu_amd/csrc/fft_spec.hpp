@@ -5,7 +5,8 @@
 // the only fused multiply-adds are the explicit fmaf calls.
 //
 //   complex multiply   re = fma(a.re, b.re, -(a.im*b.im)), im = fma(a.re, b.im, a.im*b.re)
-//   FFT                Stockham autosort passes, radix plan per length (plan_for),
+//   FFT                Stockham autosort passes, radix plan per length (plan_for
+//                      by default; the generic LDS kernels take any valid plan),
 //                      DFT-2/4/8/16 butterflies, twiddles e^{+2 pi i m/n} from a
 //                      host-built table (negated imaginary part for forward)
 //   amplitude          sqrtf(fma(re, re, im*im))
@@ -177,8 +178,8 @@ __device__ __forceinline__ void lds_pass(const float2* __restrict__ src, float2*
 // __syncthreads() before (data written) -- this function syncs between and
 // after the passes.
 template <int DIR>
-__device__ float2* lds_fft_rows(float2* a, float2* b, int n, int nrows, const float2* __restrict__ tw) {
-  const FftPlan pl = plan_for(n);
+__device__ float2* lds_fft_rows(float2* a, float2* b, int n, int nrows, const float2* __restrict__ tw,
+                                const FftPlan& pl) {
   float2* src = a;
   float2* dst = b;
   int L = 1;
@@ -196,6 +197,21 @@ __device__ float2* lds_fft_rows(float2* a, float2* b, int n, int nrows, const fl
     __syncthreads();
   }
   return src;
+}
+template <int DIR>
+__device__ float2* lds_fft_rows(float2* a, float2* b, int n, int nrows, const float2* __restrict__ tw) {
+  return lds_fft_rows<DIR>(a, b, n, nrows, tw, plan_for(n));
+}
+
+// A radix plan is valid for n when its radices (2, 4, 8 or 16) multiply to n.
+__host__ __device__ inline bool plan_valid(const FftPlan& p, int n) {
+  if (p.np < 0 || p.np > 4) return false;
+  long prod = 1;
+  for (int i = 0; i < p.np; i++) {
+    if (p.R[i] != 2 && p.R[i] != 4 && p.R[i] != 8 && p.R[i] != 16) return false;
+    prod *= p.R[i];
+  }
+  return prod == n;
 }
 
 }  // namespace xc

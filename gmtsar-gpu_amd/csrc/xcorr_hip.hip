@@ -24,13 +24,15 @@
 
 #include <algorithm>
 #include <climits>
+#include <functional>
 #include <string>
 #include <vector>
 
 namespace xc {
 
 constexpr int kThreads = 256;
-constexpr int kTile = 2048;     // complex elements per LDS FFT buffer
+constexpr int kMaxThreads = 512;  // generic LDS row/column kernels: launched with 64..512 threads
+constexpr int kTile = 2048;     // complex elements per LDS FFT buffer (default tile)
 
 struct DevParams {
   int NX, NY, N;
@@ -38,6 +40,7 @@ struct DevParams {
   int ri, interp_flag, ifc, n2x, n2y, Lx, Ly;
   int corr_flag, format;
   int G1, G4, C3;                 // rows per block (k_range_ampl/k_rowfft, k_rowifft_surf), columns per block
+  FftPlan plx, ply;               // radix plans of the generic LDS transforms along x (NX) and y (NY)
   int groups4;                    // k_rowifft_surf blocks per patch
   int tLR, tRC;                   // k_time_search: surface rows per block, template rows per LDS chunk
   int tw_nx, tw_ny, tw_lx, tw_ly, phi;   // offsets into the table buffer
@@ -126,14 +129,14 @@ __device__ __forceinline__ double patch_mean(unsigned long long s, int N) {
 // stage (a): gather + range interpolation + amplitude + fixed-point sums
 // xcorr.cu:237-292, fft_interpolate_routines.cu:76-113, conv_ampl_demean.cu:98-108, 703-726
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) k_range_ampl(DevParams P, const void* __restrict__ master,
+__global__ void __launch_bounds__(kMaxThreads) k_range_ampl(DevParams P, const void* __restrict__ master,
                                                          const void* __restrict__ slave,
                                                          const xcorr_hip_window* __restrict__ win,
                                                          float* __restrict__ amp,
                                                          unsigned long long* __restrict__ sums,
                                                          const float2* __restrict__ tabs) {
   extern __shared__ float2 smem[];
-  __shared__ long long red[kThreads / 64];
+  __shared__ long long red[kMaxThreads / 64];
   const int NX = P.NX, G = P.G1, nel = G * NX;
   const int groups = P.NY / G;
   const int g = blockIdx.x % groups;
@@ -151,7 +154,7 @@ __global__ void __launch_bounds__(kThreads) k_range_ampl(DevParams P, const void
   float2* Cb = B + nel;                       // only with ri > 2
   float* ab = (float*)(smem + (ri > 2 ? 3 : 2) * nel);
 
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     const int row = e / NX, x = e - row * NX;
     const int gy = y0 + row, gx = x0 + x;
     float2 v = make_float2(0.0f, 0.0f);
@@ -169,28 +172,28 @@ __global__ void __launch_bounds__(kThreads) k_range_ampl(DevParams P, const void
   }
   __syncthreads();
   if (ri == 1) {
-    for (int e = threadIdx.x; e < nel; e += kThreads) ab[e] = ampl(A[e]);
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) ab[e] = ampl(A[e]);
   } else {
-    for (int e = threadIdx.x; e < nel; e += kThreads) {
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
       const int row = e / NX, j = e - row * NX;
       const int t = NX / 2 + j;
       if (t % ri == 0) ab[e] = ampl(A[row * NX + t / ri]);
     }
     const float2* twn = tabs + P.tw_nx;
-    float2* X = lds_fft_rows<-1>(A, B, NX, G, twn);
+    float2* X = lds_fft_rows<-1>(A, B, NX, G, twn, P.plx);
     float2* other = (X == A) ? B : A;
     const float sc = 1.0f / (float)NX;
     for (int r = 1; r < ri; r++) {
       const float2* phi = tabs + P.phi + (size_t)(r - 1) * NX;
       float2* Y = other;
       float2* Ys = (ri > 2) ? Cb : X;         // scratch for the inverse passes
-      for (int e = threadIdx.x; e < nel; e += kThreads) {
+      for (int e = threadIdx.x; e < nel; e += blockDim.x) {
         const int k = e & (NX - 1);
         Y[e] = cmul(X[e], phi[k]);
       }
       __syncthreads();
-      float2* Rr = lds_fft_rows<+1>(Y, Ys, NX, G, twn);
-      for (int e = threadIdx.x; e < nel; e += kThreads) {
+      float2* Rr = lds_fft_rows<+1>(Y, Ys, NX, G, twn, P.plx);
+      for (int e = threadIdx.x; e < nel; e += blockDim.x) {
         const int row = e / NX, j = e - row * NX;
         const int t = NX / 2 + j;
         if (t % ri == r) {
@@ -206,7 +209,7 @@ __global__ void __launch_bounds__(kThreads) k_range_ampl(DevParams P, const void
   __syncthreads();
   float* out = amp + ((size_t)(b * 2 + img) * P.NY + (size_t)g * G) * NX;
   long long s = 0;
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     const float a = ab[e];
     out[e] = a;
     s += fxp20(a);
@@ -883,7 +886,7 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
 // stage (b): demean, mask, pack z = c1 + i c2, FFT along x
 // conv_ampl_demean.cu:111-139, xcorr.cu:385-401, do_freq_xcorr.cu:90-101
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) k_rowfft(DevParams P, const float* __restrict__ amp,
+__global__ void __launch_bounds__(kMaxThreads) k_rowfft(DevParams P, const float* __restrict__ amp,
                                                      const unsigned long long* __restrict__ sums,
                                                      float2* __restrict__ Z, const float2* __restrict__ tabs) {
   extern __shared__ float2 smem[];
@@ -895,16 +898,16 @@ __global__ void __launch_bounds__(kThreads) k_rowfft(DevParams P, const float* _
   float2* A = smem;
   float2* Bf = A + nel;
   // raw amplitudes; the means come off in the row spectrum (demean_row_bin)
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     const int row = e / NX, x = e - row * NX;
     const int y = g * G + row;
     const bool in = y >= P.ys && y < NY - P.ys && x >= P.xs && x < NX - P.xs;
     A[e] = make_float2(a1[e], in ? a2[e] : 0.0f);
   }
   __syncthreads();
-  float2* R = lds_fft_rows<-1>(A, Bf, NX, G, tabs + P.tw_nx);
+  float2* R = lds_fft_rows<-1>(A, Bf, NX, G, tabs + P.tw_nx, P.plx);
   float2* out = Z + (size_t)b * P.N + (size_t)g * G * NX;
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     const int row = e / NX, x = e - row * NX;
     const int y = g * G + row;
     float2 v = R[e];
@@ -917,7 +920,7 @@ __global__ void __launch_bounds__(kThreads) k_rowfft(DevParams P, const float* _
 }
 
 // FFT along y, in place on Z (columns tile per block).
-__global__ void __launch_bounds__(kThreads) k_colfft(DevParams P, float2* __restrict__ Z,
+__global__ void __launch_bounds__(kMaxThreads) k_colfft(DevParams P, float2* __restrict__ Z,
                                                      const unsigned long long* __restrict__ sums,
                                                      const float2* __restrict__ tabs) {
   extern __shared__ float2 smem[];
@@ -929,15 +932,15 @@ __global__ void __launch_bounds__(kThreads) k_colfft(DevParams P, float2* __rest
   float2* Bf = A + nel;
   const float m1NX = (float)patch_mean(sums[2 * b], P.N) * (float)NX;
   const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     const int y = e / C, c = e - y * C;
     const int kx = ct * C + c;
     A[c * NY + y] = demean_row_bin(zb[(size_t)y * NX + c], kx, y >= P.ys && y < NY - P.ys, m1NX, m2f,
                                    tabs[P.mk_x + kx]);
   }
   __syncthreads();
-  float2* R = lds_fft_rows<-1>(A, Bf, NY, C, tabs + P.tw_ny);
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
+  float2* R = lds_fft_rows<-1>(A, Bf, NY, C, tabs + P.tw_ny, P.ply);
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     const int y = e / C, c = e - y * C;
     zb[(size_t)y * NX + c] = R[c * NY + y];
   }
@@ -945,7 +948,7 @@ __global__ void __launch_bounds__(kThreads) k_colfft(DevParams P, float2* __rest
 
 // C3 = C1 conj(C2) (-1)^(kx+ky) with C1, C2 split from Z = FFT2(c1 + i c2);
 // IFFT along y; keep rows [ny_corr/2, ny_corr/2 + nyc).   do_freq_xcorr.cu:46-64, 103-113
-__global__ void __launch_bounds__(kThreads) k_product_colifft(DevParams P, const float2* __restrict__ Z,
+__global__ void __launch_bounds__(kMaxThreads) k_product_colifft(DevParams P, const float2* __restrict__ Z,
                                                               float2* __restrict__ D,
                                                               const float2* __restrict__ tabs) {
   extern __shared__ float2 smem[];
@@ -956,7 +959,7 @@ __global__ void __launch_bounds__(kThreads) k_product_colifft(DevParams P, const
   float2* A = smem;
   float2* M = A + nel;
   float2* O = M + nel;
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     const int y = e / C, c = e - y * C;
     const int kx = ct * C + c;
     const int mx = (NX - kx) & (NX - 1);
@@ -964,17 +967,17 @@ __global__ void __launch_bounds__(kThreads) k_product_colifft(DevParams P, const
     M[c * NY + y] = zb[(size_t)y * NX + mx];
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     const int c = e / NY, ky = e - c * NY;
     const int kx = ct * C + c;
     const float2 zm = M[c * NY + ((NY - ky) & (NY - 1))];
     O[e] = c3_bin(A[e], zm, kx, ky);
   }
   __syncthreads();
-  float2* R = lds_fft_rows<+1>(O, A, NY, C, tabs + P.tw_ny);
+  float2* R = lds_fft_rows<+1>(O, A, NY, C, tabs + P.tw_ny, P.ply);
   const int r0 = P.ny_corr / 2;
   float2* db = D + (size_t)b * P.nyc * NX + ct * C;
-  for (int e = threadIdx.x; e < P.nyc * C; e += kThreads) {
+  for (int e = threadIdx.x; e < P.nyc * C; e += blockDim.x) {
     const int iy = e / C, c = e - iy * C;
     db[(size_t)iy * NX + c] = R[c * NY + r0 + iy];
   }
@@ -1009,7 +1012,7 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf(DevParams P, const fl
     A[e] = v;
   }
   __syncthreads();
-  float2* R = lds_fft_rows<+1>(A, Bf, NX, G, tabs + P.tw_nx);
+  float2* R = lds_fft_rows<+1>(A, Bf, NX, G, tabs + P.tw_nx, P.plx);
   const int q0 = P.nx_corr / 2;
   const float sc = 1.0f / (float)P.N;
   float* sb = surf + (size_t)b * P.nyc * nxc;
@@ -1531,6 +1534,9 @@ struct xcorr_hip_ctx {
   int chunk_next = 0;
   xcorr_hip_result* d_res = nullptr;
   size_t d_res_n = 0;
+  // tuning (xcorr_hip_set_tuning / xcorr_hip_autotune): none of these change a result bit
+  int tune_batch = 0;                     // locations per launch (0: max_batch)
+  int nt_rows = kThreads;                 // threads of the generic row / column kernels
 };
 
 static std::string g_create_err;
@@ -1549,6 +1555,13 @@ static int fail(xcorr_hip_ctx* c, int code, const std::string& msg) {
   } while (0)
 
 static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+static bool same_plan(const FftPlan& a, const FftPlan& b) {
+  if (a.np != b.np) return false;
+  for (int i = 0; i < a.np; i++)
+    if (a.R[i] != b.R[i]) return false;
+  return true;
+}
+static long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
 
 extern "C" int xcorr_hip_abi_version(void) { return XCORR_HIP_ABI_VERSION; }
 
@@ -1643,6 +1656,8 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   d.ifc = p->interp_factor; d.n2x = p->n2x; d.n2y = p->n2y;
   d.Lx = p->n2x * p->interp_factor; d.Ly = p->n2y * p->interp_factor;
   d.corr_flag = p->corr_flag; d.format = p->format;
+  d.plx = plan_for(d.NX);
+  d.ply = plan_for(d.NY);
   d.G1 = std::min(d.NY, std::max(1, kTile / d.NX));
   d.G4 = std::min(d.nyc, std::max(1, kTile / d.NX));
   d.C3 = std::min(d.NX, std::max(1, kTile / d.NY));
@@ -1739,7 +1754,8 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
     alloc((void**)&c->Z, (c->strips ? (size_t)c->max_lines * d.NX : B * N) * sizeof(float2));
     alloc((void**)&c->D, B * d.nyc * d.NX * sizeof(float2));
     alloc((void**)&c->surf, B * ns * sizeof(float));
-    alloc((void**)&c->partial, B * d.groups4 * sizeof(double));
+    // room for the generic row path's groups too (a tuned radix plan can select it for npx = 256)
+    alloc((void**)&c->partial, B * std::max(d.groups4, d.nyc / d.G4) * sizeof(double));
   } else {
     alloc((void**)&c->surfd, B * ns * sizeof(double));
     alloc((void**)&c->tpeak, B * 4 * sizeof(int));
@@ -1896,8 +1912,9 @@ extern "C" int xcorr_hip_correlate(xcorr_hip_ctx* c, const xcorr_hip_window* win
     const int rc = run_strips(c, hw, n, out, dev_out, timing);
     if (rc) return rc;
   } else {
-    for (int off = 0; off < n; off += c->max_batch) {
-      const int nb = std::min(c->max_batch, n - off);
+    const int bcap = c->tune_batch > 0 ? std::min(c->tune_batch, c->max_batch) : c->max_batch;
+    for (int off = 0; off < n; off += bcap) {
+      const int nb = std::min(bcap, n - off);
       const xcorr_hip_window* dw = dev_win ? win + off : c->d_win;
       xcorr_hip_result* dout = dev_out ? out + off : c->d_out;
       if (!dev_win)
@@ -1918,18 +1935,16 @@ extern "C" int xcorr_hip_correlate(xcorr_hip_ctx* c, const xcorr_hip_window* win
   return 0;
 }
 
-static long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
-
 // Strip mode.  Windows are sorted by (master column, slave column, slave line
 // offset, master line); a batch takes them in that order and merges each run
-// of overlapping windows with one key into a strip.  Per batch the host
-// uploads the row base and location index of every patch, the strip of every
-// line block and the strips themselves.
-static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_hip_result* out, bool dev_out,
-                      bool timing) {
-  const DevParams& d = c->dp;
-  const int NY = d.NY, mb = c->max_batch, ML = c->max_lines, blkcap = 2 * (ML / kStripBlockLines);
-  std::vector<int>& ord = c->order;
+// of overlapping windows with one key into a strip.  Per batch the plan holds
+// the row base and location index of every patch, the strip of every line
+// block and the strips themselves (layout of plan_ints).
+struct StripBatch {
+  int nb, ns, nl, nblk, nblk_in;
+};
+
+static void strip_order(const xcorr_hip_window* w, int n, std::vector<int>& ord) {
   ord.resize(n);
   for (int i = 0; i < n; i++) ord[i] = i;
   std::sort(ord.begin(), ord.end(), [w](int a, int b) {
@@ -1941,127 +1956,163 @@ static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_
     if (A.my0 != B.my0) return A.my0 < B.my0;
     return a < b;
   });
+}
+
+// Plans the next batch (at most `cap` windows) from ord[*pos] on into h
+// (plan_ints(c) ints); advances *pos.  Returns false if a window does not fit.
+static bool plan_strip_batch(const xcorr_hip_ctx* c, const DevParams& d, const xcorr_hip_window* w,
+                             const std::vector<int>& ord, int* pos, int n, int cap, int* h, StripBatch* sb) {
+  const int NY = d.NY, mb = c->max_batch, ML = c->max_lines, blkcap = 2 * (ML / kStripBlockLines);
   const long long pad_ny = round_up(NY, kStripBlockLines);
+  int* rb = h;
+  int* perm = h + mb;
+  int* bs = h + 2 * mb;
+  StripDesc* st = (StripDesc*)(h + 2 * mb + blkcap);
+  int nb = 0, ns = 0;
+  long long L = 0, cur_end = 0;
+  while (*pos < n && nb < cap) {
+    const xcorr_hip_window& q = w[ord[*pos]];
+    bool ext = false;
+    if (ns > 0) {
+      const StripDesc& S = st[ns - 1];
+      ext = S.mx0 == q.mx0 && S.sx0 == q.sx0 && (long long)S.sy0 - S.my0 == (long long)q.sy0 - q.my0 &&
+            (long long)q.my0 <= cur_end;
+    }
+    if (ext) {
+      StripDesc& S = st[ns - 1];
+      const long long end = std::max(cur_end, (long long)q.my0 + NY);
+      const long long lines = S.line0 + round_up(end - S.my0, kStripBlockLines);
+      if (lines > ML) break;
+      cur_end = end;
+      S.nlines = (int)(end - S.my0);
+      L = lines;
+      rb[nb] = S.line0 + (q.my0 - S.my0);
+    } else {
+      if (L + pad_ny > ML) break;
+      StripDesc& S = st[ns++];
+      S.mx0 = q.mx0; S.my0 = q.my0; S.sx0 = q.sx0; S.sy0 = q.sy0;
+      S.line0 = (int)L; S.nlines = NY; S.pad0 = S.pad1 = 0;
+      cur_end = (long long)q.my0 + NY;
+      rb[nb] = (int)L;
+      L += pad_ny;
+    }
+    perm[nb] = ord[*pos];
+    nb++;
+    (*pos)++;
+  }
+  if (nb == 0) return false;
+  // line blocks: strips whose windows lie inside both images first, then the rest
+  int nblk = 0, nblk_in = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    for (int k = 0; k < ns; k++) {
+      const StripDesc& S = st[k];
+      const bool in = S.mx0 >= 0 && S.mx0 + d.NX <= d.m_nx && S.sx0 >= 0 && S.sx0 + d.NX <= d.s_nx;
+      if (in != (pass == 0)) continue;
+      for (int l0 = S.line0; l0 < S.line0 + S.nlines; l0 += kStripBlockLines) {
+        bs[2 * nblk] = k;
+        bs[2 * nblk + 1] = l0;
+        nblk++;
+      }
+    }
+    if (pass == 0) nblk_in = nblk;
+  }
+  sb->nb = nb;
+  sb->ns = ns;
+  sb->nl = (int)L;
+  sb->nblk = nblk;
+  sb->nblk_in = nblk_in;
+  return true;
+}
+
+static size_t strip_plan_bytes(const xcorr_hip_ctx* c, const StripBatch& sb) {
+  return (size_t)(2 * c->max_batch + 2 * (c->max_lines / kStripBlockLines)) * sizeof(int) +
+         (size_t)sb.ns * sizeof(StripDesc);
+}
+
+// Kernels of one strip batch whose plan is in device memory at d_plan, on
+// stream st with images (master, slave) and image dims in d.  Results go to
+// out[perm[b]] (dev_out) or to c->d_out[b].
+static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* master, const void* slave,
+                              hipStream_t st, const int* d_plan, const StripBatch& sb, xcorr_hip_result* out,
+                              bool dev_out, bool timing) {
+  const int mb = c->max_batch, ML = c->max_lines, blkcap = 2 * (ML / kStripBlockLines);
+  const int nb = sb.nb, nl = sb.nl, nblk = sb.nblk, nblk_in = sb.nblk_in;
+  const int* d_rb = d_plan;
+  const int* d_perm = d_plan + mb;
+  const int* d_bs = d_plan + 2 * mb;
+  const StripDesc* d_st = (const StripDesc*)(d_plan + 2 * mb + blkcap);
+  unsigned char* needM = c->need;
+  unsigned char* needU = c->need + ML;
+  int rc = launch_timed(c, timing, "k_need", [&] {
+    hipLaunchKernelGGL(k_need_clear, dim3((nl + 255) / 256), dim3(256), 0, st, needM, needU, nl);
+    hipLaunchKernelGGL(k_need_mark, dim3(nb), dim3(256), 0, st, d, d_rb, needM, needU, nb);
+  });
+  if (rc) return rc;
+  rc = launch_timed(c, timing, "k_stripA256", [&] {
+    const bool ci = d.format == XCORR_HIP_CINT16;
+    auto kin = ci ? k_stripA256<XCORR_HIP_CINT16, false> : k_stripA256<XCORR_HIP_FLOAT32, false>;
+    auto kedge = ci ? k_stripA256<XCORR_HIP_CINT16, true> : k_stripA256<XCORR_HIP_FLOAT32, true>;
+    if (nblk_in > 0)
+      hipLaunchKernelGGL(kin, dim3(nblk_in), dim3(kThreads), 0, st, d, master, slave, d_st, d_bs, 0,
+                         (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s, c->amp2s, c->lsum, c->Z,
+                         c->Zu, (const float2*)c->tabs);
+    if (nblk > nblk_in)
+      hipLaunchKernelGGL(kedge, dim3(nblk - nblk_in), dim3(kThreads), 0, st, d, master, slave, d_st, d_bs, nblk_in,
+                         (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s, c->amp2s, c->lsum, c->Z,
+                         c->Zu, (const float2*)c->tabs);
+  });
+  if (rc) return rc;
+  rc = launch_timed(c, timing, "k_patch_sums", [&] {
+    hipLaunchKernelGGL(k_patch_sums, dim3((nb + 3) / 4), dim3(256), 0, st, d, d_rb, (const long long*)c->lsum,
+                       c->sums, nb);
+  });
+  if (rc) return rc;
+  rc = launch_timed(c, timing, "k_colpass256", [&] {
+    hipLaunchKernelGGL(k_colpass256, dim3(nb * (d.NX / 32)), dim3(kThreads), 0, st, d, (const float2*)c->Z,
+                       (const float2*)c->Zu, d_rb, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs);
+  });
+  if (rc) return rc;
+  rc = launch_timed(c, timing, "k_rowifft_surf256", [&] {
+    hipLaunchKernelGGL(k_rowifft_surf256, dim3(nb), dim3(kThreads), 0, st, d, (const float2*)c->D, c->surf, c->peak,
+                       c->partial, (const float2*)c->tabs);
+  });
+  if (rc) return rc;
+  const size_t sh5 = d.interp_flag ? ((size_t)d.Lx * 16 + (size_t)d.Ly * 8) * sizeof(float2) : 16;
+  return launch_timed(c, timing, "k_finish", [&] {
+    hipLaunchKernelGGL(k_finish, dim3(nb), dim3(kThreads), sh5, st, d, (const float*)c->amp1s,
+                       (const float*)c->amp2s, d_rb, 0LL, dev_out ? d_perm : (const int*)nullptr,
+                       (const unsigned long long*)c->sums, (const float*)c->surf, (const double*)nullptr,
+                       (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)nullptr,
+                       (const double*)nullptr, dev_out ? out : c->d_out, (const float2*)c->tabs);
+  });
+}
+
+static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_hip_result* out, bool dev_out,
+                      bool timing) {
+  strip_order(w, n, c->order);
+  const int cap = std::min(c->max_batch, c->tune_batch > 0 ? c->tune_batch : c->max_batch);
   for (int pos = 0; pos < n;) {
     const int si = c->stage_next;
     c->stage_next ^= 1;
     if (c->stage_used[si]) HIPCK(c, hipEventSynchronize(c->ev_stage[si]));
     int* h = (int*)c->h_stage[si];
-    int* rb = h;
-    int* perm = h + mb;
-    int* bs = h + 2 * mb;
-    StripDesc* st = (StripDesc*)(h + 2 * mb + blkcap);
-    int nb = 0, ns = 0;
-    long long L = 0, cur_end = 0;
-    while (pos < n && nb < mb) {
-      const xcorr_hip_window& q = w[ord[pos]];
-      bool ext = false;
-      if (ns > 0) {
-        const StripDesc& S = st[ns - 1];
-        ext = S.mx0 == q.mx0 && S.sx0 == q.sx0 && (long long)S.sy0 - S.my0 == (long long)q.sy0 - q.my0 &&
-              (long long)q.my0 <= cur_end;
-      }
-      if (ext) {
-        StripDesc& S = st[ns - 1];
-        const long long end = std::max(cur_end, (long long)q.my0 + NY);
-        const long long lines = S.line0 + round_up(end - S.my0, kStripBlockLines);
-        if (lines > ML) break;
-        cur_end = end;
-        S.nlines = (int)(end - S.my0);
-        L = lines;
-        rb[nb] = S.line0 + (q.my0 - S.my0);
-      } else {
-        if (L + pad_ny > ML) break;
-        StripDesc& S = st[ns++];
-        S.mx0 = q.mx0; S.my0 = q.my0; S.sx0 = q.sx0; S.sy0 = q.sy0;
-        S.line0 = (int)L; S.nlines = NY; S.pad0 = S.pad1 = 0;
-        cur_end = (long long)q.my0 + NY;
-        rb[nb] = (int)L;
-        L += pad_ny;
-      }
-      perm[nb] = ord[pos];
-      nb++;
-      pos++;
-    }
-    if (nb == 0) return fail(c, XCORR_HIP_EINVAL, "correlate: a window does not fit the line capacity");
-    // line blocks: strips whose windows lie inside both images first, then the rest
-    int nblk = 0, nblk_in = 0;
-    for (int pass = 0; pass < 2; pass++) {
-      for (int k = 0; k < ns; k++) {
-        const StripDesc& S = st[k];
-        const bool in = S.mx0 >= 0 && S.mx0 + d.NX <= d.m_nx && S.sx0 >= 0 && S.sx0 + d.NX <= d.s_nx;
-        if (in != (pass == 0)) continue;
-        for (int l0 = S.line0; l0 < S.line0 + S.nlines; l0 += kStripBlockLines) {
-          bs[2 * nblk] = k;
-          bs[2 * nblk + 1] = l0;
-          nblk++;
-        }
-      }
-      if (pass == 0) nblk_in = nblk;
-    }
-    const int nl = (int)L;
-    const size_t bytes = (size_t)(2 * mb + blkcap) * sizeof(int) + (size_t)ns * sizeof(StripDesc);
-    HIPCK(c, hipMemcpyAsync(c->d_plan, h, bytes, hipMemcpyHostToDevice, c->stream));
+    StripBatch sb;
+    if (!plan_strip_batch(c, c->dp, w, c->order, &pos, n, cap, h, &sb))
+      return fail(c, XCORR_HIP_EINVAL, "correlate: a window does not fit the line capacity");
+    HIPCK(c, hipMemcpyAsync(c->d_plan, h, strip_plan_bytes(c, sb), hipMemcpyHostToDevice, c->stream));
     HIPCK(c, hipEventRecord(c->ev_stage[si], c->stream));
     c->stage_used[si] = true;
-    c->last_perm.assign(perm, perm + nb);
-    c->last_rb.assign(rb, rb + nb);
-    const int* d_rb = c->d_plan;
-    const int* d_perm = c->d_plan + mb;
-    const int* d_bs = c->d_plan + 2 * mb;
-    const StripDesc* d_st = (const StripDesc*)(c->d_plan + 2 * mb + blkcap);
-    unsigned char* needM = c->need;
-    unsigned char* needU = c->need + ML;
-    int rc = launch_timed(c, timing, "k_need", [&] {
-      hipLaunchKernelGGL(k_need_clear, dim3((nl + 255) / 256), dim3(256), 0, c->stream, needM, needU, nl);
-      hipLaunchKernelGGL(k_need_mark, dim3(nb), dim3(256), 0, c->stream, d, d_rb, needM, needU, nb);
-    });
-    if (rc) return rc;
-    rc = launch_timed(c, timing, "k_stripA256", [&] {
-      const bool ci = d.format == XCORR_HIP_CINT16;
-      auto kin = ci ? k_stripA256<XCORR_HIP_CINT16, false> : k_stripA256<XCORR_HIP_FLOAT32, false>;
-      auto kedge = ci ? k_stripA256<XCORR_HIP_CINT16, true> : k_stripA256<XCORR_HIP_FLOAT32, true>;
-      if (nblk_in > 0)
-        hipLaunchKernelGGL(kin, dim3(nblk_in), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_st, d_bs,
-                           0, (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s, c->amp2s, c->lsum,
-                           c->Z, c->Zu, (const float2*)c->tabs);
-      if (nblk > nblk_in)
-        hipLaunchKernelGGL(kedge, dim3(nblk - nblk_in), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave,
-                           d_st, d_bs, nblk_in, (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s,
-                           c->amp2s, c->lsum, c->Z, c->Zu, (const float2*)c->tabs);
-    });
-    if (rc) return rc;
-    rc = launch_timed(c, timing, "k_patch_sums", [&] {
-      hipLaunchKernelGGL(k_patch_sums, dim3((nb + 3) / 4), dim3(256), 0, c->stream, d, d_rb,
-                         (const long long*)c->lsum, c->sums, nb);
-    });
-    if (rc) return rc;
-    rc = launch_timed(c, timing, "k_colpass256", [&] {
-      hipLaunchKernelGGL(k_colpass256, dim3(nb * (d.NX / 32)), dim3(kThreads), 0, c->stream, d, (const float2*)c->Z,
-                         (const float2*)c->Zu, d_rb, (const unsigned long long*)c->sums, c->D,
-                         (const float2*)c->tabs);
-    });
-    if (rc) return rc;
-    rc = launch_timed(c, timing, "k_rowifft_surf256", [&] {
-      hipLaunchKernelGGL(k_rowifft_surf256, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)c->D, c->surf,
-                         c->peak, c->partial, (const float2*)c->tabs);
-    });
-    if (rc) return rc;
-    const size_t sh5 = d.interp_flag ? ((size_t)d.Lx * 16 + (size_t)d.Ly * 8) * sizeof(float2) : 16;
-    rc = launch_timed(c, timing, "k_finish", [&] {
-      hipLaunchKernelGGL(k_finish, dim3(nb), dim3(kThreads), sh5, c->stream, d, (const float*)c->amp1s,
-                         (const float*)c->amp2s, d_rb, 0LL, dev_out ? d_perm : (const int*)nullptr,
-                         (const unsigned long long*)c->sums, (const float*)c->surf, (const double*)nullptr,
-                         (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)nullptr,
-                         (const double*)nullptr, dev_out ? out : c->d_out, (const float2*)c->tabs);
-    });
+    c->last_perm.assign(h + c->max_batch, h + c->max_batch + sb.nb);
+    c->last_rb.assign(h, h + sb.nb);
+    const int rc = launch_strip_batch(c, c->dp, c->d_master, c->d_slave, c->stream, c->d_plan, sb, out, dev_out,
+                                      timing);
     if (rc) return rc;
     if (!dev_out) {
-      c->hres.resize(nb);
-      HIPCK(c, hipMemcpyAsync(c->hres.data(), c->d_out, nb * sizeof(xcorr_hip_result), hipMemcpyDeviceToHost,
+      c->hres.resize(sb.nb);
+      HIPCK(c, hipMemcpyAsync(c->hres.data(), c->d_out, sb.nb * sizeof(xcorr_hip_result), hipMemcpyDeviceToHost,
                               c->stream));
       HIPCK(c, hipStreamSynchronize(c->stream));
-      for (int i = 0; i < nb; i++) out[c->last_perm[i]] = c->hres[i];
+      for (int i = 0; i < sb.nb; i++) out[c->last_perm[i]] = c->hres[i];
     }
   }
   return 0;
@@ -2077,7 +2128,9 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
   if (rc0) return rc0;
   const size_t nel1 = (size_t)d.G1 * d.NX;
   const size_t sh1 = nel1 * sizeof(float2) * (d.ri > 2 ? 3 : 2) + nel1 * sizeof(float);
-  const bool row256 = d.NX == 256, col256 = d.NY == 256;
+  // the register-resident 256-point kernels run the default radix-16 x 16 plan
+  const bool row256 = d.NX == 256 && same_plan(d.plx, plan_for(256));
+  const bool col256 = d.NY == 256 && same_plan(d.ply, plan_for(256));
   const bool freq = d.corr_flag == XCORR_HIP_FREQ;
   const bool fusedA = row256 && d.ri == 2 && freq;
   int rc;
@@ -2093,7 +2146,7 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
     });
   else
     rc = launch_timed(c, timing, "k_range_ampl", [&] {
-      hipLaunchKernelGGL(k_range_ampl, dim3(nb * 2 * (d.NY / d.G1)), dim3(kThreads), sh1, c->stream, d, c->d_master,
+      hipLaunchKernelGGL(k_range_ampl, dim3(nb * 2 * (d.NY / d.G1)), dim3(c->nt_rows), sh1, c->stream, d, c->d_master,
                          c->d_slave, d_win, c->amp, c->sums, (const float2*)c->tabs);
     });
   if (rc) return rc;
@@ -2107,7 +2160,7 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
       });
     else
       rc = launch_timed(c, timing, "k_rowfft", [&] {
-        hipLaunchKernelGGL(k_rowfft, dim3(nb * (d.NY / d.G1)), dim3(kThreads), nel1 * sizeof(float2) * 2, c->stream,
+        hipLaunchKernelGGL(k_rowfft, dim3(nb * (d.NY / d.G1)), dim3(c->nt_rows), nel1 * sizeof(float2) * 2, c->stream,
                            d, (const float*)c->amp, (const unsigned long long*)c->sums, c->Z, (const float2*)c->tabs);
       });
     if (rc) return rc;
@@ -2120,12 +2173,12 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
       });
     } else {
       rc = launch_timed(c, timing, "k_colfft", [&] {
-        hipLaunchKernelGGL(k_colfft, dim3(nb * (d.NX / d.C3)), dim3(kThreads), nel3 * sizeof(float2) * 2,
+        hipLaunchKernelGGL(k_colfft, dim3(nb * (d.NX / d.C3)), dim3(c->nt_rows), nel3 * sizeof(float2) * 2,
                            c->stream, d, c->Z, (const unsigned long long*)c->sums, (const float2*)c->tabs);
       });
       if (rc) return rc;
       rc = launch_timed(c, timing, "k_product_colifft", [&] {
-        hipLaunchKernelGGL(k_product_colifft, dim3(nb * (d.NX / d.C3)), dim3(kThreads), nel3 * sizeof(float2) * 3,
+        hipLaunchKernelGGL(k_product_colifft, dim3(nb * (d.NX / d.C3)), dim3(c->nt_rows), nel3 * sizeof(float2) * 3,
                            c->stream, d, (const float2*)c->Z, c->D, (const float2*)c->tabs);
       });
     }
@@ -2332,6 +2385,480 @@ extern "C" int xcorr_hip_correlate_stream(xcorr_hip_ctx* c, xcorr_hip_line_reade
   // dumps refer to indices of this call
   for (int& q : c->last_perm) q = ord[q + (int)bands.back().begin];
   return 0;
+}
+
+// --------------------------------------------------------------------------
+// Tuning.  Batch size, LDS tiles (rows / columns per block) and threads per
+// block of the generic LDS kernels never change a result bit: every sum they
+// form is exact or keeps its order.  A radix plan other than the default is a
+// different FP32 rounding of the transforms of that length.
+// --------------------------------------------------------------------------
+static size_t lds_rows(const DevParams& d, int G) {     // k_range_ampl (the larger of the two row kernels)
+  const size_t nel = (size_t)G * d.NX;
+  return std::max(nel * sizeof(float2) * (d.ri > 2 ? 3 : 2) + nel * sizeof(float), nel * sizeof(float2) * 2);
+}
+static size_t lds_cols(const DevParams& d, int C) { return (size_t)C * d.NY * sizeof(float2) * 3; }
+constexpr size_t kMaxDynLds = 65536;
+
+static void plan_to_ints(const FftPlan& p, int32_t* R) {
+  for (int i = 0; i < 4; i++) R[i] = i < p.np ? p.R[i] : 0;
+}
+static FftPlan ints_to_plan(const int32_t* R) {
+  FftPlan p{0, {0, 0, 0, 0}};
+  while (p.np < 4 && R[p.np] != 0) {
+    p.R[p.np] = R[p.np];
+    p.np++;
+  }
+  return p;
+}
+
+extern "C" int xcorr_hip_get_tuning(xcorr_hip_ctx* c, xcorr_hip_tuning* t) {
+  if (!c || !t) return XCORR_HIP_EINVAL;
+  memset(t, 0, sizeof(*t));
+  t->batch = c->tune_batch > 0 ? std::min(c->tune_batch, c->max_batch) : c->max_batch;
+  t->rows_per_block = c->dp.G1;
+  t->cols_per_block = c->dp.C3;
+  t->threads = c->nt_rows;
+  plan_to_ints(c->dp.plx, t->radix_x);
+  plan_to_ints(c->dp.ply, t->radix_y);
+  return 0;
+}
+
+extern "C" int xcorr_hip_set_tuning(xcorr_hip_ctx* c, const xcorr_hip_tuning* t) {
+  if (!c || !t) return XCORR_HIP_EINVAL;
+  DevParams& d = c->dp;
+  if (t->batch < 1 || t->batch > c->max_batch) return fail(c, XCORR_HIP_EINVAL, "set_tuning: batch out of range");
+  if (t->threads < 64 || t->threads > kMaxThreads || t->threads % 64)
+    return fail(c, XCORR_HIP_EINVAL, "set_tuning: threads must be a multiple of 64 in [64, 512]");
+  const int G = t->rows_per_block, C = t->cols_per_block;
+  if (!is_pow2(G) || G > d.NY || lds_rows(d, G) > kMaxDynLds)
+    return fail(c, XCORR_HIP_EINVAL, "set_tuning: rows_per_block must be a power of two <= npy within 64 KB of LDS");
+  if (!is_pow2(C) || C > d.NX || lds_cols(d, C) > kMaxDynLds)
+    return fail(c, XCORR_HIP_EINVAL, "set_tuning: cols_per_block must be a power of two <= npx within 64 KB of LDS");
+  const FftPlan px = ints_to_plan(t->radix_x), py = ints_to_plan(t->radix_y);
+  if (!plan_valid(px, d.NX) || !plan_valid(py, d.NY))
+    return fail(c, XCORR_HIP_EINVAL, "set_tuning: a radix plan (2/4/8/16, up to 4 passes) must multiply to its length");
+  if (d.NX == d.NY && !same_plan(px, py))
+    return fail(c, XCORR_HIP_EINVAL, "set_tuning: transforms of equal length share one radix plan");
+  if (c->strips && (!same_plan(px, plan_for(d.NX)) || !same_plan(py, plan_for(d.NY))))
+    return fail(c, XCORR_HIP_EINVAL, "set_tuning: the 256-point strip path runs the default radix-16 plan only");
+  c->tune_batch = t->batch;
+  c->nt_rows = t->threads;
+  d.G1 = G;
+  d.C3 = C;
+  d.plx = px;
+  d.ply = py;
+  // the one-block-per-surface row path (npx = 256) runs only with the default plan
+  d.groups4 = (d.NX == 256 && same_plan(px, plan_for(256))) ? 1 : d.nyc / d.G4;
+  return 0;
+}
+
+// Radix plans for length n: non-increasing and non-decreasing radix sequences
+// of 2/4/8/16, at most one radix 2, at most 4 passes.
+static void plan_candidates(int n, std::vector<FftPlan>& out) {
+  out.clear();
+  const int rad[4] = {16, 8, 4, 2};
+  std::vector<FftPlan> all;
+  FftPlan cur{0, {0, 0, 0, 0}};
+  std::function<void(int, int, int)> rec = [&](int rem, int maxi, int twos) {
+    if (rem == 1) {
+      if (cur.np > 0) all.push_back(cur);
+      return;
+    }
+    if (cur.np == 4) return;
+    for (int i = maxi; i < 4; i++) {
+      const int r = rad[i];
+      if (rem % r) continue;
+      if (r == 2 && twos) continue;
+      cur.R[cur.np++] = r;
+      rec(rem / r, i, twos + (r == 2));
+      cur.np--;
+    }
+  };
+  rec(n, 0, 0);
+  for (const FftPlan& p : all) {
+    out.push_back(p);
+    FftPlan q = p;
+    std::reverse(q.R, q.R + q.np);
+    if (!same_plan(p, q)) out.push_back(q);
+  }
+}
+
+extern "C" int xcorr_hip_autotune(xcorr_hip_ctx* c, const xcorr_hip_window* win, int n, unsigned flags,
+                                  xcorr_hip_tuning* out) {
+  if (!c) return XCORR_HIP_EINVAL;
+  if (!win || n <= 0) return fail(c, XCORR_HIP_EINVAL, "autotune: no sample windows");
+  if (!c->images) return fail(c, XCORR_HIP_ESTATE, "autotune: no images set");
+  HIPCK(c, hipSetDevice(c->device));
+  const DevParams& d = c->dp;
+  n = std::min(n, 2 * c->max_batch);
+  if (c->d_res_n < (size_t)n) {
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if (c->d_res) (void)hipFree(c->d_res);
+    c->d_res = nullptr;
+    c->d_res_n = 0;
+    if (hipMalloc((void**)&c->d_res, (size_t)n * sizeof(xcorr_hip_result)) != hipSuccess)
+      return fail(c, XCORR_HIP_ENOMEM, "autotune: result buffer allocation failed");
+    c->d_res_n = n;
+  }
+  hipEvent_t e0, e1;
+  HIPCK(c, hipEventCreate(&e0));
+  HIPCK(c, hipEventCreate(&e1));
+  int timed = 0;
+  // time of one pass over the sample: best of three after one untimed run
+  auto measure = [&](const xcorr_hip_tuning& t, float* ms) -> int {
+    int rc = xcorr_hip_set_tuning(c, &t);
+    if (rc) return rc;
+    float best = 1e30f;
+    for (int r = 0; r < 4; r++) {
+      HIPCK(c, hipEventRecord(e0, c->stream));
+      rc = xcorr_hip_correlate(c, win, n, c->d_res, XCORR_HIP_DEVICE_PTRS | XCORR_HIP_HOST_WINDOWS | XCORR_HIP_NO_SYNC);
+      if (rc) return rc;
+      HIPCK(c, hipEventRecord(e1, c->stream));
+      HIPCK(c, hipEventSynchronize(e1));
+      float ms1 = 0.0f;
+      HIPCK(c, hipEventElapsedTime(&ms1, e0, e1));
+      if (r > 0) best = std::min(best, ms1);
+    }
+    timed++;
+    *ms = best;
+    return 0;
+  };
+  xcorr_hip_tuning best;
+  xcorr_hip_get_tuning(c, &best);
+  float best_ms = 0.0f;
+  int rc = measure(best, &best_ms);
+  auto consider = [&](const xcorr_hip_tuning& t) -> int {
+    float ms = 0.0f;
+    const int r = measure(t, &ms);
+    if (r == XCORR_HIP_EINVAL) return 0;      // not a valid setting for this shape
+    if (r) return r;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+    return 0;
+  };
+  // coordinate descent: threads, row tile, column tile, radix plans, batch
+  const bool generic = !c->strips;
+  if (!rc && generic)
+    for (int th : {128, 256, 512}) {
+      xcorr_hip_tuning t = best;
+      t.threads = th;
+      if (th != best.threads && (rc = consider(t))) break;
+    }
+  if (!rc && generic)
+    for (int G = 1; G <= d.NY && !rc; G *= 2) {
+      xcorr_hip_tuning t = best;
+      t.rows_per_block = G;
+      const size_t nel = (size_t)G * d.NX;
+      if (G != best.rows_per_block && nel >= 256 && lds_rows(d, G) <= kMaxDynLds) rc = consider(t);
+    }
+  if (!rc && generic)
+    for (int C = 1; C <= d.NX && !rc; C *= 2) {
+      xcorr_hip_tuning t = best;
+      t.cols_per_block = C;
+      const size_t nel = (size_t)C * d.NY;
+      if (C != best.cols_per_block && nel >= 256 && lds_cols(d, C) <= kMaxDynLds) rc = consider(t);
+    }
+  if (!rc && generic && d.corr_flag == XCORR_HIP_FREQ && (flags & XCORR_HIP_TUNE_RADIX)) {
+    std::vector<FftPlan> cand;
+    plan_candidates(d.NX, cand);
+    for (const FftPlan& pl : cand) {
+      if (rc) break;
+      xcorr_hip_tuning t = best;
+      plan_to_ints(pl, t.radix_x);
+      if (d.NX == d.NY) plan_to_ints(pl, t.radix_y);
+      if (!same_plan(pl, ints_to_plan(best.radix_x))) rc = consider(t);
+    }
+    if (d.NX != d.NY) {
+      plan_candidates(d.NY, cand);
+      for (const FftPlan& pl : cand) {
+        if (rc) break;
+        xcorr_hip_tuning t = best;
+        plan_to_ints(pl, t.radix_y);
+        if (!same_plan(pl, ints_to_plan(best.radix_y))) rc = consider(t);
+      }
+    }
+  }
+  if (!rc)
+    for (int b = c->max_batch / 2; b >= 512 && !rc; b /= 2) {
+      xcorr_hip_tuning t = best;
+      t.batch = b;
+      rc = consider(t);
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  rc = xcorr_hip_set_tuning(c, &best);
+  if (rc) return rc;
+  best.ms_per_batch = best_ms * (float)std::min(best.batch, n) / (float)n;
+  best.candidates = timed;
+  if (out) *out = best;
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// Multi-pair workloads: many SLC pairs with their own patch shapes, captured
+// once into one HIP graph.  Pairs with equal parameters form a class that
+// shares one context (scratch, tables, tuning); each class is one branch of
+// the graph, so the branches of different shapes run concurrently.  Every
+// plan (strip plans, device windows) is built and uploaded at create, so a
+// run is a single graph launch with no host work.
+// --------------------------------------------------------------------------
+struct MultiBatch {
+  int pair;                               // pair index
+  int off;                                // strips: plan offset (ints); generic: index in the class's device windows
+  int first;                              // generic: index of the batch's first window within its pair
+  int nb;                                 // windows in this batch
+  StripBatch sb;                          // strips only
+};
+
+struct xcorr_hip_multi {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<xcorr_hip_pair> pairs;
+  std::vector<int> pair_class;
+  std::vector<xcorr_hip_ctx*> ctx;        // per class
+  std::vector<xcorr_hip_tuning> tuning;   // per class
+  std::vector<void*> dbuf;                // per class: strip plans or windows
+  std::vector<std::vector<MultiBatch>> batches;
+  std::vector<hipEvent_t> events;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string err;
+};
+
+static std::string g_multi_err;
+
+static int mfail(xcorr_hip_multi* m, int code, const std::string& msg) {
+  if (m) m->err = msg;
+  else g_multi_err = msg;
+  return code;
+}
+
+static void multi_free(xcorr_hip_multi* m) {
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  if (m->exec) (void)hipGraphExecDestroy(m->exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  for (hipEvent_t e : m->events) (void)hipEventDestroy(e);
+  for (void* q : m->dbuf)
+    if (q) (void)hipFree(q);
+  for (xcorr_hip_ctx* c : m->ctx) xcorr_hip_destroy(c);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+// Point a context at a pair's device images.
+static void bind_pair(xcorr_hip_ctx* c, const xcorr_hip_pair& q) {
+  c->d_master = q.master;
+  c->d_slave = q.slave;
+  c->dp.m_nx = q.m_nx; c->dp.m_ny = q.m_ny; c->dp.s_nx = q.s_nx; c->dp.s_ny = q.s_ny;
+  c->dp.m_lo = 0; c->dp.m_hi = q.m_ny; c->dp.s_lo = 0; c->dp.s_hi = q.s_ny;
+  c->images = true;
+}
+
+#define MHIPCK(m, call)                                                                     \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return mfail(m, XCORR_HIP_EDEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+static int multi_build(xcorr_hip_multi* m, unsigned flags) {
+  const int np = (int)m->pairs.size();
+  // classes of equal parameters
+  std::vector<xcorr_hip_params> cls;
+  for (int i = 0; i < np; i++) {
+    int k = -1;
+    for (size_t j = 0; j < cls.size(); j++)
+      if (!memcmp(&cls[j], &m->pairs[i].params, sizeof(xcorr_hip_params))) k = (int)j;
+    if (k < 0) {
+      k = (int)cls.size();
+      cls.push_back(m->pairs[i].params);
+    }
+    m->pair_class.push_back(k);
+  }
+  const int nc = (int)cls.size();
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
+  const size_t budget = std::min<size_t>((size_t)12 << 30, free_b / (2 * (size_t)std::max(nc, 1)));
+  m->ctx.assign(nc, nullptr);
+  m->tuning.assign(nc, xcorr_hip_tuning{});
+  m->dbuf.assign(nc, nullptr);
+  m->batches.assign(nc, {});
+  for (int k = 0; k < nc; k++) {
+    const xcorr_hip_params& p = cls[k];
+    int biggest = -1, nmax = 0;
+    for (int i = 0; i < np; i++)
+      if (m->pair_class[i] == k && m->pairs[i].n > nmax) {
+        nmax = m->pairs[i].n;
+        biggest = i;
+      }
+    const size_t N = (size_t)p.npx * p.npy;
+    const size_t per = N * 4 * (2 + 4) + (size_t)p.nyc * p.npx * 8 + (size_t)p.nxc * p.nyc * 8 + 256;
+    int mb = (int)std::max<size_t>(256, std::min<size_t>(8192, budget / per));
+    mb = std::max(64, std::min(mb, std::max(nmax, 64)));
+    int rc = xcorr_hip_create(&p, mb, m->device, &m->ctx[k]);
+    if (rc) return mfail(m, rc, std::string("multi: context for a pair class: ") + xcorr_hip_last_error(nullptr));
+    xcorr_hip_ctx* c = m->ctx[k];
+    if ((flags & XCORR_HIP_TUNE) && biggest >= 0) {
+      bind_pair(c, m->pairs[biggest]);
+      rc = xcorr_hip_autotune(c, m->pairs[biggest].win, m->pairs[biggest].n, flags, &m->tuning[k]);
+      if (rc) return mfail(m, rc, std::string("multi: autotune: ") + c->err);
+    } else {
+      xcorr_hip_get_tuning(c, &m->tuning[k]);
+    }
+    const int cap = m->tuning[k].batch;
+    // plans / windows of every pair of the class, uploaded once
+    std::vector<int> host;
+    std::vector<xcorr_hip_window> hw;
+    for (int i = 0; i < np; i++) {
+      if (m->pair_class[i] != k || m->pairs[i].n == 0) continue;
+      const xcorr_hip_pair& q = m->pairs[i];
+      if (c->strips) {
+        bind_pair(c, q);
+        std::vector<int> ord;
+        strip_order(q.win, q.n, ord);
+        const size_t pi = plan_ints(c);
+        for (int pos = 0; pos < q.n;) {
+          MultiBatch mbt{};
+          mbt.pair = i;
+          mbt.off = (int)host.size();
+          host.resize(host.size() + pi, 0);
+          if (!plan_strip_batch(c, c->dp, q.win, ord, &pos, q.n, cap, host.data() + mbt.off, &mbt.sb))
+            return mfail(m, XCORR_HIP_EINVAL, "multi: a window does not fit the line capacity");
+          mbt.nb = mbt.sb.nb;
+          m->batches[k].push_back(mbt);
+        }
+      } else {
+        const int base = (int)hw.size();
+        hw.insert(hw.end(), q.win, q.win + q.n);
+        for (int off = 0; off < q.n; off += cap) {
+          MultiBatch mbt{};
+          mbt.pair = i;
+          mbt.off = base + off;
+          mbt.first = off;
+          mbt.nb = std::min(cap, q.n - off);
+          m->batches[k].push_back(mbt);
+        }
+      }
+    }
+    const size_t bytes = c->strips ? host.size() * sizeof(int) : hw.size() * sizeof(xcorr_hip_window);
+    if (bytes) {
+      if (hipMalloc(&m->dbuf[k], bytes) != hipSuccess) return mfail(m, XCORR_HIP_ENOMEM, "multi: plan allocation failed");
+      MHIPCK(m, hipMemcpy(m->dbuf[k], c->strips ? (const void*)host.data() : (const void*)hw.data(), bytes,
+                          hipMemcpyHostToDevice));
+    }
+  }
+  // capture: one branch per class, forked from and joined into m->stream
+  hipEvent_t fork;
+  MHIPCK(m, hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  m->events.push_back(fork);
+  for (int k = 0; k < nc; k++) {
+    hipEvent_t e;
+    MHIPCK(m, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    m->events.push_back(e);
+  }
+  MHIPCK(m, hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  hipError_t e = hipEventRecord(fork, m->stream);
+  for (int k = 0; k < nc && e == hipSuccess && !rc; k++) {
+    xcorr_hip_ctx* c = m->ctx[k];
+    e = hipStreamWaitEvent(c->stream, fork, 0);
+    for (const MultiBatch& b : m->batches[k]) {
+      if (e != hipSuccess || rc) break;
+      const xcorr_hip_pair& q = m->pairs[b.pair];
+      bind_pair(c, q);
+      if (c->strips)
+        rc = launch_strip_batch(c, c->dp, q.master, q.slave, c->stream, (const int*)m->dbuf[k] + b.off, b.sb, q.out,
+                                true, false);
+      else
+        rc = run_batch(c, (const xcorr_hip_window*)m->dbuf[k] + b.off, b.nb, q.out + b.first, false);
+    }
+    if (e == hipSuccess && !rc) e = hipEventRecord(m->events[1 + k], c->stream);
+    if (e == hipSuccess && !rc) e = hipStreamWaitEvent(m->stream, m->events[1 + k], 0);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(m->stream, &g);
+  m->graph = g;
+  if (rc) return mfail(m, rc, std::string("multi: capture: ") + m->ctx[0]->err);
+  if (e != hipSuccess) return mfail(m, XCORR_HIP_EDEVICE, std::string("multi: capture: ") + hipGetErrorString(e));
+  if (ee != hipSuccess) return mfail(m, XCORR_HIP_EDEVICE, std::string("multi: end capture: ") + hipGetErrorString(ee));
+  MHIPCK(m, hipGraphInstantiate(&m->exec, m->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+extern "C" int xcorr_hip_multi_create(const xcorr_hip_pair* pairs, int npairs, int device, unsigned flags,
+                                      xcorr_hip_multi** out) {
+  if (!out || npairs < 0 || (npairs > 0 && !pairs)) return mfail(nullptr, XCORR_HIP_EINVAL, "multi_create: bad arguments");
+  *out = nullptr;
+  for (int i = 0; i < npairs; i++) {
+    const xcorr_hip_pair& q = pairs[i];
+    std::string why;
+    if (validate(&q.params, &why)) return mfail(nullptr, XCORR_HIP_EINVAL, "multi_create: pair " + std::to_string(i) + ": " + why);
+    if (!q.master || !q.slave || q.m_nx <= 0 || q.m_ny <= 0 || q.s_nx <= 0 || q.s_ny <= 0 || q.n < 0 ||
+        (q.n > 0 && (!q.win || !q.out)))
+      return mfail(nullptr, XCORR_HIP_EINVAL, "multi_create: pair " + std::to_string(i) + ": bad images or windows");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return mfail(nullptr, XCORR_HIP_EDEVICE, "no HIP device");
+  if (device < 0 && hipGetDevice(&device) != hipSuccess) device = 0;
+  if (device >= ndev) return mfail(nullptr, XCORR_HIP_EINVAL, "device index out of range");
+  xcorr_hip_multi* m = new xcorr_hip_multi();
+  m->device = device;
+  m->pairs.assign(pairs, pairs + npairs);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+    multi_free(m);
+    return mfail(nullptr, XCORR_HIP_EDEVICE, "multi_create: stream creation failed");
+  }
+  const int rc = multi_build(m, flags);
+  if (rc) {
+    const std::string msg = m->err;
+    multi_free(m);
+    return mfail(nullptr, rc, msg);
+  }
+  *out = m;
+  return 0;
+}
+
+extern "C" int xcorr_hip_multi_run(xcorr_hip_multi* m, unsigned flags) {
+  if (!m) return XCORR_HIP_EINVAL;
+  MHIPCK(m, hipSetDevice(m->device));
+  MHIPCK(m, hipGraphLaunch(m->exec, m->stream));
+  if (!(flags & XCORR_HIP_NO_SYNC)) MHIPCK(m, hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+extern "C" int xcorr_hip_multi_sync(xcorr_hip_multi* m) {
+  if (!m) return XCORR_HIP_EINVAL;
+  MHIPCK(m, hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+extern "C" void* xcorr_hip_multi_stream(xcorr_hip_multi* m) { return m ? (void*)m->stream : nullptr; }
+
+extern "C" int xcorr_hip_multi_classes(xcorr_hip_multi* m, xcorr_hip_params* params, xcorr_hip_tuning* tuning,
+                                       int* pair_class, int cap) {
+  if (!m) return XCORR_HIP_EINVAL;
+  const int nc = (int)m->ctx.size();
+  for (int k = 0; k < nc && k < cap; k++) {
+    if (params) params[k] = m->ctx[k]->p;
+    if (tuning) tuning[k] = m->tuning[k];
+  }
+  if (pair_class)
+    for (size_t i = 0; i < m->pair_class.size(); i++) pair_class[i] = m->pair_class[i];
+  return nc;
+}
+
+extern "C" void xcorr_hip_multi_destroy(xcorr_hip_multi* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  multi_free(m);
+}
+
+extern "C" const char* xcorr_hip_multi_last_error(const xcorr_hip_multi* m) {
+  return m ? m->err.c_str() : g_multi_err.c_str();
 }
 
 extern "C" int xcorr_hip_kernel_times(xcorr_hip_ctx* c, const char** names, float* ms, int* launches, int cap) {

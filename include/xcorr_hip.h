@@ -15,6 +15,8 @@
  *                           search do_time_corr do_time_int_xcorr.cu:146-189 (-time/-time4)
  *   xcorr_hip_destroy       d_free_arrays                          xcorr.cu:122-158
  *   xcorr_hip_last_error    gErrCk / gfftErrCk abort messages      gpu_err.h:13-32
+ * Tuning and multi-pair entry points have no counterpart in the reference
+ * (each says so below).
  */
 #ifndef XCORR_HIP_H
 #define XCORR_HIP_H
@@ -133,6 +135,75 @@ typedef int (*xcorr_hip_line_reader)(void *user, int image, int first, int count
 int xcorr_hip_correlate_stream(xcorr_hip_ctx *ctx, xcorr_hip_line_reader reader, void *user,
                                int m_nx, int m_ny, int s_nx, int s_ny, int band_lines,
                                const xcorr_hip_window *win, int n, xcorr_hip_result *out);
+
+/* Launch settings of a context.  batch, rows_per_block, cols_per_block and
+ * threads never change a result bit (every sum they touch is exact or keeps
+ * its order).  A radix plan other than the default (npx/npy = 128: 16 x 8,
+ * 256: 16 x 16, 512: 8 x 8 x 8, ...) is a different FP32 rounding of the
+ * transforms of that length; the CPU restatement accepts the same plan.
+ * Radix plans apply to the generic LDS kernels; the 256 x 256 strip path
+ * (xsearch = ysearch = 64, ri 2, frequency) runs the default plan only.
+ * No counterpart in the reference (cuFFT picks its own plans, xcorr.cu:161-230
+ * sizes batches but is unused). */
+typedef struct xcorr_hip_tuning {
+  int32_t batch;            /* locations per launch, 1..max_batch */
+  int32_t rows_per_block;   /* generic row kernels: window rows per block (LDS tile), power of two */
+  int32_t cols_per_block;   /* generic column kernels: columns per block (LDS tile), power of two */
+  int32_t threads;          /* generic row / column kernels: threads per block, 64..512, multiple of 64 */
+  int32_t radix_x[4];       /* radix plan (2/4/8/16 per pass, 0-terminated) of the npx-point transforms */
+  int32_t radix_y[4];       /* the same for npy (equal lengths share one plan) */
+  float ms_per_batch;       /* autotune: measured time of one batch with these settings */
+  int32_t candidates;       /* autotune: settings timed */
+} xcorr_hip_tuning;
+
+#define XCORR_HIP_TUNE 16u        /* multi_create: autotune each pair class */
+#define XCORR_HIP_TUNE_RADIX 32u  /* autotune / multi_create: also try radix plans (changes FP32 rounding) */
+
+int xcorr_hip_get_tuning(xcorr_hip_ctx *ctx, xcorr_hip_tuning *t);
+int xcorr_hip_set_tuning(xcorr_hip_ctx *ctx, const xcorr_hip_tuning *t);
+
+/* Time candidate settings on up to 2*max_batch of the given host windows (the
+ * images must be set) by coordinate descent -- threads, row tile, column
+ * tile, radix plans (with XCORR_HIP_TUNE_RADIX), batch -- keep the fastest,
+ * apply it and report it in *out (may be NULL). */
+int xcorr_hip_autotune(xcorr_hip_ctx *ctx, const xcorr_hip_window *win, int n, unsigned flags,
+                       xcorr_hip_tuning *out);
+
+/* Multi-pair workloads: many SLC pairs, each with its own parameters (patch
+ * size, interpolation factor, time or frequency correlation), images and
+ * windows, run as ONE HIP graph launch.  Pairs with equal parameters form a
+ * class sharing one context; classes are concurrent branches of the graph.
+ * Everything a run needs (strip plans, device windows, tuned settings) is
+ * prepared at create, so a run does no host work.  Images and results are
+ * device memory the caller keeps alive; windows are host memory read at
+ * create.  No counterpart in the reference: it replaces running the
+ * per-row loop of do_correlation (xcorr.cu:299-378) once per pair. */
+typedef struct xcorr_hip_pair {
+  xcorr_hip_params params;
+  const void *master;       /* device memory, m_ny lines of m_nx samples */
+  int32_t m_nx, m_ny;
+  const void *slave;        /* device memory, s_ny lines of s_nx samples */
+  int32_t s_nx, s_ny;
+  const xcorr_hip_window *win;   /* host memory, n windows */
+  int32_t n;
+  xcorr_hip_result *out;    /* device memory, n results (out[i] belongs to win[i]) */
+} xcorr_hip_pair;
+
+typedef struct xcorr_hip_multi xcorr_hip_multi;
+
+/* flags: XCORR_HIP_TUNE, XCORR_HIP_TUNE_RADIX. */
+int xcorr_hip_multi_create(const xcorr_hip_pair *pairs, int npairs, int device, unsigned flags,
+                           xcorr_hip_multi **out);
+/* One graph launch over every pair; XCORR_HIP_NO_SYNC returns at once. */
+int xcorr_hip_multi_run(xcorr_hip_multi *m, unsigned flags);
+int xcorr_hip_multi_sync(xcorr_hip_multi *m);
+void *xcorr_hip_multi_stream(xcorr_hip_multi *m);
+/* Classes of the workload: params[k], tuning[k] (k < cap) and pair_class[i]
+ * for every pair (any pointer may be NULL).  Returns the number of classes. */
+int xcorr_hip_multi_classes(xcorr_hip_multi *m, xcorr_hip_params *params, xcorr_hip_tuning *tuning,
+                            int *pair_class, int cap);
+void xcorr_hip_multi_destroy(xcorr_hip_multi *m);
+const char *xcorr_hip_multi_last_error(const xcorr_hip_multi *m);
 
 /* Wait for all work queued on the context's stream. */
 int xcorr_hip_sync(xcorr_hip_ctx *ctx);

@@ -64,6 +64,7 @@ def lib():
         _lib.xo32_range_interp_row.argtypes = [P, ctypes.c_int, ctypes.c_int, P]
         _lib.xo64_range_interp_row.argtypes = [P, ctypes.c_int, ctypes.c_int, P]
         _lib.xo_twiddle64.argtypes = [ctypes.c_int, ctypes.c_long, P, P]
+        _lib.xo32_set_fft_plan.argtypes = [ctypes.c_int, ctypes.c_int, P]
         for f in ("xo32_patch", "xo64_patch"):
             getattr(_lib, f).argtypes = [ctypes.POINTER(Params), P, P, ctypes.POINTER(Result), P]
         for f in ("xo32_run", "xo64_run"):
@@ -88,6 +89,31 @@ def make_params(xsearch=64, ysearch=64, ri=2, interp_flag=1, interp_factor=16, c
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def set_fft_plan(n, radices=None):
+    """Radix plan of every n-point FP32 transform (None: the default plan)."""
+    r = np.zeros(4, np.int32)
+    k = 0 if not radices else len(radices)
+    r[:k] = radices or []
+    if lib().xo32_set_fft_plan(n, k, _ptr(r)) != 0:
+        raise ValueError("invalid radix plan %s for length %d" % (radices, n))
+
+
+class fft_plans:
+    """Context manager: radix plans {n: [radices]} for the FP32 restatement."""
+
+    def __init__(self, plans):
+        self.plans = {int(n): list(r) for n, r in (plans or {}).items() if r}
+
+    def __enter__(self):
+        for n, r in self.plans.items():
+            set_fft_plan(n, r)
+        return self
+
+    def __exit__(self, *a):
+        for n in self.plans:
+            set_fft_plan(n, None)
 
 
 def fft32(x, direction=-1):

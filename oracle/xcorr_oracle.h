@@ -87,6 +87,10 @@ typedef struct {
 /* --- spec building blocks (exported for unit tests) --- */
 void xo_twiddle64(int n, long m, double *c, double *s);       /* e^{+2 pi i m/n} */
 int  xo32_fft(xo_c32 *x, int n, int dir);                      /* in place; dir -1 fwd, +1 inv (unnormalised) */
+/* Radix plan of every n-point FP32 transform from now on (R[0..np-1] in
+ * {2, 4, 8, 16}, product n, np <= 4); np <= 0 restores the default plan.
+ * Follows the HIP engine's tuned plans (xcorr_hip_set_tuning). */
+int  xo32_set_fft_plan(int n, int np, const int *R);
 int  xo64_fft(xo_c64 *x, int n, int dir);
 int  xo32_range_interp_row(const xo_c32 *x, int n, int ri, float *amp);
 int  xo64_range_interp_row(const xo_c64 *x, int n, int ri, double *amp);

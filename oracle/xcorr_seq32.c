@@ -170,8 +170,42 @@ static inline void dftR(c32 *v, int R, int dir) {
   }
 }
 
-/* radix plan per length; the HIP kernels use the same table */
+/* Plans set with xo32_set_fft_plan (index log2 n); np 0 = none set. */
+static int g_plan_np[13];
+static int g_plan_R[13][4];
+
+static int ilog2(int n) {
+  int l = 0;
+  while ((1 << l) < n) l++;
+  return (1 << l) == n ? l : -1;
+}
+
+int xo32_set_fft_plan(int n, int np, const int *R) {
+  int l = ilog2(n);
+  if (l < 0 || l > 12) return -1;
+  if (np <= 0) {
+    g_plan_np[l] = 0;
+    return 0;
+  }
+  if (np > 4) return -1;
+  long prod = 1;
+  for (int i = 0; i < np; i++) {
+    if (R[i] != 2 && R[i] != 4 && R[i] != 8 && R[i] != 16) return -1;
+    prod *= R[i];
+  }
+  if (prod != n) return -1;
+  for (int i = 0; i < np; i++) g_plan_R[l][i] = R[i];
+  g_plan_np[l] = np;
+  return 0;
+}
+
+/* radix plan per length (the HIP kernels' default table), unless one was set */
 static int fft_plan(int n, int *R) {
+  int l = ilog2(n);
+  if (l >= 0 && l <= 12 && g_plan_np[l] > 0) {
+    for (int i = 0; i < g_plan_np[l]; i++) R[i] = g_plan_R[l][i];
+    return g_plan_np[l];
+  }
   switch (n) {
     case 1: return 0;
     case 2: R[0] = 2; return 1;
