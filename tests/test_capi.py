@@ -21,6 +21,15 @@ def test_library_exports_declared_symbols(xg):
     assert set(declared) == set(xg.EXPORTED_SYMBOLS)
 
 
+def test_library_exports_cnn_symbols(xg):
+    text = open(os.path.join(ROOT, "include", "xcorr_cnn.h")).read()
+    declared = sorted(set(re.findall(r"\b(xcorr_cnn_[a-z_0-9]+)\s*\(", text)))
+    assert len(declared) >= 14
+    lib = ctypes.CDLL(xg.LIB_PATH)
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert not missing, missing
+
+
 def test_abi_version(xg):
     assert xg.load_library().xcorr_hip_abi_version() == 1
 
