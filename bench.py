@@ -69,9 +69,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="config 1 (1024 x 2048, 1000 patches) instead")
-    ap.add_argument("--workload", default="strip", choices=("strip", "small", "batched"),
+    ap.add_argument("--workload", default="strip", choices=("strip", "small", "batched", "proposal"),
                     help="strip: 10800 x 27648 pair, 40,000 patches (default); small: 1024 x 2048, 1000 patches; "
-                         "batched: 64 uneven pairs with mixed patch sizes in one graph launch")
+                         "batched: 64 uneven pairs with mixed patch sizes in one graph launch; proposal: the "
+                         "learned-proposal form beside plain xcorr on the strip workload")
+    ap.add_argument("--tau", type=float, default=0.9, help="proposal: confidence threshold of the shrunken search")
+    ap.add_argument("--tau-skip", type=float, default=1.01, help="proposal: confidence above which high-res is skipped")
+    ap.add_argument("--xs-small", type=int, default=32, help="proposal: xsearch of the shrunken search")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "gmtsar-gpu_amd", "weights", "proposal_cnn.npy"))
     ap.add_argument("--pairs", type=int, default=64, help="batched: number of SLC pairs")
     ap.add_argument("--no-tune", action="store_true", help="batched: skip the autotuner")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -105,6 +110,8 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "batched":
         return bench_batched(args, xg, synth, torch, dev)
+    if args.workload == "proposal":
+        return bench_proposal(args, xg, synth, torch, dev)
 
     t0 = time.time()
     if args.small or args.workload == "small":
@@ -328,6 +335,84 @@ def bench_batched(args, xg, synth, torch, dev):
            "roofline": roofline, "cpu_baseline": cpu, "path_roofline": path, "accuracy": accuracy, "parity": parity}
     print(json.dumps(out), flush=True)
     mu.close()
+    return 0
+
+
+def bench_proposal(args, xg, synth, torch, dev):
+    """Learned-proposal form beside plain xcorr on the 10800 x 27648 strip
+    (40,000 windows of 256 x 256).  Both are timed end to end per step with
+    host results (plain: one correlate call; proposal: box-downsampling, CNN
+    proposals, the shrunken and fallback searches on concurrent streams, host
+    composition of the offsets)."""
+    import importlib
+    prop = importlib.import_module("gmtsar_gpu_amd.proposal")
+    params = np.load(args.weights, allow_pickle=False)
+    n_range, n_az, nxl, nyl = 10800, 27648, 100, 400
+    m, s = synth.strip_pair_torch(n_range, n_az, lambda x: 2.1 + 1e-4 * x, lambda y: -7.6 + 3e-4 * y, device="cuda")
+    torch.cuda.synchronize()
+    p = xg.make_params(64, 64)
+    x, y = xg.get_locations(n_range, n_az, nxl, nyl, 64, 64)
+    w = xg.make_windows(x, y, p.npx, p.npy)
+    n = len(w)
+    pr = prop.Proposer(params, tau=args.tau, tau_skip=args.tau_skip, max_batch=min(args.batch, n),
+                       xs_small=args.xs_small)
+    pr.set_images(m, s)
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            r = fn()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t1) / args.steps, r
+
+    t_plain, plain = timed(lambda: pr.full.correlate(w))
+    pdx, pdy = prop.compose(plain, w, 2)
+    t_prop, (dx, dy, res, path) = timed(lambda: pr.run(w))
+    t_pred, pred = timed(lambda: pr.predict(w))
+    ex, ey = dx - pdx, dy - pdy
+    summary = {}
+    for tau in (0.3, 0.5, 0.7, 0.9):
+        for skip in (False, True):
+            pr.tau, pr.tau_skip = tau, (tau if skip else 1.01)
+            dxx, dyy, _, pth = pr.run(w, pred=pred)
+            e = np.concatenate([dxx - pdx, dyy - pdy])
+            summary["tau%.1f%s" % (tau, "_skip" if skip else "")] = {
+                "shrunken": round(float((pth > 0).mean()), 4),
+                "rmse_px": round(float(np.sqrt(np.mean(e ** 2))), 4)}
+    pr.tau, pr.tau_skip = args.tau, args.tau_skip
+    conf = pred[2]
+    sel = path > 0
+    pe = np.abs(pred[0] - pdx), np.abs(pred[1] - pdy)
+    proposal = {
+        "plain_patches_per_s": round(n / t_plain, 2), "proposal_patches_per_s": round(n / t_prop, 2),
+        "speedup": round(t_plain / t_prop, 4), "cnn_ms": round(t_pred * 1e3, 3),
+        "tau": args.tau, "tau_skip": args.tau_skip,
+        "shrunken_fraction": round(float(sel.mean()), 4),
+        "skipped_highres_fraction": round(float((path == 2).mean()), 4),
+        "rmse_dx_px": round(float(np.sqrt(np.mean(ex ** 2))), 4),
+        "rmse_dy_px": round(float(np.sqrt(np.mean(ey ** 2))), 4),
+        "max_abs_err_px": round(float(max(np.abs(ex).max(), np.abs(ey).max())), 4),
+        "same_integer_offsets": round(float(np.mean((np.rint(2 * dx) == np.rint(2 * pdx)) &
+                                                    (np.rint(dy) == np.rint(pdy)))), 4),
+        "cnn_median_abs_err_px": [round(float(np.median(pe[0])), 3), round(float(np.median(pe[1])), 3)],
+        "mean_confidence": round(float(conf.mean()), 4), "tau_sweep": summary}
+    log("plain %.0f patches/s, proposal %.0f patches/s (x%.2f), shrunken %.1f %%, rmse %.4f / %.4f px"
+        % (n / t_plain, n / t_prop, t_plain / t_prop, 100 * sel.mean(), proposal["rmse_dx_px"],
+           proposal["rmse_dy_px"]))
+    out = {"metric": METRIC, "value": round(n / t_prop, 2), "unit": "patches/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(t_prop * 1e3, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 (CNN: bf16 MFMA, f32 accumulate)",
+           "data": "synthetic",
+           "config": {"workload": "learned proposal beside plain xcorr, 10800 x 27648 cint16 pair, -nx 100 -ny 400 "
+                                  "-xsearch 64 -ysearch 64, interp 16; shrunken search xsearch %d" % args.xs_small,
+                      "patches": n, "weights": os.path.relpath(args.weights, ROOT)},
+           "roofline": None, "cpu_baseline": None, "proposal": proposal}
+    print(json.dumps(out), flush=True)
+    pr.close()
     return 0
 
 

@@ -460,7 +460,8 @@ using namespace xn;
 
 struct xcorr_cnn {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // the stream work is queued on: own_stream or the caller's
+  hipStream_t own_stream = nullptr;
   int max_batch = 0;
   float* prm = nullptr;      // FP32 master parameters
   float* m1 = nullptr;       // Adam moments
@@ -495,7 +496,7 @@ static void cnn_free(xcorr_cnn* c) {
                   c->d1, c->d2, c->d3, c->d4, c->hout, c->corners};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
 }
 
@@ -510,7 +511,8 @@ extern "C" int xcorr_cnn_create(int device, int max_batch, xcorr_cnn** out) {
   c->device = device;
   c->max_batch = max_batch;
   hipError_t e = hipSetDevice(device);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  c->stream = c->own_stream;
   const size_t B = (size_t)max_batch;
   auto alloc = [&](void** p, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(p, bytes);
@@ -547,6 +549,12 @@ extern "C" void xcorr_cnn_destroy(xcorr_cnn* c) {
 
 extern "C" const char* xcorr_cnn_last_error(const xcorr_cnn* c) { return c ? c->err.c_str() : g_cnn_err.c_str(); }
 extern "C" void* xcorr_cnn_stream(xcorr_cnn* c) { return c ? (void*)c->stream : nullptr; }
+
+extern "C" int xcorr_cnn_set_stream(xcorr_cnn* c, void* stream) {
+  if (!c) return XCORR_HIP_EINVAL;
+  c->stream = (hipStream_t)stream;          // NULL: the null stream
+  return 0;
+}
 
 extern "C" int xcorr_cnn_sync(xcorr_cnn* c) {
   if (!c) return XCORR_HIP_EINVAL;

@@ -44,6 +44,11 @@ int xcorr_cnn_create(int device, int max_batch, xcorr_cnn **out);
 void xcorr_cnn_destroy(xcorr_cnn *net);
 const char *xcorr_cnn_last_error(const xcorr_cnn *net);
 void *xcorr_cnn_stream(xcorr_cnn *net);
+/* Queue all later work on `stream` (a hipStream_t; NULL is the null stream,
+ * the default stream of most callers), so that it is ordered with the caller's
+ * other work on that stream.  Until the first call the context's own stream
+ * is used. */
+int xcorr_cnn_set_stream(xcorr_cnn *net, void *stream);
 int xcorr_cnn_sync(xcorr_cnn *net);
 
 /* Host FP32 parameters in / out.  set_params also clears the optimiser state. */

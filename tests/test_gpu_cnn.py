@@ -100,6 +100,7 @@ def test_box4_and_crops_match_oracle(env, synth):
     corners = np.array([[10, 20, 12, 18], [-5, -8, -3, -10], [140, 200, 150, 190], [0, 0, 0, 0],
                         [100, 150, 101, 151]], np.int32)
     with prop.ProposalNet(max_batch=8) as net:
+        net.set_params(_params(co, 6))
         dm, ds = net.box4(mt), net.box4(st)
         cr = net.crops(dm, ds, torch.from_numpy(corners).cuda())
         o1 = net.propose(dm, ds, torch.from_numpy(corners).cuda())
@@ -111,7 +112,9 @@ def test_box4_and_crops_match_oracle(env, synth):
     refc = co.crops(dmh.astype(np.float64), ds.cpu().numpy().astype(np.float64), corners)
     got = (crh.view(np.uint16).astype(np.uint32) << 16).view(np.float32)
     np.testing.assert_allclose(got, co.bf16(refc), rtol=2 ** -7, atol=1e-3)
-    assert np.array_equal(o1.cpu().numpy().view(np.uint32), o2.cpu().numpy().view(np.uint32))
+    o1h = o1.cpu().numpy()
+    assert np.array_equal(o1h.view(np.uint32), o2.cpu().numpy().view(np.uint32))
+    assert np.all(o1h != 0.0)
 
 
 def test_adam_steps_reduce_loss(env):

@@ -1,0 +1,119 @@
+"""GPU: the learned-proposal form beside plain xcorr.
+
+* Every window's result is a plain xcorr run: the fallback windows equal the
+  plain engine's results, the shrunken-search windows equal the small engine's
+  results on the displaced windows, bit for bit, and the composed offsets add
+  the slave window's displacement as print_results.c:37-38 adds the PRM shifts.
+* The trained network (gmtsar-gpu_amd/weights) gains over the method without
+  it: its offset predictions are closer to plain xcorr than an untrained
+  network's, and at its own prior the shrunken search reproduces plain xcorr
+  on more windows than at an untrained network's prior.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, gpu_present
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(ROOT, "gmtsar-gpu_amd", "weights", "proposal_cnn.npy")
+
+
+@pytest.fixture(scope="module")
+def env(xg):
+    if not gpu_present():
+        pytest.skip("no GPU")
+    import importlib
+
+    import torch
+    prop = importlib.import_module("gmtsar_gpu_amd.proposal")
+    train = importlib.import_module("gmtsar_gpu_amd.train_proposal")
+    return torch, prop, train
+
+
+def _same(a, b):
+    for f in ("xoff", "yoff", "corr", "xfrac", "yfrac", "ip", "jp", "hip", "hjp"):
+        x, y = a[f], b[f]
+        assert np.array_equal(x.view(np.uint32) if x.dtype.kind == "f" else x,
+                              y.view(np.uint32) if y.dtype.kind == "f" else y), f
+
+
+def test_proposal_paths_are_plain_xcorr_runs(xg, synth, env):
+    torch, prop, _ = env
+    m, s = synth.constant_shift_pair(1024, 2048, 3.37, -12.81, 0.8)
+    mt, st = torch.from_numpy(m).cuda(), torch.from_numpy(s).cuda()
+    p = xg.make_params(64, 64)
+    x, y = xg.get_locations(1024, 2048, 20, 50, 64, 64)
+    w = xg.make_windows(x, y, p.npx, p.npy)
+    params = np.load(WEIGHTS, allow_pickle=False)
+    pr = prop.Proposer(params, tau=0.6, tau_skip=0.8, max_batch=256)
+    try:
+        pr.set_images(mt, st)
+        torch.cuda.synchronize()
+        pred = pr.predict(w)
+        dx, dy, res, path = pr.run(w, pred=pred)
+        print("paths:", np.bincount(path, minlength=3))
+        with xg.Engine(p, max_batch=256) as full:
+            full.set_images(m, s)
+            plain = full.correlate(w)
+        pdx, pdy = prop.compose(plain, w, 2)
+        k0 = path == 0
+        _same(res[k0], plain[k0])
+        rx, ry = np.rint(pred[0]).astype(np.int64), np.rint(pred[1]).astype(np.int64)
+        for k, interp in ((1, 1), (2, 0)):
+            sel = path == k
+            if not sel.any():
+                continue
+            ww = prop.shrunk_windows(w[sel], rx[sel], ry[sel], pr.xs_small)
+            with xg.Engine(xg.make_params(pr.xs_small, pr.xs_small, interp_flag=interp), max_batch=256) as e:
+                e.set_images(m, s)
+                ref = e.correlate(ww)
+            _same(res[sel], ref)
+            cdx, cdy = prop.compose(ref, ww, 2)
+            assert np.array_equal(dx[sel], cdx) and np.array_equal(dy[sel], cdy)
+            f32 = np.float32
+            want = ref["xoff"] / f32(2) - ref["xfrac"] / f32(2) + rx[sel].astype(f32)
+            assert np.array_equal(cdx, want)
+        assert np.array_equal(dx[k0], pdx[k0]) and np.array_equal(dy[k0], pdy[k0])
+        err = np.sqrt(np.mean(np.concatenate([dx - pdx, dy - pdy]) ** 2))
+        print("config-1 pair: shrunken %.1f %%, rmse vs plain %.4f px" % (100 * (path > 0).mean(), err))
+    finally:
+        pr.close()
+
+
+def test_trained_network_gains_over_untrained(xg, synth, env):
+    torch, prop, train = env
+    dev = torch.device("cuda:0")
+    nx, ny = 2048, 4096
+    m, s = synth.strip_pair_torch(nx, ny, lambda x: -18.0 + 30.0 * x / nx, lambda y: 14.0 - 26.0 * y / ny,
+                                  coh_range=(0.5, 0.95), seed=4242, device=dev)
+    torch.cuda.synchronize()
+    gx, gy = np.arange(0, nx - 255, 96), np.arange(0, ny - 255, 96)
+    yy, xx = np.meshgrid(gy, gx, indexing="ij")
+    w = np.zeros(xx.size, xg.WINDOW_DTYPE)
+    w["mx0"], w["my0"] = xx.ravel(), yy.ravel()
+    w["sx0"], w["sy0"] = w["mx0"], w["my0"]
+    with xg.Engine(xg.make_params(64, 64), max_batch=1024) as full:
+        full.set_images(m, s)
+        plain = full.correlate(w)
+    pdx, pdy = prop.compose(plain, w, 2)
+    good = plain["corr"] > 30
+    stats = {}
+    for name, params in (("trained", np.load(WEIGHTS, allow_pickle=False)), ("untrained", train.init_params(5))):
+        pr = prop.Proposer(params, max_batch=1024)
+        try:
+            pr.set_images(m, s)
+            qx, qy, conf = pr.predict(w)
+            err = np.hypot(qx - pdx, qy - pdy)
+            ww = prop.shrunk_windows(w, np.rint(qx), np.rint(qy), pr.xs_small)
+            r = pr.small.correlate(ww)
+            sx, sy = prop.compose(r, ww, 2)
+            agree = (np.abs(sx - pdx) <= 0.125) & (np.abs(sy - pdy) <= 0.125)
+            stats[name] = (float(np.median(err[good])), float(agree[good].mean()))
+        finally:
+            pr.close()
+    print("median |prediction - plain| px, shrunken-search agreement:", stats)
+    assert stats["trained"][0] < 0.6 * stats["untrained"][0]
+    assert stats["trained"][1] > stats["untrained"][1] + 0.1
