@@ -5,9 +5,10 @@
   results on the displaced windows, bit for bit, and the composed offsets add
   the slave window's displacement as print_results.c:37-38 adds the PRM shifts.
 * The trained network (gmtsar-gpu_amd/weights) gains over the method without
-  it: its offset predictions are closer to plain xcorr than an untrained
-  network's, and at its own prior the shrunken search reproduces plain xcorr
-  on more windows than at an untrained network's prior.
+  it: on a pair whose offsets (17..27 px in range, -16..-26 px in azimuth) lie
+  beyond the shrunken search's reach from a zero prior, its predictions are
+  closer to plain xcorr than an untrained network's, and at its prior the
+  shrunken search reproduces plain xcorr on far more windows.
 """
 import os
 
@@ -87,7 +88,8 @@ def test_trained_network_gains_over_untrained(xg, synth, env):
     torch, prop, train = env
     dev = torch.device("cuda:0")
     nx, ny = 2048, 4096
-    m, s = synth.strip_pair_torch(nx, ny, lambda x: -18.0 + 30.0 * x / nx, lambda y: 14.0 - 26.0 * y / ny,
+    # offsets beyond the shrunken search's reach from a zero prior (|dx| > 16 px)
+    m, s = synth.strip_pair_torch(nx, ny, lambda x: 17.0 + 10.0 * x / nx, lambda y: -16.0 - 10.0 * y / ny,
                                   coh_range=(0.5, 0.95), seed=4242, device=dev)
     torch.cuda.synchronize()
     gx, gy = np.arange(0, nx - 255, 96), np.arange(0, ny - 255, 96)
@@ -116,4 +118,4 @@ def test_trained_network_gains_over_untrained(xg, synth, env):
             pr.close()
     print("median |prediction - plain| px, shrunken-search agreement:", stats)
     assert stats["trained"][0] < 0.6 * stats["untrained"][0]
-    assert stats["trained"][1] > stats["untrained"][1] + 0.1
+    assert stats["trained"][1] > stats["untrained"][1] + 0.3
