@@ -14,6 +14,7 @@
 #include "../../include/xcorr_hip.h"
 #include "fft_spec.hpp"
 #include "fft256.hpp"
+#include "fftreg.hpp"
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -377,6 +378,139 @@ __global__ void __launch_bounds__(kThreads, 3) k_stageA256(DevParams P, const vo
     hermitian_upper256(v, j, tile, !in_y);        // unmasked rows: real input
 #pragma unroll
     for (int r = 0; r < 16; r++) zb[(size_t)row * 256 + j + 16 * r] = v[r];
+  }
+  s1 = block_sum_i64(s1, red);
+  s2 = block_sum_i64(s2, red);
+  if (threadIdx.x == 0) {
+    sums[2 * b] = (unsigned long long)s1;
+    sums[2 * b + 1] = (unsigned long long)s2;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Register-resident stage (a) + row transform for npx = npy = N in {128, 512},
+// ri = 2, frequency mode (the arithmetic of k_range_ampl + k_rowfft).
+// --------------------------------------------------------------------------
+// One line of stage (a) held by a group of G lanes (v[r] = line sample
+// x0 + j + G r): ea[q] / oa[q] are the amplitudes of output columns 2c and
+// 2c + 1, c = j + G q (q < V / 2): the original sample c + N / 4 and the
+// half-sample interpolant after it.
+template <int N>
+__device__ __forceinline__ void range_row(const void* __restrict__ src, int format, int inx, int iny, bool inside,
+                                          int gy, int x0, int j, float2* tile,
+                                          const float2 (&t)[RegFft<N>::TW], const float2 (&ph)[RegFft<N>::V],
+                                          float (&ea)[RegFft<N>::V / 2], float (&oa)[RegFft<N>::V / 2]) {
+  constexpr int G = RegFft<N>::G, V = RegFft<N>::V;
+  float2 v[V];
+  if (format == XCORR_HIP_CINT16) {
+    const unsigned* im = (const unsigned*)src;
+#pragma unroll
+    for (int r = 0; r < V; r++) {
+      const int gx = x0 + j + G * r;
+      unsigned u = 0u;
+      if (inside || (gy >= 0 && gy < iny && gx >= 0 && gx < inx)) u = im[(size_t)gy * inx + gx];
+      v[r] = make_float2((float)(short)(u & 0xffffu), (float)(short)(u >> 16));
+    }
+  } else {
+    const float* im = (const float*)src;
+#pragma unroll
+    for (int r = 0; r < V; r++) {
+      const int gx = x0 + j + G * r;
+      float u = 0.0f;
+      if (inside || (gy >= 0 && gy < iny && gx >= 0 && gx < inx)) u = im[(size_t)gy * inx + gx];
+      v[r] = make_float2(u, 0.0f);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < V / 2; q++) ea[q] = ampl(v[q + V / 4]);
+  RegFft<N>::template run<-1>(v, j, tile, t);
+#pragma unroll
+  for (int r = 0; r < V; r++) v[r] = cmul(v[r], ph[r]);
+  RegFft<N>::template run<+1>(v, j, tile, t);
+  const float sc = 1.0f / (float)N;
+#pragma unroll
+  for (int q = 0; q < V / 2; q++) {
+    float2 o = v[q + V / 4];
+    o.x = o.x * sc;
+    o.y = o.y * sc;
+    oa[q] = ampl(o);
+  }
+}
+
+// One block per patch, 256 / G groups, one line per group and iteration: both
+// images' range interpolation and amplitudes (kept for the NCC: the master's
+// everywhere, the slave's on the template), exact fixed-point sums, then
+// z = amp1 + i amp2 mask and its FFT along x (the means come off in the
+// spectrum, demean_row_bin), Hermitian completion of unmasked rows.
+template <int N>
+__global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __restrict__ master,
+                                                     const void* __restrict__ slave,
+                                                     const xcorr_hip_window* __restrict__ win,
+                                                     float* __restrict__ amp, unsigned long long* __restrict__ sums,
+                                                     float2* __restrict__ Z, const float2* __restrict__ tabs) {
+  using F = RegFft<N>;
+  constexpr int G = F::G, V = F::V, GPB = kThreads / G;
+  __shared__ float2 tiles[GPB * F::TILE];
+  __shared__ float2 mrow[GPB * (N / 2)];           // each group's master amplitude line (column pairs)
+  __shared__ long long red[kThreads / 64];
+  const int g = threadIdx.x / G, j = threadIdx.x % G;
+  const int b = blockIdx.x;
+  const xcorr_hip_window w = win[b];
+  float2* tile = tiles + g * F::TILE;
+  float2* mr = mrow + g * (N / 2);
+  float2 t[F::TW];
+  F::load_tw(tabs + P.tw_nx, j, t);
+  float2 ph[V];
+#pragma unroll
+  for (int r = 0; r < V; r++) ph[r] = tabs[P.phi + j + G * r];
+  const bool in_m = w.mx0 >= 0 && w.mx0 + N <= P.m_nx && w.my0 >= 0 && w.my0 + N <= P.m_ny;
+  const bool in_s = w.sx0 >= 0 && w.sx0 + N <= P.s_nx && w.sy0 >= 0 && w.sy0 + N <= P.s_ny;
+  float2* out1 = (float2*)(amp + (size_t)(2 * b) * P.N);
+  float2* out2 = (float2*)(amp + (size_t)(2 * b + 1) * P.N);
+  float2* zb = Z + (size_t)b * P.N;
+  const int c_lo = P.xs / 2, c_hi = (P.xs + P.nx_corr) / 2;   // template column pairs
+  long long s1 = 0, s2 = 0;
+  for (int row = g; row < N; row += GPB) {        // uniform trip count (N is a multiple of GPB)
+    {
+      float ea[V / 2], oa[V / 2];
+      range_row<N>(master, P.format, P.m_nx, P.m_ny, in_m, w.my0 + row, w.mx0, j, tile, t, ph, ea, oa);
+#pragma unroll
+      for (int q = 0; q < V / 2; q++) {
+        const float2 a = make_float2(ea[q], oa[q]);
+        out1[(size_t)row * (N / 2) + j + G * q] = a;
+        mr[j + G * q] = a;
+        s1 += fxp20(ea[q]) + fxp20(oa[q]);
+      }
+    }
+    {
+      float ea[V / 2], oa[V / 2];
+      range_row<N>(slave, P.format, P.s_nx, P.s_ny, in_s, w.sy0 + row, w.sx0, j, tile, t, ph, ea, oa);
+      const bool trow = row >= P.ys && row < P.ys + P.ny_corr;
+#pragma unroll
+      for (int q = 0; q < V / 2; q++) {
+        const float2 a = make_float2(ea[q], oa[q]);
+        const int c = j + G * q;
+        if (trow && c >= c_lo && c < c_hi) out2[(size_t)row * (N / 2) + c] = a;
+        tile[c] = a;                               // the tile doubles as the slave line buffer
+        s2 += fxp20(ea[q]) + fxp20(oa[q]);
+      }
+    }
+    __syncthreads();
+    const float* m1 = (const float*)mr;
+    const float* a2 = (const float*)tile;
+    const bool in_y = row >= P.ys && row < N - P.ys;
+    float2 v[V];
+#pragma unroll
+    for (int r = 0; r < V; r++) {
+      const int x = j + G * r;
+      const bool in = in_y && x >= P.xs && x < N - P.xs;
+      v[r] = make_float2(m1[x], in ? a2[x] : 0.0f);
+    }
+    __syncthreads();
+    F::template run<-1>(v, j, tile, t);
+    hermitian_upper<N>(v, j, tile, !in_y);         // unmasked rows: real input
+#pragma unroll
+    for (int r = 0; r < V; r++) zb[(size_t)row * N + j + G * r] = v[r];
   }
   s1 = block_sum_i64(s1, red);
   s2 = block_sum_i64(s2, red);
@@ -1537,6 +1671,7 @@ struct xcorr_hip_ctx {
   // tuning (xcorr_hip_set_tuning / xcorr_hip_autotune): none of these change a result bit
   int tune_batch = 0;                     // locations per launch (0: max_batch)
   int nt_rows = kThreads;                 // threads of the generic row / column kernels
+  bool reg_paths = true;                  // register-resident kernels where the shape has them
 };
 
 static std::string g_create_err;
@@ -2133,8 +2268,20 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
   const bool col256 = d.NY == 256 && same_plan(d.ply, plan_for(256));
   const bool freq = d.corr_flag == XCORR_HIP_FREQ;
   const bool fusedA = row256 && d.ri == 2 && freq;
+  // register-resident stage (a) + row transform for square 128 / 512 windows
+  const bool regA = freq && d.ri == 2 && d.NX == d.NY && (d.NX == 128 || d.NX == 512) &&
+                    same_plan(d.plx, plan_for(d.NX)) && same_plan(d.ply, plan_for(d.NY)) && c->reg_paths;
   int rc;
-  if (fusedA)
+  if (regA)
+    rc = launch_timed(c, timing, "k_stageA", [&] {
+      if (d.NX == 128)
+        hipLaunchKernelGGL(k_stageA<128>, dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_win,
+                           c->amp, c->sums, c->Z, (const float2*)c->tabs);
+      else
+        hipLaunchKernelGGL(k_stageA<512>, dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_win,
+                           c->amp, c->sums, c->Z, (const float2*)c->tabs);
+    });
+  else if (fusedA)
     rc = launch_timed(c, timing, "k_stageA256", [&] {
       hipLaunchKernelGGL(k_stageA256, dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_win,
                          c->amp, c->sums, c->Z, (const float2*)c->tabs);
@@ -2151,7 +2298,7 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
     });
   if (rc) return rc;
   if (freq) {
-    if (fusedA)
+    if (fusedA || regA)
       rc = 0;
     else if (row256)
       rc = launch_timed(c, timing, "k_rowfft256", [&] {
