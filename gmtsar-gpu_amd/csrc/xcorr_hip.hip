@@ -1016,6 +1016,176 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
   }
 }
 
+// Stage (b) columns for npx = npy = N in {128, 512}: FFT along y, C3 = C1
+// conj(C2) (-1)^(kx+ky), IFFT along y, keep lag rows N/4 .. 3N/4 - 1 (the
+// arithmetic of k_colfft + k_product_colifft).  Block p of a patch owns the TC
+// columns T = [TC p, TC p + TC) and their mirrors M = {N - kx} (N/2 in place of
+// the duplicate of column 0), each group CPG of each; only columns 0 .. N/2 of
+// C3 are inverse-transformed (C3 is Hermitian in this arithmetic).
+template <int N>
+struct ColCfg {
+  static constexpr int GPB = kThreads / RegFft<N>::G;
+  static constexpr int CPG = GPB >= 8 ? 1 : 8 / GPB;   // columns per group (512: 2)
+  static constexpr int TC = GPB * CPG;                 // T columns per block (128: 32, 512: 8)
+  static constexpr int PITCH = TC + 1;
+};
+
+template <int N>
+__global__ void __launch_bounds__(kThreads) k_colpass(DevParams P, const float2* __restrict__ Z,
+                                                      const unsigned long long* __restrict__ sums,
+                                                      float2* __restrict__ D, const float2* __restrict__ tabs) {
+  using F = RegFft<N>;
+  using C = ColCfg<N>;
+  constexpr int G = F::G, V = F::V, CPG = C::CPG, TC = C::TC, PITCH = C::PITCH;
+  __shared__ float2 T[N * PITCH];
+  __shared__ float2 ftiles[C::GPB * F::TILE];
+  __shared__ float2 sx[N];
+  const int tiles = N / (2 * TC);
+  const int p = blockIdx.x % tiles, b = blockIdx.x / tiles;
+  const int g = threadIdx.x / G, j = threadIdx.x % G;
+  float2* ftile = ftiles + g * F::TILE;
+  float2 t[F::TW];
+  F::load_tw(tabs + P.tw_ny, j, t);
+  const float m1NX = (float)patch_mean(sums[2 * b], P.N) * (float)N;
+  const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
+  const float2* zb = Z + (size_t)b * P.N;
+  auto kx_of = [&](int i, bool mirror) { return !mirror ? TC * p + i : ((p == 0 && i == 0) ? N / 2 : N - TC * p - i); };
+  float2 vT[CPG][V], vM[CPG][V];
+  for (int half = 0; half < 2; half++) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < N * TC; e += kThreads) {
+      const int y = e / TC, i = e - (e / TC) * TC;
+      const int kx = kx_of(i, half == 1);
+      T[y * PITCH + i] = demean_row_bin(zb[(size_t)y * N + kx], kx, y >= P.ys && y < N - P.ys, m1NX, m2f,
+                                        tabs[P.mk_x + kx]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CPG; c++)
+#pragma unroll
+      for (int r = 0; r < V; r++) {
+        const float2 a = T[(j + G * r) * PITCH + g * CPG + c];
+        if (half == 0) vT[c][r] = a;
+        else vM[c][r] = a;
+      }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < CPG; c++) {
+    F::template run<-1>(vT[c], j, ftile, t);
+    F::template run<-1>(vM[c], j, ftile, t);
+  }
+  const bool holder = p == 0 && g == 0;           // holds column 0 (T) and N/2 (M): self-mirrored
+  // M spectra in natural row order for the T products; column 0's own spectrum in sx
+#pragma unroll
+  for (int c = 0; c < CPG; c++)
+#pragma unroll
+    for (int r = 0; r < V; r++) T[(j + G * r) * PITCH + g * CPG + c] = vM[c][r];
+  if (holder)
+#pragma unroll
+    for (int r = 0; r < V; r++) sx[j + G * r] = vT[0][r];
+  __syncthreads();
+  float2 pT[CPG][V];
+#pragma unroll
+  for (int c = 0; c < CPG; c++) {
+    const int i = g * CPG + c, kx = TC * p + i;
+    const bool self = holder && c == 0;
+#pragma unroll
+    for (int r = 0; r < V; r++) {
+      const int ky = j + G * r, my = (N - ky) & (N - 1);
+      const float2 zm = self ? sx[my] : T[my * PITCH + i];
+      pT[c][r] = c3_bin(vT[c][r], zm, kx, ky);
+    }
+  }
+  if (p == 0) {                                    // the product of column N/2 from its own spectrum
+    __syncthreads();
+    if (holder)
+#pragma unroll
+      for (int r = 0; r < V; r++) sx[j + G * r] = vM[0][r];
+    __syncthreads();
+    if (holder)
+#pragma unroll
+      for (int r = 0; r < V; r++) {
+        const int ky = j + G * r;
+        vM[0][r] = c3_bin(vM[0][r], sx[(N - ky) & (N - 1)], N / 2, ky);
+      }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < CPG; c++) F::template run<+1>(pT[c], j, ftile, t);
+  if (p == 0) F::template run<+1>(vM[0], j, ftile, t);
+  // lag rows N/4 .. 3N/4 - 1 (r = V/4 .. 3V/4 - 1) through LDS, written as rows of D
+#pragma unroll
+  for (int c = 0; c < CPG; c++)
+#pragma unroll
+    for (int q = 0; q < V / 2; q++) T[(j + G * q) * PITCH + g * CPG + c] = pT[c][q + V / 4];
+  if (holder)
+#pragma unroll
+    for (int q = 0; q < V / 2; q++) sx[j + G * q] = vM[0][q + V / 4];
+  __syncthreads();
+  float2* db = D + (size_t)b * (N / 2) * N;
+  for (int e = threadIdx.x; e < (N / 2) * TC; e += kThreads) {
+    const int iy = e / TC, i = e - (e / TC) * TC;
+    db[(size_t)iy * N + TC * p + i] = T[iy * PITCH + i];
+  }
+  if (p == 0)
+    for (int e = threadIdx.x; e < N / 2; e += kThreads) db[(size_t)e * N + N / 2] = sx[e];
+}
+
+// Stage (b) row IFFT + surface + argmax for npx = npy = N in {128, 512}: one
+// block per patch (the arithmetic of k_rowifft_surf; the surface sum is one
+// partial per patch).
+template <int N>
+__global__ void __launch_bounds__(kThreads) k_rowifft_surf_reg(DevParams P, const float2* __restrict__ D,
+                                                               float* __restrict__ surf,
+                                                               unsigned long long* __restrict__ peak,
+                                                               double* __restrict__ partial,
+                                                               const float2* __restrict__ tabs) {
+  using F = RegFft<N>;
+  constexpr int G = F::G, V = F::V, GPB = kThreads / G, NYC = N / 2;
+  __shared__ float2 tiles[GPB * F::TILE];
+  __shared__ unsigned long long redk[kThreads / 64];
+  __shared__ double redd[kThreads / 64];
+  const int g = threadIdx.x / G, j = threadIdx.x % G;
+  const int b = blockIdx.x;
+  const float2* db = D + (size_t)b * NYC * N;
+  float* sb = surf + (size_t)b * NYC * NYC;
+  float2* tile = tiles + g * F::TILE;
+  float2 t[F::TW];
+  F::load_tw(tabs + P.tw_nx, j, t);
+  const float sc = 1.0f / (float)P.N;
+  unsigned long long best = 0ull;
+  double sum = 0.0;
+  for (int row0 = 0; row0 < NYC; row0 += GPB) {   // uniform trip count (NYC is a multiple of GPB)
+    const int row = row0 + g;
+    float2 v[V];
+#pragma unroll
+    for (int r = 0; r < V; r++) {
+      const int x = j + G * r;
+      v[r] = db[(size_t)row * N + (x <= N / 2 ? x : N - x)];
+      if (x > N / 2) v[r].y = -v[r].y;              // upper half: conjugate mirror
+    }
+    F::template run<+1>(v, j, tile, t);
+#pragma unroll
+    for (int q = 0; q < V / 2; q++) {
+      const float re = v[q + V / 4].x * sc, im = v[q + V / 4].y * sc;
+      const double dr = re, di = im;
+      const float cv = (float)sqrt(dr * dr + di * di);
+      const unsigned idx = (unsigned)(row * NYC + j + G * q);
+      sb[idx] = cv;
+      const unsigned long long k = peak_key(cv, idx);
+      best = k > best ? k : best;
+      sum += (double)cv;
+    }
+  }
+  best = block_max_u64(best, redk);
+  sum = block_sum_f64(sum, redd);
+  if (threadIdx.x == 0) {
+    peak[b] = best;
+    partial[b] = sum;
+  }
+}
+
 // --------------------------------------------------------------------------
 // stage (b): demean, mask, pack z = c1 + i c2, FFT along x
 // conv_ampl_demean.cu:111-139, xcorr.cu:385-401, do_freq_xcorr.cu:90-101
@@ -1754,6 +1924,23 @@ static void free_scratch(xcorr_hip_ctx* c) {
   c->lsum = nullptr; c->Zu = nullptr; c->need = nullptr; c->d_plan = nullptr;
 }
 
+// Frequency-mode square 128 / 512 windows with ri 2 and the default radix
+// plans run the register-resident kernels (k_stageA, k_colpass,
+// k_rowifft_surf_reg).
+static bool reg_shape(const xcorr_hip_ctx* c) {
+  const DevParams& d = c->dp;
+  return c->reg_paths && d.corr_flag == XCORR_HIP_FREQ && d.ri == 2 && d.NX == d.NY &&
+         (d.NX == 128 || d.NX == 512) && same_plan(d.plx, plan_for(d.NX)) && same_plan(d.ply, plan_for(d.NY));
+}
+// Surface partial sums per patch: one for the one-block-per-surface row
+// kernels (npx = 256 with the default plan, the register kernels), else one
+// per k_rowifft_surf block.
+static int groups4_for(const xcorr_hip_ctx* c) {
+  const DevParams& d = c->dp;
+  if ((d.NX == 256 && same_plan(d.plx, plan_for(256))) || reg_shape(c)) return 1;
+  return d.nyc / d.G4;
+}
+
 // Plan layout (ints): rb[max_batch] | perm[max_batch] | blk[2 max_lines / 64] | StripDesc[max_batch]
 static size_t plan_ints(const xcorr_hip_ctx* c) {
   return 2 * (size_t)c->max_batch + 2 * (size_t)c->max_lines / kStripBlockLines +
@@ -1795,8 +1982,8 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   d.ply = plan_for(d.NY);
   d.G1 = std::min(d.NY, std::max(1, kTile / d.NX));
   d.G4 = std::min(d.nyc, std::max(1, kTile / d.NX));
+  d.groups4 = groups4_for(c);                      // needs the plans and G4
   d.C3 = std::min(d.NX, std::max(1, kTile / d.NY));
-  d.groups4 = (d.NX == 256) ? 1 : d.nyc / d.G4;   // the 256-sample row path sums each surface in one block
   // time search: LR surface rows x nxc lags per block (<= 2 lags per thread),
   // template rows staged RC at a time within 64 KB of LDS
   d.tLR = std::min(d.nyc, std::max(1, kThreads / d.nxc));
@@ -2269,8 +2456,7 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
   const bool freq = d.corr_flag == XCORR_HIP_FREQ;
   const bool fusedA = row256 && d.ri == 2 && freq;
   // register-resident stage (a) + row transform for square 128 / 512 windows
-  const bool regA = freq && d.ri == 2 && d.NX == d.NY && (d.NX == 128 || d.NX == 512) &&
-                    same_plan(d.plx, plan_for(d.NX)) && same_plan(d.ply, plan_for(d.NY)) && c->reg_paths;
+  const bool regA = reg_shape(c);
   int rc;
   if (regA)
     rc = launch_timed(c, timing, "k_stageA", [&] {
@@ -2312,7 +2498,16 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
       });
     if (rc) return rc;
     const size_t nel3 = (size_t)d.C3 * d.NY;
-    if (col256 && d.NX >= 32) {
+    if (regA) {
+      rc = launch_timed(c, timing, "k_colpass", [&] {
+        if (d.NX == 128)
+          hipLaunchKernelGGL(k_colpass<128>, dim3(nb * (128 / (2 * ColCfg<128>::TC))), dim3(kThreads), 0, c->stream,
+                             d, (const float2*)c->Z, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs);
+        else
+          hipLaunchKernelGGL(k_colpass<512>, dim3(nb * (512 / (2 * ColCfg<512>::TC))), dim3(kThreads), 0, c->stream,
+                             d, (const float2*)c->Z, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs);
+      });
+    } else if (col256 && d.NX >= 32) {
       rc = launch_timed(c, timing, "k_colpass256", [&] {
         hipLaunchKernelGGL(k_colpass256, dim3(nb * (d.NX / 32)), dim3(kThreads), 0, c->stream, d,
                            (const float2*)c->Z, (const float2*)nullptr, (const int*)nullptr,
@@ -2331,7 +2526,16 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
     }
     if (rc) return rc;
     const size_t nel4 = (size_t)d.G4 * d.NX;
-    if (row256)
+    if (regA)
+      rc = launch_timed(c, timing, "k_rowifft_surf_reg", [&] {
+        if (d.NX == 128)
+          hipLaunchKernelGGL(k_rowifft_surf_reg<128>, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)c->D,
+                             c->surf, c->peak, c->partial, (const float2*)c->tabs);
+        else
+          hipLaunchKernelGGL(k_rowifft_surf_reg<512>, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)c->D,
+                             c->surf, c->peak, c->partial, (const float2*)c->tabs);
+      });
+    else if (row256)
       rc = launch_timed(c, timing, "k_rowifft_surf256", [&] {
         hipLaunchKernelGGL(k_rowifft_surf256, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)c->D,
                            c->surf, c->peak, c->partial, (const float2*)c->tabs);
@@ -2595,8 +2799,7 @@ extern "C" int xcorr_hip_set_tuning(xcorr_hip_ctx* c, const xcorr_hip_tuning* t)
   d.C3 = C;
   d.plx = px;
   d.ply = py;
-  // the one-block-per-surface row path (npx = 256) runs only with the default plan
-  d.groups4 = (d.NX == 256 && same_plan(px, plan_for(256))) ? 1 : d.nyc / d.G4;
+  d.groups4 = groups4_for(c);
   return 0;
 }
 
