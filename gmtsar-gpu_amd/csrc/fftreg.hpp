@@ -5,7 +5,8 @@
 // default radix plan of fft_spec.hpp (128: 16 x 8, 512: 8 x 8 x 8) -- same
 // butterflies, same twiddles, same operation order -- so results are
 // bit-identical to lds_fft_rows; only the data movement differs: one LDS
-// transpose per pass boundary in a padded per-group tile.
+// transpose per pass boundary in a padded per-group tile.  A group never
+// spans two waves, so the transposes synchronise the wave only.
 //
 //   N = 128: G = 8 lanes x 16 values; pass 1 radix 16 (one butterfly per
 //            lane), pass 2 radix 8 (two butterflies per lane: j and j + 8).
@@ -14,6 +15,15 @@
 #include "fft_spec.hpp"
 
 namespace xc {
+
+// Barrier for LDS data exchanged among the lanes of one wave (every group
+// here lies within one wave): completes the wave's LDS writes and keeps the
+// compiler from moving LDS accesses across it.  Other waves are not waited for.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <int N>
 struct RegFft;
@@ -37,14 +47,14 @@ struct RegFft<128> {
     Dft<16, DIR>::run(v);                        // y[16 j + r] = v[r]
 #pragma unroll
     for (int r = 0; r < 16; r++) tile[j * 17 + r] = v[r];
-    __syncthreads();
+    wave_sync();
     float2 u0[8], u1[8];
 #pragma unroll
     for (int r = 0; r < 8; r++) {                // y[j + 16 r], y[j + 8 + 16 r]
       u0[r] = tile[r * 17 + j];
       u1[r] = tile[r * 17 + j + 8];
     }
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int r = 1; r < 8; r++) {
       float2 w0 = t[r - 1], w1 = t[7 + r - 1];
@@ -84,10 +94,10 @@ struct RegFft<512> {
     Dft<8, DIR>::run(v);                         // y[8 j + r]
 #pragma unroll
     for (int r = 0; r < 8; r++) tile[j * 9 + r] = v[r];
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int r = 0; r < 8; r++) v[r] = tile[(hi + 8 * r) * 9 + k];   // y[j + 64 r]
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int r = 1; r < 8; r++) {
       float2 w = t[r - 1];
@@ -97,10 +107,10 @@ struct RegFft<512> {
     Dft<8, DIR>::run(v);                         // z[64 hi + 8 r + k]
 #pragma unroll
     for (int r = 0; r < 8; r++) tile[(8 * hi + r) * 9 + k] = v[r];
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int r = 0; r < 8; r++) v[r] = tile[(hi + 8 * r) * 9 + k];   // z[j + 64 r]
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int r = 1; r < 8; r++) {
       float2 w = t[7 + r - 1];
@@ -112,13 +122,13 @@ struct RegFft<512> {
 };
 
 // Hermitian completion of a row spectrum held as v[r] = X[j + G r]: when
-// `apply`, X[k] = conj(X[N - k]) for k > N / 2.  Block-wide barriers.
+// `apply`, X[k] = conj(X[N - k]) for k > N / 2.  `tile` is the group's.
 template <int N>
 __device__ __forceinline__ void hermitian_upper(float2 (&v)[RegFft<N>::V], int j, float2* tile, bool apply) {
   constexpr int G = RegFft<N>::G, V = RegFft<N>::V;
 #pragma unroll
   for (int r = 0; r < V; r++) tile[RegFft<N>::slot(j + G * r)] = v[r];
-  __syncthreads();
+  wave_sync();
   if (apply) {
 #pragma unroll
     for (int r = 0; r < V; r++) {
@@ -129,7 +139,7 @@ __device__ __forceinline__ void hermitian_upper(float2 (&v)[RegFft<N>::V], int j
       }
     }
   }
-  __syncthreads();
+  wave_sync();
 }
 
 }  // namespace xc

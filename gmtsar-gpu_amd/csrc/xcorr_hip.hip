@@ -470,7 +470,7 @@ __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __
   float2* zb = Z + (size_t)b * P.N;
   const int c_lo = P.xs / 2, c_hi = (P.xs + P.nx_corr) / 2;   // template column pairs
   long long s1 = 0, s2 = 0;
-  for (int row = g; row < N; row += GPB) {        // uniform trip count (N is a multiple of GPB)
+  for (int row = g; row < N; row += GPB) {        // groups never wait for other waves inside the loop
     {
       float ea[V / 2], oa[V / 2];
       range_row<N>(master, P.format, P.m_nx, P.m_ny, in_m, w.my0 + row, w.mx0, j, tile, t, ph, ea, oa);
@@ -495,7 +495,7 @@ __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __
         s2 += fxp20(ea[q]) + fxp20(oa[q]);
       }
     }
-    __syncthreads();
+    wave_sync();                                   // the group's line buffers are its own
     const float* m1 = (const float*)mr;
     const float* a2 = (const float*)tile;
     const bool in_y = row >= P.ys && row < N - P.ys;
@@ -506,7 +506,7 @@ __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __
       const bool in = in_y && x >= P.xs && x < N - P.xs;
       v[r] = make_float2(m1[x], in ? a2[x] : 0.0f);
     }
-    __syncthreads();
+    wave_sync();
     F::template run<-1>(v, j, tile, t);
     hermitian_upper<N>(v, j, tile, !in_y);         // unmasked rows: real input
 #pragma unroll

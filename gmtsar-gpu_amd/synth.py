@@ -122,3 +122,101 @@ def strip_pair_torch(n_range, n_azimuth, dx_fn, dy_fn, coh_range=(0.3, 0.95), bl
         return out
 
     return q(m), q(s)
+
+
+def dense_pair_torch(n_range, n_azimuth, field, coherence, block=128, margin=64, seed=11, device="cuda"):
+    """Pair with a spatially varying offset field and coherence, generated on
+    the GPU: each block x block tile is Fourier-shifted by field(xc, yc) ->
+    (dx, dy) at its centre (per-tile Fourier resampling with a `margin`-sample
+    apron) and mixed with noise at coherence(xc, yc).  field and coherence take
+    numpy arrays of tile-centre coordinates.  Returns (master, slave) as
+    (lines, samples, 2) int16 tensors on `device`."""
+    import math
+
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    ny, nx = n_azimuth, n_range
+
+    def randn(*shape):
+        return torch.randn(*shape, generator=g, device=device, dtype=torch.float32)
+
+    m = torch.complex(randn(ny, nx), randn(ny, nx)) / math.sqrt(2.0)
+    m = m * torch.exp(SIGMA_LN * randn(ny, nx) - SIGMA_LN ** 2 / 2.0)
+    power = float((m.real.square() + m.imag.square()).mean())
+    T = block + 2 * margin
+    nby, nbx = -(-ny // block), -(-nx // block)
+    mp = torch.zeros(nby * block + 2 * margin, nbx * block + 2 * margin, dtype=torch.complex64, device=device)
+    mp[margin:margin + ny, margin:margin + nx] = m
+    s = torch.empty(nby * block, nbx * block, dtype=torch.complex64, device=device)
+    kx = torch.fft.fftfreq(T, device=device)[None, None, :]
+    ky = torch.fft.fftfreq(T, device=device)[None, :, None]
+    xc = np.arange(nbx) * block + block / 2.0
+    rows = 16                                         # tile rows per FFT batch
+    for by0 in range(0, nby, rows):
+        nr = min(rows, nby - by0)
+        strip = mp[by0 * block: (by0 + nr) * block + 2 * margin]
+        tiles = strip.unfold(0, T, block).unfold(1, T, block)          # (nr, nbx, T, T)
+        yc = (by0 + np.arange(nr)) * block + block / 2.0
+        X, Y = np.meshgrid(xc, yc)
+        dx, dy = field(X, Y)
+        gam = np.clip(coherence(X, Y), 0.0, 1.0)
+        dxt = torch.as_tensor(dx, device=device, dtype=torch.float32).reshape(-1, 1, 1)
+        dyt = torch.as_tensor(dy, device=device, dtype=torch.float32).reshape(-1, 1, 1)
+        F = torch.fft.fft2(tiles.reshape(-1, T, T))
+        F = F * torch.exp(-2j * math.pi * (kx * dxt + ky * dyt)).to(torch.complex64)
+        sh = torch.fft.ifft2(F)[:, margin:margin + block, margin:margin + block]
+        gt = torch.as_tensor(gam, device=device, dtype=torch.float32).reshape(-1, 1, 1)
+        w = torch.complex(randn(*sh.shape), randn(*sh.shape)) * math.sqrt(power / 2.0)
+        sh = gt * sh + torch.sqrt(1.0 - gt * gt) * w
+        s[by0 * block:(by0 + nr) * block] = sh.reshape(nr, nbx, block, block).permute(0, 2, 1, 3).reshape(
+            nr * block, nbx * block)
+    s = s[:ny, :nx]
+    scale = STD_IQ / math.sqrt(power / 2.0)
+
+    def q(z):
+        out = torch.empty(ny, nx, 2, dtype=torch.int16, device=device)
+        out[..., 0] = torch.clamp(torch.round(z.real * scale), -32768, 32767).to(torch.int16)
+        out[..., 1] = torch.clamp(torch.round(z.imag * scale), -32768, 32767).to(torch.int16)
+        return out
+
+    return q(m), q(s)
+
+
+def sinusoid_field(rng, n=8, amp=15.0, wl=(133.0, 667.0)):
+    """Smooth random offset field: sum of n random 2-D sinusoids per component,
+    total amplitude <= amp px, wavelengths in wl px (2-10 km at 15 m pixels)."""
+    comps = []
+    for _ in range(2):
+        a = rng.uniform(0.2, 1.0, n)
+        a *= amp / a.sum()
+        lam = rng.uniform(wl[0], wl[1], n)
+        th = rng.uniform(0, 2 * np.pi, n)
+        ph = rng.uniform(0, 2 * np.pi, n)
+        comps.append((a, 2 * np.pi / lam * np.cos(th), 2 * np.pi / lam * np.sin(th), ph))
+
+    def f(x, y):
+        out = []
+        for a, kx, ky, ph in comps:
+            v = np.zeros(np.shape(x))
+            for i in range(n):
+                v = v + a[i] * np.sin(kx[i] * x + ky[i] * y + ph[i])
+            out.append(v)
+        return out[0], out[1]
+    return f
+
+
+def smooth_coherence(rng, lo=0.15, hi=0.95, n=6, wl=(1000.0, 6000.0)):
+    """Spatially correlated coherence in [lo, hi]: a sum of random sinusoids
+    mapped through a logistic."""
+    lam = rng.uniform(wl[0], wl[1], n)
+    th = rng.uniform(0, 2 * np.pi, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+
+    def f(x, y):
+        v = np.zeros(np.shape(x))
+        for i in range(n):
+            v = v + np.sin(2 * np.pi / lam[i] * (np.cos(th[i]) * x + np.sin(th[i]) * y) + ph[i])
+        return lo + (hi - lo) / (1.0 + np.exp(-1.5 * v))
+    return f
