@@ -99,6 +99,34 @@ __device__ __forceinline__ void conv_tile(const bf16_t* xb, const bf16_t* Wf, in
   }
 }
 
+// As conv_tile with the B fragments of all 9 taps already in registers
+// (bw[tap][n-tile]); the same MFMA sequence on the same operands.
+template <int HIN, int S>
+__device__ __forceinline__ void conv_tile_w(const bf16_t* xb, const bf16x8 (&bw)[9][2], int p0, int lane,
+                                            f32x4 (&acc)[2]) {
+  constexpr int HO = (HIN - 1) / S + 1;
+  const int m = lane & 15, g = lane >> 4;
+  const int p = p0 + m, oy = p / HO, ox = p - (p / HO) * HO;
+  acc[0] = f32x4{0.f, 0.f, 0.f, 0.f};
+  acc[1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int tap = 0; tap < 9; tap++) {
+    const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+    const int iy = oy * S + ky - 1, ix = ox * S + kx - 1;
+    const bf16x8 a = (iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) ? ld8(xb + (iy * HIN + ix) * kC + 8 * g) : zero8();
+    acc[0] = mfma(a, bw[tap][0], acc[0]);
+    acc[1] = mfma(a, bw[tap][1], acc[1]);
+  }
+}
+__device__ __forceinline__ void load_w(const bf16_t* Wf, int lane, bf16x8 (&bw)[9][2]) {
+  const int m = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int tap = 0; tap < 9; tap++) {
+    bw[tap][0] = ld8(Wf + (tap * kC + m) * kC + 8 * g);
+    bw[tap][1] = ld8(Wf + (tap * kC + 16 + m) * kC + 8 * g);
+  }
+}
+
 // bias + ReLU + bf16 store of a conv tile: y[(p0 + 4g + i) * 32 + co]
 __device__ __forceinline__ void store_tile(bf16_t* yb, const float* bias, int p0, int lane, const f32x4 (&acc)[2]) {
   const int m = lane & 15, g = lane >> 4;
@@ -427,16 +455,20 @@ __global__ void __launch_bounds__(256) k_infer(const bf16_t* __restrict__ x0g, c
     conv1_tile(R0, q + kQ1, t * 16, lane, acc);
     store_tile(R1, prm + kB1, t * 16, lane, acc);
   }
+  bf16x8 bw[9][2];                                          // a layer's B fragments, loaded once per wave
+  load_w(q + kQf2, lane, bw);
   __syncthreads();
   for (int t = wave; t < 16; t += 4) {                      // conv2 -> R0 (16 x 16 x 32)
-    conv_tile<32, 2>(R1, q + kQf2, t * 16, lane, acc);
+    conv_tile_w<32, 2>(R1, bw, t * 16, lane, acc);
     store_tile(R0, prm + kB2, t * 16, lane, acc);
   }
+  load_w(q + kQf3, lane, bw);
   __syncthreads();
-  conv_tile<16, 2>(R0, q + kQf3, wave * 16, lane, acc);      // conv3 -> R1 (8 x 8 x 32)
+  conv_tile_w<16, 2>(R0, bw, wave * 16, lane, acc);          // conv3 -> R1 (8 x 8 x 32)
   store_tile(R1, prm + kB3, wave * 16, lane, acc);
+  load_w(q + kQf4, lane, bw);
   __syncthreads();
-  conv_tile<8, 1>(R1, q + kQf4, wave * 16, lane, acc);       // conv4 -> R0 (8 x 8 x 32)
+  conv_tile_w<8, 1>(R1, bw, wave * 16, lane, acc);           // conv4 -> R0 (8 x 8 x 32)
   store_tile(R0, prm + kB4, wave * 16, lane, acc);
   __syncthreads();
   // head, in k_head's order of operations (so both paths give the same bits)
