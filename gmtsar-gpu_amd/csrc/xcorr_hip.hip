@@ -1677,6 +1677,93 @@ __global__ void __launch_bounds__(kThreads) k_time_search(DevParams P, const flo
   }
 }
 
+// -time search with register blocking: each thread owns TL consecutive lags
+// (jc0 .. jc0 + TL - 1) of one surface row; master and template rows are held
+// as doubles in LDS (converted once), so the inner loop is one broadcast
+// template load, one master load and TL + 1 FMAs per template sample: the TL
+// products and the master energy of lag jc0, from which the energies of the
+// other lags follow by exact sliding updates.  Every partial sum is an integer
+// below 2^53 (|a|, |b| < 2^17, at most 2^16 terms), so the values equal
+// k_time_search<0>'s.
+template <int TL>
+__global__ void __launch_bounds__(kThreads) k_time_search_blk(DevParams P, const float* __restrict__ amp,
+                                                              const unsigned long long* __restrict__ sums,
+                                                              double* __restrict__ surfd, int LR, int RC) {
+  extern __shared__ double dsm[];
+  __shared__ long long redl[kThreads / 64];
+  const int NX = P.NX, nxc = P.nxc, ncx = P.nx_corr, ncy = P.ny_corr;
+  const int qg = nxc / TL;                        // lag groups per surface row
+  const int nblk = P.nyc / LR;
+  const int blk = blockIdx.x % nblk, b = blockIdx.x / nblk;
+  const int ic0 = blk * LR;
+  const float m1f = (float)patch_mean(sums[2 * b], P.N);
+  const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
+  const float* a1 = amp + (size_t)(2 * b) * P.N;
+  const float* a2 = a1 + P.N;
+  double* Mb = dsm;                               // (RC + LR - 1) x NX master values
+  double* Tb = dsm + (size_t)(RC + LR - 1) * NX;  // RC x nx_corr template values
+  const int t = threadIdx.x;
+  const bool act = t < LR * qg;
+  const int lr = act ? t / qg : 0, jc0 = act ? TL * (t - (t / qg) * qg) : 0;
+  double ab[TL], aa[TL];
+#pragma unroll
+  for (int k = 0; k < TL; k++) {
+    ab[k] = 0.0;
+    aa[k] = 0.0;
+  }
+  long long b2 = 0;
+  for (int r0 = 0; r0 < ncy; r0 += RC) {
+    const int rc = min(RC, ncy - r0);
+    __syncthreads();
+    const int mrows = rc + LR - 1;
+    const size_t mrow0 = (size_t)(ic0 + r0);
+    for (int e = threadIdx.x; e < mrows * NX; e += kThreads) {
+      const int rr = e / NX, x = e - rr * NX;
+      Mb[e] = (double)(int)(a1[(mrow0 + rr) * NX + x] - m1f);
+    }
+    for (int e = threadIdx.x; e < rc * ncx; e += kThreads) {
+      const int rr = e / ncx, c = e - rr * ncx;
+      const int v = (int)(a2[(size_t)(P.ys + r0 + rr) * NX + P.xs + c] - m2f);
+      Tb[e] = (double)v;
+      b2 += (long long)v * v;
+    }
+    __syncthreads();
+    if (act) {
+      for (int rr = 0; rr < rc; rr++) {
+        const double* arow = Mb + (lr + rr) * NX + jc0;
+        const double* brow = Tb + rr * ncx;
+        double x[TL];
+#pragma unroll
+        for (int k = 0; k < TL; k++) x[k] = arow[k];
+        double e0 = 0.0;
+        for (int c = 0; c < ncx; c++) {
+          const double bv = brow[c];
+#pragma unroll
+          for (int k = 0; k < TL; k++) ab[k] = __builtin_fma(x[k], bv, ab[k]);
+          e0 = __builtin_fma(x[0], x[0], e0);
+#pragma unroll
+          for (int k = 0; k + 1 < TL; k++) x[k] = x[k + 1];
+          x[TL - 1] = arow[c + TL];               // jc0 + ncx + TL - 1 <= NX - 1: in the row
+        }
+        aa[0] += e0;
+#pragma unroll
+        for (int k = 1; k < TL; k++) {
+          e0 = e0 - arow[k - 1] * arow[k - 1] + arow[ncx + k - 1] * arow[ncx + k - 1];
+          aa[k] += e0;
+        }
+      }
+    }
+  }
+  const double d2d = (double)block_sum_i64(b2, redl);
+  if (!act) return;
+  double* sd = surfd + (size_t)b * nxc * P.nyc + (size_t)(ic0 + lr) * nxc + jc0;
+#pragma unroll
+  for (int k = 0; k < TL; k++) {
+    const double den = sqrt(1.0 * aa[k] * d2d);
+    sd[k] = (den == 0.0) ? 0.0 : 100.0 * fabs(ab[k] / den);
+  }
+}
+
 // Peak of the time-domain surface with do_time_corr's scan semantics
 // (do_time_int_xcorr.cu:162-180): m = -1; for each lag in row-major order,
 // if |g| > m then m = (float)|g| and the lag becomes the peak.  Equivalently the
@@ -2566,7 +2653,18 @@ static int time_search(xcorr_hip_ctx* c, const xcorr_hip_window*, int nb, bool t
   const size_t sh = ((size_t)(d.tRC + d.tLR - 1) * d.NX + (size_t)d.tRC * d.nx_corr) * sizeof(int);
   const dim3 grid(nb * (d.nyc / d.tLR));
   int rc;
-  if (d.corr_flag == XCORR_HIP_TIME)
+  // blocked search: LR surface rows x nxc lags per block, 4 lags per thread; RC
+  // template rows per LDS chunk (doubles, within 64 KB)
+  constexpr int TL = 8;                           // lags per thread
+  const int LRb = std::max(1, std::min(d.nyc, kThreads * TL / d.nxc));
+  const int RCb = std::min(8, (5120 - (LRb - 1) * d.NX) / (d.NX + d.nx_corr));   // <= 40 KB of LDS
+  if (d.corr_flag == XCORR_HIP_TIME && d.nxc % TL == 0 && RCb >= 1)
+    rc = launch_timed(c, timing, "k_time_search", [&] {
+      const size_t shb = ((size_t)(RCb + LRb - 1) * d.NX + (size_t)RCb * d.nx_corr) * sizeof(double);
+      hipLaunchKernelGGL(k_time_search_blk<TL>, dim3(nb * (d.nyc / LRb)), dim3(kThreads), shb, c->stream, d,
+                         (const float*)c->amp, (const unsigned long long*)c->sums, c->surfd, LRb, RCb);
+    });
+  else if (d.corr_flag == XCORR_HIP_TIME)
     rc = launch_timed(c, timing, "k_time_search", [&] {
       hipLaunchKernelGGL(k_time_search<0>, grid, dim3(kThreads), sh, c->stream, d, (const float*)c->amp,
                          (const unsigned long long*)c->sums, c->surfd);
