@@ -395,10 +395,11 @@ __global__ void __launch_bounds__(kThreads, 3) k_stageA256(DevParams P, const vo
 // x0 + j + G r): ea[q] / oa[q] are the amplitudes of output columns 2c and
 // 2c + 1, c = j + G q (q < V / 2): the original sample c + N / 4 and the
 // half-sample interpolant after it.
+// sph: the phase factors e^{2 pi i k~ / 2N} in LDS, read at the point of use.
 template <int N>
 __device__ __forceinline__ void range_row(const void* __restrict__ src, int format, int inx, int iny, bool inside,
                                           int gy, int x0, int j, float2* tile,
-                                          const float2 (&t)[RegFft<N>::TW], const float2 (&ph)[RegFft<N>::V],
+                                          const float2 (&t)[RegFft<N>::TW], const float2* sph,
                                           float (&ea)[RegFft<N>::V / 2], float (&oa)[RegFft<N>::V / 2]) {
   constexpr int G = RegFft<N>::G, V = RegFft<N>::V;
   float2 v[V];
@@ -424,8 +425,9 @@ __device__ __forceinline__ void range_row(const void* __restrict__ src, int form
 #pragma unroll
   for (int q = 0; q < V / 2; q++) ea[q] = ampl(v[q + V / 4]);
   RegFft<N>::template run<-1>(v, j, tile, t);
+  const float2* phj = sph + j + opaque_zero();
 #pragma unroll
-  for (int r = 0; r < V; r++) v[r] = cmul(v[r], ph[r]);
+  for (int r = 0; r < V; r++) v[r] = cmul(v[r], phj[G * r]);
   RegFft<N>::template run<+1>(v, j, tile, t);
   const float sc = 1.0f / (float)N;
 #pragma unroll
@@ -452,6 +454,7 @@ __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __
   constexpr int G = F::G, V = F::V, GPB = kThreads / G;
   __shared__ float2 tiles[GPB * F::TILE];
   __shared__ float2 mrow[GPB * (N / 2)];           // each group's master amplitude line (column pairs)
+  __shared__ float2 sph[N];                         // range-interpolation phase factors
   __shared__ long long red[kThreads / 64];
   const int g = threadIdx.x / G, j = threadIdx.x % G;
   const int b = blockIdx.x;
@@ -460,9 +463,8 @@ __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __
   float2* mr = mrow + g * (N / 2);
   float2 t[F::TW];
   F::load_tw(tabs + P.tw_nx, j, t);
-  float2 ph[V];
-#pragma unroll
-  for (int r = 0; r < V; r++) ph[r] = tabs[P.phi + j + G * r];
+  for (int k = threadIdx.x; k < N; k += kThreads) sph[k] = tabs[P.phi + k];
+  __syncthreads();
   const bool in_m = w.mx0 >= 0 && w.mx0 + N <= P.m_nx && w.my0 >= 0 && w.my0 + N <= P.m_ny;
   const bool in_s = w.sx0 >= 0 && w.sx0 + N <= P.s_nx && w.sy0 >= 0 && w.sy0 + N <= P.s_ny;
   float2* out1 = (float2*)(amp + (size_t)(2 * b) * P.N);
@@ -473,7 +475,7 @@ __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __
   for (int row = g; row < N; row += GPB) {        // groups never wait for other waves inside the loop
     {
       float ea[V / 2], oa[V / 2];
-      range_row<N>(master, P.format, P.m_nx, P.m_ny, in_m, w.my0 + row, w.mx0, j, tile, t, ph, ea, oa);
+      range_row<N>(master, P.format, P.m_nx, P.m_ny, in_m, w.my0 + row, w.mx0, j, tile, t, sph, ea, oa);
 #pragma unroll
       for (int q = 0; q < V / 2; q++) {
         const float2 a = make_float2(ea[q], oa[q]);
@@ -484,7 +486,7 @@ __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __
     }
     {
       float ea[V / 2], oa[V / 2];
-      range_row<N>(slave, P.format, P.s_nx, P.s_ny, in_s, w.sy0 + row, w.sx0, j, tile, t, ph, ea, oa);
+      range_row<N>(slave, P.format, P.s_nx, P.s_ny, in_s, w.sy0 + row, w.sx0, j, tile, t, sph, ea, oa);
       const bool trow = row >= P.ys && row < P.ys + P.ny_corr;
 #pragma unroll
       for (int q = 0; q < V / 2; q++) {
@@ -1037,9 +1039,12 @@ __global__ void __launch_bounds__(kThreads) k_colpass(DevParams P, const float2*
   using F = RegFft<N>;
   using C = ColCfg<N>;
   constexpr int G = F::G, V = F::V, CPG = C::CPG, TC = C::TC, PITCH = C::PITCH;
-  __shared__ float2 T[N * PITCH];
-  __shared__ float2 ftiles[C::GPB * F::TILE];
+  // the groups' transform tiles share the column tile's memory: they are live
+  // only between block barriers that separate them from every use of T
+  constexpr int TSZ = N * PITCH > C::GPB * F::TILE ? N * PITCH : C::GPB * F::TILE;
+  __shared__ float2 T[TSZ];
   __shared__ float2 sx[N];
+  float2* const ftiles = T;
   const int tiles = N / (2 * TC);
   const int p = blockIdx.x % tiles, b = blockIdx.x / tiles;
   const int g = threadIdx.x / G, j = threadIdx.x % G;
@@ -1075,6 +1080,7 @@ __global__ void __launch_bounds__(kThreads) k_colpass(DevParams P, const float2*
     F::template run<-1>(vT[c], j, ftile, t);
     F::template run<-1>(vM[c], j, ftile, t);
   }
+  __syncthreads();                                 // transform tiles (aliasing T) are done
   const bool holder = p == 0 && g == 0;           // holds column 0 (T) and N/2 (M): self-mirrored
   // M spectra in natural row order for the T products; column 0's own spectrum in sx
 #pragma unroll
@@ -1114,6 +1120,7 @@ __global__ void __launch_bounds__(kThreads) k_colpass(DevParams P, const float2*
 #pragma unroll
   for (int c = 0; c < CPG; c++) F::template run<+1>(pT[c], j, ftile, t);
   if (p == 0) F::template run<+1>(vM[0], j, ftile, t);
+  __syncthreads();                                 // transform tiles (aliasing T) are done
   // lag rows N/4 .. 3N/4 - 1 (r = V/4 .. 3V/4 - 1) through LDS, written as rows of D
 #pragma unroll
   for (int c = 0; c < CPG; c++)
