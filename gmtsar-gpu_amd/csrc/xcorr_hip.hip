@@ -1771,6 +1771,196 @@ __global__ void __launch_bounds__(kThreads) k_time_search_blk(DevParams P, const
   }
 }
 
+// --------------------------------------------------------------------------
+// -time search on the int8 matrix cores for xsearch = ysearch = 32 (128 x 128
+// windows, 64 x 64 template, 64 x 64 lags).  The integer values a (window)
+// and b (template), |a|, |b| < 2^20, are written as three signed 7-bit limbs,
+// a = l2 2^14 + l1 2^7 + l0 with l0, l1 in [-64, 63]; every limb product is
+// below 2^12 in magnitude, a lag sums 4096 of them, and at most three limb
+// pairs share a weight 2^(7 w), so the five per-weight int32 accumulators
+// cannot overflow and sum_w acc_w 2^(7 w) (int64) is the exact correlation.
+// For master row r the lags (dy, dx) get sum_x A[r][x + dx] B[r - dy][x]: a
+// Toeplitz (dx, x) times template (x, dy) product, one 16 x 16 x 64 MFMA per
+// (dx tile, dy tile, limb pair).  A patch with a value outside the limb range
+// takes an exact int64 loop instead.
+// --------------------------------------------------------------------------
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+
+// Exact master energies sum_{y, x < 64} A[y + dy][x + dx]^2 of every lag
+// (int64 sliding sums), as doubles (exact: < 2^53).  One block per patch.
+__global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const float* __restrict__ amp,
+                                                            const unsigned long long* __restrict__ sums,
+                                                            double* __restrict__ aa) {
+  __shared__ int Ai[128 * 128];
+  __shared__ long long E[127 * 64];
+  const int b = blockIdx.x;
+  const float m1f = (float)patch_mean(sums[2 * b], P.N);
+  const float* a1 = amp + (size_t)(2 * b) * P.N;
+  for (int e = threadIdx.x; e < 128 * 128; e += kThreads) Ai[e] = (int)(a1[e] - m1f);
+  __syncthreads();
+  if (threadIdx.x < 127) {                        // row energies E[r][dx] = sum_{x<64} A[r][x + dx]^2
+    const int* row = Ai + threadIdx.x * 128;
+    long long s = 0;
+    for (int x = 0; x < 64; x++) s += (long long)row[x] * row[x];
+    E[threadIdx.x * 64] = s;
+    for (int dx = 1; dx < 64; dx++) {
+      s += (long long)row[dx + 63] * row[dx + 63] - (long long)row[dx - 1] * row[dx - 1];
+      E[threadIdx.x * 64 + dx] = s;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {                         // column sums over 64 rows, slid down the lags
+    const int dx = threadIdx.x;
+    long long s = 0;
+    for (int y = 0; y < 64; y++) s += E[y * 64 + dx];
+    double* ob = aa + (size_t)b * 64 * 64;
+    ob[dx] = (double)s;
+    for (int dy = 1; dy < 64; dy++) {
+      s += E[(dy + 63) * 64 + dx] - E[(dy - 1) * 64 + dx];
+      ob[dy * 64 + dx] = (double)s;
+    }
+  }
+}
+
+__device__ __forceinline__ void limbs7(int v, signed char* l) {
+  const int l0 = ((v + 64) & 127) - 64;
+  const int v1 = (v - l0) >> 7;
+  const int l1 = ((v1 + 64) & 127) - 64;
+  l[0] = (signed char)l0;
+  l[1] = (signed char)l1;
+  l[2] = (signed char)((v1 - l1) >> 7);
+}
+
+// 16 bytes of a limb row starting at byte offset o (any alignment): two
+// aligned 16-byte LDS loads and a byte funnel shift.
+__device__ __forceinline__ v4i_t ld16_unaligned(const signed char* row, int o) {
+  const int base = o & ~15, sh = o & 15;
+  const uint4 lo = *(const uint4*)(row + base);
+  const uint4 hi = *(const uint4*)(row + base + 16);
+  const unsigned w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+  const int q = sh >> 2, bsh = (sh & 3) * 8;
+  v4i_t r;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    // select words q + k and q + k + 1 (q is 0..3) without dynamic register indexing
+    unsigned a = w[k], c = w[k + 1];
+    if (q == 1) { a = w[k + 1]; c = w[k + 2]; }
+    if (q == 2) { a = w[k + 2]; c = w[k + 3]; }
+    if (q == 3) { a = w[k + 3]; c = w[k + 4]; }
+    r[k] = (int)(bsh ? ((a >> bsh) | (c << (32 - bsh))) : a);
+  }
+  return r;
+}
+
+constexpr int kLimbPitch = 144;                   // bytes per limb row of the window (128 + 16 for the tail loads)
+
+__global__ void __launch_bounds__(kThreads) k_time_search_i8(DevParams P, const float* __restrict__ amp,
+                                                             const unsigned long long* __restrict__ sums,
+                                                             const double* __restrict__ aa,
+                                                             double* __restrict__ surfd) {
+  __shared__ __attribute__((aligned(16))) signed char Al[3 * 128 * kLimbPitch];
+  __shared__ __attribute__((aligned(16))) signed char Bl[3 * 64 * 64];
+  __shared__ long long redl[kThreads / 64];
+  __shared__ int sbig;
+  const int b = blockIdx.x;
+  const float m1f = (float)patch_mean(sums[2 * b], P.N);
+  const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
+  const float* a1 = amp + (size_t)(2 * b) * P.N;
+  const float* a2 = a1 + P.N;
+  if (threadIdx.x == 0) sbig = 0;
+  __syncthreads();
+  int big = 0;
+  for (int e = threadIdx.x; e < 128 * kLimbPitch; e += kThreads) {
+    const int r = e / kLimbPitch, x = e - r * kLimbPitch;
+    int v = 0;
+    if (x < 128) v = (int)(a1[r * 128 + x] - m1f);
+    big |= (v >= (1 << 20) || v <= -(1 << 20));
+    signed char l[3];
+    limbs7(v, l);
+#pragma unroll
+    for (int k = 0; k < 3; k++) Al[(k * 128 + r) * kLimbPitch + x] = l[k];
+  }
+  long long b2 = 0;
+  for (int e = threadIdx.x; e < 64 * 64; e += kThreads) {
+    const int y = e >> 6, x = e & 63;
+    const int v = (int)(a2[(size_t)(P.ys + y) * 128 + P.xs + x] - m2f);
+    big |= (v >= (1 << 20) || v <= -(1 << 20));
+    b2 += (long long)v * v;
+    signed char l[3];
+    limbs7(v, l);
+#pragma unroll
+    for (int k = 0; k < 3; k++) Bl[(k * 64 + y) * 64 + x] = l[k];
+  }
+  if (big) atomicOr(&sbig, 1);
+  const double d2d = (double)block_sum_i64(b2, redl);     // (its barriers also publish the limbs)
+  double* sd = surfd + (size_t)b * 64 * 64;
+  const double* ab_aa = aa + (size_t)b * 64 * 64;
+  if (sbig) {                                     // exact int64 fallback (values beyond the limb range)
+    for (int lag = threadIdx.x; lag < 64 * 64; lag += kThreads) {
+      const int dy = lag >> 6, dx = lag & 63;
+      long long num = 0;
+      for (int y = 0; y < 64; y++)
+        for (int x = 0; x < 64; x++) {
+          const long long av = (int)(a1[(y + dy) * 128 + x + dx] - m1f);
+          const long long bv = (int)(a2[(size_t)(P.ys + y) * 128 + P.xs + x] - m2f);
+          num += av * bv;
+        }
+      const double den = sqrt(1.0 * ab_aa[lag] * d2d);
+      sd[lag] = (den == 0.0) ? 0.0 : 100.0 * fabs((double)num / den);
+    }
+    return;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+  const int dx0 = 32 * (wave & 1), dy0 = 32 * (wave >> 1);   // the wave's 2 x 2 tiles of 16 x 16 lags
+  v4i_t acc[2][2][5];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int w = 0; w < 5; w++) acc[i][j][w] = v4i_t{0, 0, 0, 0};
+  const v4i_t zero = v4i_t{0, 0, 0, 0};
+  for (int r = dy0; r < dy0 + 32 + 63; r++) {     // master rows that meet the wave's lag rows
+    v4i_t A[2][3], B[2][3];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++)                 // Toeplitz rows: A[r][x + dx], dx = dx0 + 16 i + m, x = 16 g ..
+        A[i][k] = ld16_unaligned(Al + (k * 128 + r) * kLimbPitch, dx0 + 16 * i + m + 16 * g);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int y = r - (dy0 + 16 * j + m);       // template row of lag row dy = dy0 + 16 j + m
+      const bool ok = y >= 0 && y < 64;
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        B[j][k] = ok ? *(const v4i_t*)(Bl + (k * 64 + (ok ? y : 0)) * 64 + 16 * g) : zero;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int ka = 0; ka < 3; ka++)
+#pragma unroll
+          for (int kb = 0; kb < 3; kb++)
+            acc[i][j][ka + kb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[i][ka], B[j][kb], acc[i][j][ka + kb], 0, 0, 0);
+  }
+  // lane holds lags dx = dx0 + 16 i + 4 g + q, dy = dy0 + 16 j + m
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        long long num = 0;
+#pragma unroll
+        for (int w = 0; w < 5; w++) num += (long long)acc[i][j][w][q] << (7 * w);
+        const int dx = dx0 + 16 * i + 4 * g + q, dy = dy0 + 16 * j + m;
+        const double den = sqrt(1.0 * ab_aa[dy * 64 + dx] * d2d);
+        sd[dy * 64 + dx] = (den == 0.0) ? 0.0 : 100.0 * fabs((double)num / den);
+      }
+}
+
 // Peak of the time-domain surface with do_time_corr's scan semantics
 // (do_time_int_xcorr.cu:162-180): m = -1; for each lag in row-major order,
 // if |g| > m then m = (float)|g| and the lag becomes the peak.  Equivalently the
@@ -1936,6 +2126,7 @@ struct xcorr_hip_ctx {
   int tune_batch = 0;                     // locations per launch (0: max_batch)
   int nt_rows = kThreads;                 // threads of the generic row / column kernels
   bool reg_paths = true;                  // register-resident kernels where the shape has them
+  double* tcaa = nullptr;                 // -time, xsearch = ysearch = 32: master energies of every lag
 };
 
 static std::string g_create_err;
@@ -2003,7 +2194,8 @@ static int validate(const xcorr_hip_params* p, std::string* why) {
 
 static void free_scratch(xcorr_hip_ctx* c) {
   void* ptrs[] = {c->amp, c->sums, c->Z, c->D, c->surf, c->surfd, c->peak, c->partial, c->tpeak,
-                  c->tcave, c->d_win, c->d_out, c->tabs, c->amp1s, c->amp2s, c->lsum, c->Zu, c->need, c->d_plan};
+                  c->tcave, c->d_win, c->d_out, c->tabs, c->amp1s, c->amp2s, c->lsum, c->Zu, c->need, c->d_plan,
+                  c->tcaa};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (int i = 0; i < 2; i++) {
@@ -2015,7 +2207,7 @@ static void free_scratch(xcorr_hip_ctx* c) {
   c->amp = nullptr; c->sums = nullptr; c->Z = nullptr; c->D = nullptr; c->surf = nullptr;
   c->surfd = nullptr; c->peak = nullptr; c->partial = nullptr; c->tpeak = nullptr; c->tcave = nullptr;
   c->d_win = nullptr; c->d_out = nullptr; c->tabs = nullptr; c->amp1s = nullptr; c->amp2s = nullptr;
-  c->lsum = nullptr; c->Zu = nullptr; c->need = nullptr; c->d_plan = nullptr;
+  c->lsum = nullptr; c->Zu = nullptr; c->need = nullptr; c->d_plan = nullptr; c->tcaa = nullptr;
 }
 
 // Frequency-mode square 128 / 512 windows with ri 2 and the default radix
@@ -2174,6 +2366,8 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
     alloc((void**)&c->partial, B * std::max(d.groups4, d.nyc / d.G4) * sizeof(double));
   } else {
     alloc((void**)&c->surfd, B * ns * sizeof(double));
+    if (p->corr_flag == XCORR_HIP_TIME && p->xsearch == 32 && p->ysearch == 32)
+      alloc((void**)&c->tcaa, B * ns * sizeof(double));
     alloc((void**)&c->tpeak, B * 4 * sizeof(int));
     alloc((void**)&c->tcave, B * sizeof(double));
   }
@@ -2665,7 +2859,14 @@ static int time_search(xcorr_hip_ctx* c, const xcorr_hip_window*, int nb, bool t
   constexpr int TL = 8;                           // lags per thread
   const int LRb = std::max(1, std::min(d.nyc, kThreads * TL / d.nxc));
   const int RCb = std::min(8, (5120 - (LRb - 1) * d.NX) / (d.NX + d.nx_corr));   // <= 40 KB of LDS
-  if (d.corr_flag == XCORR_HIP_TIME && d.nxc % TL == 0 && RCb >= 1)
+  if (d.corr_flag == XCORR_HIP_TIME && d.xs == 32 && d.ys == 32 && c->tcaa)
+    rc = launch_timed(c, timing, "k_time_search", [&] {
+      hipLaunchKernelGGL(k_time_energy32, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float*)c->amp,
+                         (const unsigned long long*)c->sums, c->tcaa);
+      hipLaunchKernelGGL(k_time_search_i8, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float*)c->amp,
+                         (const unsigned long long*)c->sums, (const double*)c->tcaa, c->surfd);
+    });
+  else if (d.corr_flag == XCORR_HIP_TIME && d.nxc % TL == 0 && RCb >= 1)
     rc = launch_timed(c, timing, "k_time_search", [&] {
       const size_t shb = ((size_t)(RCb + LRb - 1) * d.NX + (size_t)RCb * d.nx_corr) * sizeof(double);
       hipLaunchKernelGGL(k_time_search_blk<TL>, dim3(nb * (d.nyc / LRb)), dim3(kThreads), shb, c->stream, d,

@@ -42,7 +42,7 @@ def _exact(got, ref):
     np.testing.assert_allclose(got["cave"], ref["cave"], rtol=1e-12)
 
 
-@pytest.mark.parametrize("xs,ys", [(8, 8), (16, 8), (16, 32)])
+@pytest.mark.parametrize("xs,ys", [(8, 8), (16, 8), (16, 32), (32, 32)])
 @pytest.mark.parametrize("corr_flag", [0, 1], ids=["time", "time4"])
 def test_time_bit_exact(xg, orc, pair, xs, ys, corr_flag):
     p, got, ref, _, _ = _run(xg, orc, pair, xs, ys, corr_flag)
@@ -72,3 +72,22 @@ def test_time_nointerp_and_norange(xg, orc, pair):
     _, got, ref, _, _ = _run(xg, orc, pair, 16, 16, 0, ri=1, interp=0)
     _exact(got, ref)
     assert np.all(got["xfrac"] == 0) and np.all(got["hip"] == -1)
+
+
+def test_time_surface_xs32_matches_int64_bruteforce(xg, orc, pair):
+    """-xsearch 32 -ysearch 32 -time runs on the int8 matrix cores (limb
+    decomposition): its surface must equal the exact int64 correlation."""
+    from numpy.lib.stride_tricks import sliding_window_view
+    p, got, ref, surf, (a1, a2, means) = _run(xg, orc, pair, 32, 32, 0, nxl=2, nyl=1)
+    _exact(got, ref)
+    m1f, m2f = np.float32(means[0]), np.float32(means[1])
+    i1 = (a1 - m1f).astype(np.int64)
+    i2 = (a2 - m2f).astype(np.int64)
+    b = i2[32:96, 32:96]
+    win = sliding_window_view(i1[:127, :127], (64, 64))          # (64, 64, 64, 64): lag (dy, dx), then (y, x)
+    num = np.einsum("ijkl,kl->ij", win, b)
+    d1 = np.einsum("ijkl,ijkl->ij", win, win)
+    d2 = int((b * b).sum())
+    den = np.sqrt(d1.astype(np.float64) * float(d2))
+    brute = np.where(den == 0, 0.0, 100.0 * np.abs(num / np.where(den == 0, 1.0, den)))
+    assert np.array_equal(surf, brute)
