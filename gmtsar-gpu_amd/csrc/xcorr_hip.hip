@@ -444,7 +444,7 @@ __device__ __forceinline__ void range_row(const void* __restrict__ src, int form
 // everywhere, the slave's on the template), exact fixed-point sums, then
 // z = amp1 + i amp2 mask and its FFT along x (the means come off in the
 // spectrum, demean_row_bin), Hermitian completion of unmasked rows.
-template <int N>
+template <int N, bool ROWFFT>
 __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __restrict__ master,
                                                      const void* __restrict__ slave,
                                                      const xcorr_hip_window* __restrict__ win,
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __
   const bool in_s = w.sx0 >= 0 && w.sx0 + N <= P.s_nx && w.sy0 >= 0 && w.sy0 + N <= P.s_ny;
   float2* out1 = (float2*)(amp + (size_t)(2 * b) * P.N);
   float2* out2 = (float2*)(amp + (size_t)(2 * b + 1) * P.N);
-  float2* zb = Z + (size_t)b * P.N;
+  float2* zb = ROWFFT ? Z + (size_t)b * P.N : nullptr;
   const int c_lo = P.xs / 2, c_hi = (P.xs + P.nx_corr) / 2;   // template column pairs
   long long s1 = 0, s2 = 0;
   for (int row = g; row < N; row += GPB) {        // groups never wait for other waves inside the loop
@@ -480,7 +480,7 @@ __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __
       for (int q = 0; q < V / 2; q++) {
         const float2 a = make_float2(ea[q], oa[q]);
         out1[(size_t)row * (N / 2) + j + G * q] = a;
-        mr[j + G * q] = a;
+        if (ROWFFT) mr[j + G * q] = a;
         s1 += fxp20(ea[q]) + fxp20(oa[q]);
       }
     }
@@ -493,10 +493,11 @@ __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __
         const float2 a = make_float2(ea[q], oa[q]);
         const int c = j + G * q;
         if (trow && c >= c_lo && c < c_hi) out2[(size_t)row * (N / 2) + c] = a;
-        tile[c] = a;                               // the tile doubles as the slave line buffer
+        if (ROWFFT) tile[c] = a;                   // the tile doubles as the slave line buffer
         s2 += fxp20(ea[q]) + fxp20(oa[q]);
       }
     }
+    if (!ROWFFT) continue;                         // time modes: amplitudes and sums only
     wave_sync();                                   // the group's line buffers are its own
     const float* m1 = (const float*)mr;
     const float* a2 = (const float*)tile;
@@ -2745,15 +2746,27 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
   const bool fusedA = row256 && d.ri == 2 && freq;
   // register-resident stage (a) + row transform for square 128 / 512 windows
   const bool regA = reg_shape(c);
+  // time modes on square 128 / 512 windows: the same register-resident stage A without the row transform
+  const bool regT = !freq && c->reg_paths && d.ri == 2 && d.NX == d.NY && (d.NX == 128 || d.NX == 512) &&
+                    same_plan(d.plx, plan_for(d.NX));
   int rc;
   if (regA)
     rc = launch_timed(c, timing, "k_stageA", [&] {
       if (d.NX == 128)
-        hipLaunchKernelGGL(k_stageA<128>, dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_win,
-                           c->amp, c->sums, c->Z, (const float2*)c->tabs);
+        hipLaunchKernelGGL((k_stageA<128, true>), dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave,
+                           d_win, c->amp, c->sums, c->Z, (const float2*)c->tabs);
       else
-        hipLaunchKernelGGL(k_stageA<512>, dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_win,
-                           c->amp, c->sums, c->Z, (const float2*)c->tabs);
+        hipLaunchKernelGGL((k_stageA<512, true>), dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave,
+                           d_win, c->amp, c->sums, c->Z, (const float2*)c->tabs);
+    });
+  else if (regT)
+    rc = launch_timed(c, timing, "k_stageA", [&] {
+      if (d.NX == 128)
+        hipLaunchKernelGGL((k_stageA<128, false>), dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master,
+                           c->d_slave, d_win, c->amp, c->sums, (float2*)nullptr, (const float2*)c->tabs);
+      else
+        hipLaunchKernelGGL((k_stageA<512, false>), dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master,
+                           c->d_slave, d_win, c->amp, c->sums, (float2*)nullptr, (const float2*)c->tabs);
     });
   else if (fusedA)
     rc = launch_timed(c, timing, "k_stageA256", [&] {
