@@ -1855,10 +1855,11 @@ __device__ __forceinline__ v4i_t ld16_unaligned(const signed char* row, int o) {
 
 constexpr int kLimbPitch = 144;                   // bytes per limb row of the window (128 + 16 for the tail loads)
 
-__global__ void __launch_bounds__(kThreads) k_time_search_i8(DevParams P, const float* __restrict__ amp,
+__global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, const float* __restrict__ amp,
                                                              const unsigned long long* __restrict__ sums,
                                                              const double* __restrict__ aa,
-                                                             double* __restrict__ surfd) {
+                                                             double* __restrict__ surfd, int* __restrict__ tpeak,
+                                                             double* __restrict__ tcave) {
   __shared__ __attribute__((aligned(16))) signed char Al[3 * 128 * kLimbPitch];
   __shared__ __attribute__((aligned(16))) signed char Bl[3 * 64 * 64];
   __shared__ long long redl[kThreads / 64];
@@ -1896,7 +1897,10 @@ __global__ void __launch_bounds__(kThreads) k_time_search_i8(DevParams P, const 
   const double d2d = (double)block_sum_i64(b2, redl);     // (its barriers also publish the limbs)
   double* sd = surfd + (size_t)b * 64 * 64;
   const double* ab_aa = aa + (size_t)b * 64 * 64;
+  // the surface is staged in LDS (over the window limbs, padded 1 per 16) for the peak scan
+  double* gs = (double*)Al;
   if (sbig) {                                     // exact int64 fallback (values beyond the limb range)
+    __syncthreads();
     for (int lag = threadIdx.x; lag < 64 * 64; lag += kThreads) {
       const int dy = lag >> 6, dx = lag & 63;
       long long num = 0;
@@ -1907,10 +1911,11 @@ __global__ void __launch_bounds__(kThreads) k_time_search_i8(DevParams P, const 
           num += av * bv;
         }
       const double den = sqrt(1.0 * ab_aa[lag] * d2d);
-      sd[lag] = (den == 0.0) ? 0.0 : 100.0 * fabs((double)num / den);
+      const double gv = (den == 0.0) ? 0.0 : 100.0 * fabs((double)num / den);
+      sd[lag] = gv;
+      gs[lag + (lag >> 4)] = gv;
     }
-    return;
-  }
+  } else {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
   const int dx0 = 32 * (wave & 1), dy0 = 32 * (wave >> 1);   // the wave's 2 x 2 tiles of 16 x 16 lags
   v4i_t acc[2][2][5];
@@ -1946,6 +1951,7 @@ __global__ void __launch_bounds__(kThreads) k_time_search_i8(DevParams P, const 
           for (int kb = 0; kb < 3; kb++)
             acc[i][j][ka + kb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[i][ka], B[j][kb], acc[i][j][ka + kb], 0, 0, 0);
   }
+  __syncthreads();                                // every wave is done with the limbs
   // lane holds lags dx = dx0 + 16 i + 4 g + q, dy = dy0 + 16 j + m
 #pragma unroll
   for (int i = 0; i < 2; i++)
@@ -1957,9 +1963,58 @@ __global__ void __launch_bounds__(kThreads) k_time_search_i8(DevParams P, const 
 #pragma unroll
         for (int w = 0; w < 5; w++) num += (long long)acc[i][j][w][q] << (7 * w);
         const int dx = dx0 + 16 * i + 4 * g + q, dy = dy0 + 16 * j + m;
-        const double den = sqrt(1.0 * ab_aa[dy * 64 + dx] * d2d);
-        sd[dy * 64 + dx] = (den == 0.0) ? 0.0 : 100.0 * fabs((double)num / den);
+        const int lag = dy * 64 + dx;
+        const double den = sqrt(1.0 * ab_aa[lag] * d2d);
+        const double gv = (den == 0.0) ? 0.0 : 100.0 * fabs((double)num / den);
+        sd[lag] = gv;
+        gs[lag + (lag >> 4)] = gv;
       }
+  }
+  __syncthreads();
+  // Peak with do_time_corr's scan semantics (do_time_int_xcorr.cu:162-180), as
+  // k_time_peak: thread t scans lags 16 t .. 16 t + 15 knowing the maximum of
+  // all earlier lags (an exclusive prefix maximum over the threads).
+  const int t = threadIdx.x;
+  const double* gc = gs + 17 * t;
+  double mx = -1.0, sm = 0.0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    mx = gc[k] > mx ? gc[k] : mx;
+    sm += gc[k];
+  }
+  __shared__ double wmax[kThreads / 64];
+  double inc = mx;                                // inclusive prefix maximum within the wave
+  for (int o = 1; o < 64; o <<= 1) {
+    const double v = __shfl_up(inc, o);
+    if ((t & 63) >= o) inc = v > inc ? v : inc;
+  }
+  if ((t & 63) == 63) wmax[t >> 6] = inc;
+  __syncthreads();
+  double pm = __shfl_up(inc, 1);
+  if ((t & 63) == 0) pm = -1.0;
+  for (int u = 0; u < (t >> 6); u++) pm = wmax[u] > pm ? wmax[u] : pm;
+  long long last = -1;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const double v = gc[k];
+    if (v > (double)(float)pm) last = 16 * t + k;
+    pm = v > pm ? v : pm;
+  }
+  __shared__ unsigned long long redk[kThreads / 64];
+  const unsigned long long key = block_max_u64((unsigned long long)(last + 1), redk);
+  const double gmax = wmax[0] > wmax[1] ? (wmax[0] > wmax[2] ? (wmax[0] > wmax[3] ? wmax[0] : wmax[3])
+                                                             : (wmax[2] > wmax[3] ? wmax[2] : wmax[3]))
+                                        : (wmax[1] > wmax[2] ? (wmax[1] > wmax[3] ? wmax[1] : wmax[3])
+                                                             : (wmax[2] > wmax[3] ? wmax[2] : wmax[3]));
+  const double ssum = block_sum_f64(sm, (double*)redk);
+  if (t == 0) {
+    const int idx = (int)key - 1;
+    tpeak[4 * b + 0] = idx / 64;
+    tpeak[4 * b + 1] = idx % 64;
+    tpeak[4 * b + 2] = __float_as_int((float)gmax);
+    tpeak[4 * b + 3] = 0;
+    tcave[b] = ssum / 4096.0;
+  }
 }
 
 // Peak of the time-domain surface with do_time_corr's scan semantics
@@ -2872,12 +2927,14 @@ static int time_search(xcorr_hip_ctx* c, const xcorr_hip_window*, int nb, bool t
   constexpr int TL = 8;                           // lags per thread
   const int LRb = std::max(1, std::min(d.nyc, kThreads * TL / d.nxc));
   const int RCb = std::min(8, (5120 - (LRb - 1) * d.NX) / (d.NX + d.nx_corr));   // <= 40 KB of LDS
-  if (d.corr_flag == XCORR_HIP_TIME && d.xs == 32 && d.ys == 32 && c->tcaa)
+  const bool i8 = d.corr_flag == XCORR_HIP_TIME && d.xs == 32 && d.ys == 32 && c->tcaa;   // peak found in-kernel
+  if (i8)
     rc = launch_timed(c, timing, "k_time_search", [&] {
       hipLaunchKernelGGL(k_time_energy32, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float*)c->amp,
                          (const unsigned long long*)c->sums, c->tcaa);
       hipLaunchKernelGGL(k_time_search_i8, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float*)c->amp,
-                         (const unsigned long long*)c->sums, (const double*)c->tcaa, c->surfd);
+                         (const unsigned long long*)c->sums, (const double*)c->tcaa, c->surfd, c->tpeak,
+                         c->tcave);
     });
   else if (d.corr_flag == XCORR_HIP_TIME && d.nxc % TL == 0 && RCb >= 1)
     rc = launch_timed(c, timing, "k_time_search", [&] {
@@ -2895,7 +2952,7 @@ static int time_search(xcorr_hip_ctx* c, const xcorr_hip_window*, int nb, bool t
       hipLaunchKernelGGL(k_time_search<1>, grid, dim3(kThreads), sh, c->stream, d, (const float*)c->amp,
                          (const unsigned long long*)c->sums, c->surfd);
     });
-  if (rc) return rc;
+  if (rc || i8) return rc;
   return launch_timed(c, timing, "k_time_peak", [&] {
     hipLaunchKernelGGL(k_time_peak, dim3(nb), dim3(kThreads), 0, c->stream, d, (const double*)c->surfd, c->tpeak,
                        c->tcave);
