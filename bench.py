@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--tau", type=float, default=0.9, help="proposal: confidence threshold of the shrunken search")
     ap.add_argument("--tau-skip", type=float, default=1.01, help="proposal: confidence above which high-res is skipped")
     ap.add_argument("--xs-small", type=int, default=32, help="proposal: xsearch of the shrunken search")
+    ap.add_argument("--verify", action="store_true",
+                    help="proposal: redo shrunken searches whose peak is on the lag-window border with plain xcorr")
+    ap.add_argument("--corr-min", type=float, default=0.0, help="proposal: verification also redoes corr below this")
     ap.add_argument("--weights", default=os.path.join(ROOT, "gmtsar-gpu_amd", "weights", "proposal_cnn.npy"))
     ap.add_argument("--pairs", type=int, default=64, help="batched: number of SLC pairs")
     ap.add_argument("--no-tune", action="store_true", help="batched: skip the autotuner")
@@ -358,7 +361,7 @@ def bench_proposal(args, xg, synth, torch, dev):
     w = xg.make_windows(x, y, p.npx, p.npy)
     n = len(w)
     pr = prop.Proposer(params, tau=args.tau, tau_skip=args.tau_skip, max_batch=min(args.batch, n),
-                       xs_small=args.xs_small)
+                       xs_small=args.xs_small, verify=args.verify, corr_min=args.corr_min)
     pr.set_images(m, s)
     torch.cuda.synchronize()
 
@@ -379,22 +382,27 @@ def bench_proposal(args, xg, synth, torch, dev):
     ex, ey = dx - pdx, dy - pdy
     summary = {}
     for tau in (0.3, 0.5, 0.7, 0.9):
-        for skip in (False, True):
-            pr.tau, pr.tau_skip = tau, (tau if skip else 1.01)
+        for mode in ("", "_skip", "_verify"):
+            pr.tau, pr.tau_skip = tau, (tau if mode == "_skip" else 1.01)
+            pr.verify, pr.corr_min = mode == "_verify", args.corr_min
+            t3 = time.perf_counter()
             dxx, dyy, _, pth = pr.run(w, pred=pred)
+            t3 = time.perf_counter() - t3 + t_pred
             e = np.concatenate([dxx - pdx, dyy - pdy])
-            summary["tau%.1f%s" % (tau, "_skip" if skip else "")] = {
-                "shrunken": round(float((pth > 0).mean()), 4),
-                "rmse_px": round(float(np.sqrt(np.mean(e ** 2))), 4)}
-    pr.tau, pr.tau_skip = args.tau, args.tau_skip
+            summary["tau%.1f%s" % (tau, mode)] = {
+                "shrunken": round(float(((pth == 1) | (pth == 2)).mean()), 4),
+                "rmse_px": round(float(np.sqrt(np.mean(e ** 2))), 4),
+                "speedup": round(t_plain / t3, 3)}
+    pr.tau, pr.tau_skip, pr.verify, pr.corr_min = args.tau, args.tau_skip, args.verify, args.corr_min
     conf = pred[2]
-    sel = path > 0
+    sel = (path == 1) | (path == 2)
     pe = np.abs(pred[0] - pdx), np.abs(pred[1] - pdy)
     proposal = {
         "plain_patches_per_s": round(n / t_plain, 2), "proposal_patches_per_s": round(n / t_prop, 2),
         "speedup": round(t_plain / t_prop, 4), "cnn_ms": round(t_pred * 1e3, 3),
-        "tau": args.tau, "tau_skip": args.tau_skip,
-        "shrunken_fraction": round(float(sel.mean()), 4),
+        "tau": args.tau, "tau_skip": args.tau_skip, "verify": args.verify, "corr_min": args.corr_min,
+        "shrunken_fraction": round(float(((path == 1) | (path == 2)).mean()), 4),
+        "verified_redone_fraction": round(float((path == 3).mean()), 4),
         "skipped_highres_fraction": round(float((path == 2).mean()), 4),
         "rmse_dx_px": round(float(np.sqrt(np.mean(ex ** 2))), 4),
         "rmse_dy_px": round(float(np.sqrt(np.mean(ey ** 2))), 4),

@@ -226,14 +226,18 @@ class Proposer:
     """Proposal form of the xcorr path for 256 x 256 windows on one image pair
     (torch device tensors).  ``run`` returns the composed offsets, correlation
     and the path every window took (0 plain fallback, 1 shrunken search with
-    high-res, 2 shrunken search without high-res)."""
+    high-res, 2 shrunken search without high-res, 3 shrunken search rejected by
+    the verification -- peak on the border of the shrunken lag window or
+    correlation below corr_min -- and redone with the plain search)."""
 
-    def __init__(self, params, tau=0.5, tau_skip=1.01, max_batch=8192, device=0, interp_factor=16, xs_small=32):
+    def __init__(self, params, tau=0.5, tau_skip=1.01, max_batch=8192, device=0, interp_factor=16, xs_small=32,
+                 verify=False, corr_min=0.0):
         import torch
         self.torch = torch
         self.net = ProposalNet(max_batch=512, device=device)
         self.net.set_params(params)
         self.tau, self.tau_skip, self.xs_small = tau, tau_skip, xs_small
+        self.verify, self.corr_min = verify, corr_min
         self.full = Engine(make_params(64, 64, interp_factor=interp_factor), max_batch=max_batch, device=device)
         self.small = Engine(make_params(xs_small, xs_small, interp_factor=interp_factor), max_batch=max_batch,
                             device=device)
@@ -282,6 +286,20 @@ class Proposer:
             r = res.cpu().numpy().view(RESULT_DTYPE)
             out[idx] = r
             dx[idx], dy[idx] = compose(r, ww, ri)
+        if self.verify:
+            # a shrunken search whose integer peak lies on the border of its lag
+            # window (the true peak may lie outside it) or whose correlation is
+            # below corr_min is redone with the plain search (path 3)
+            lag = 2 * self.xs_small
+            sh = path > 0
+            bad = sh & ((out["ip"] <= 0) | (out["ip"] >= lag - 1) | (out["jp"] <= 0) | (out["jp"] >= lag - 1) |
+                        (out["corr"] < self.corr_min))
+            idx = np.nonzero(bad)[0]
+            if idx.size:
+                r = self.full.correlate(w[idx])
+                out[idx] = r
+                dx[idx], dy[idx] = compose(r, w[idx], 2)
+                path[idx] = 3
         return dx, dy, out, path
 
     def close(self):

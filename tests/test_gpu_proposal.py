@@ -4,6 +4,9 @@
   plain engine's results, the shrunken-search windows equal the small engine's
   results on the displaced windows, bit for bit, and the composed offsets add
   the slave window's displacement as print_results.c:37-38 adds the PRM shifts.
+  With verification on, exactly the shrunken windows whose peak lies on the
+  lag-window border or whose correlation is below the threshold are redone,
+  and equal the plain engine's results.
 * The trained network (gmtsar-gpu_amd/weights) gains over the method without
   it: on a pair whose offsets (17..27 px in range, -16..-26 px in azimuth) lie
   beyond the shrunken search's reach from a zero prior, its predictions are
@@ -80,6 +83,20 @@ def test_proposal_paths_are_plain_xcorr_runs(xg, synth, env):
         assert np.array_equal(dx[k0], pdx[k0]) and np.array_equal(dy[k0], pdy[k0])
         err = np.sqrt(np.mean(np.concatenate([dx - pdx, dy - pdy]) ** 2))
         print("config-1 pair: shrunken %.1f %%, rmse vs plain %.4f px" % (100 * (path > 0).mean(), err))
+        # verification: windows whose shrunken peak lies on the lag-window border
+        # or whose correlation is low are redone by the plain search (path 3)
+        pr.verify, pr.corr_min = True, float(np.median(res["corr"][path > 0]))
+        vdx, vdy, vres, vpath = pr.run(w, pred=pred)
+        k3 = vpath == 3
+        print("verified: redone %d of %d shrunken" % (k3.sum(), (path > 0).sum()))
+        assert k3.any() and np.all(path[k3] > 0) and np.all(vpath[~k3] == path[~k3])
+        _same(vres[k3], plain[k3])
+        assert np.array_equal(vdx[k3], pdx[k3]) and np.array_equal(vdy[k3], pdy[k3])
+        keep = ~k3
+        _same(vres[keep], res[keep])
+        lag = 2 * pr.xs_small
+        kept = (vpath == 1) | (vpath == 2)
+        assert np.all((vres["ip"][kept] > 0) & (vres["ip"][kept] < lag - 1) & (vres["corr"][kept] >= pr.corr_min))
     finally:
         pr.close()
 
