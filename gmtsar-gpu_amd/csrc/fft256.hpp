@@ -5,7 +5,9 @@
 // fft_spec.hpp (pass 1: radix 16, L = 1; pass 2: radix 16, L = 16, twiddle
 // w^(r j)), so results are bit-identical to lds_fft_rows for n = 256; only the
 // data movement differs: one transpose through a padded 16 x 17 LDS tile per
-// group instead of a full LDS round trip per pass.
+// group instead of a full LDS round trip per pass.  A group never spans two
+// waves, so the transposes synchronise the wave only (wave_sync): groups run
+// their transforms independently, also under divergent control flow.
 #pragma once
 #include "fft_spec.hpp"
 
@@ -21,17 +23,16 @@ __device__ __forceinline__ void load_tw256(const float2* __restrict__ tw, int j,
 }
 
 // In:  v[r] = x[j + 16 r] (lane j of the group).  Out: v[r] = X[j + 16 r].
-// `tile` is the group's private 16 x 17 scratch.  All 256 threads of the block
-// must call this together (it contains block barriers).
+// `tile` is the group's private 16 x 17 scratch.
 template <int DIR>
 __device__ __forceinline__ void fft256_group(float2 (&v)[16], int j, float2* tile, const float2 (&t)[15]) {
   Dft<16, DIR>::run(v);                       // pass 1 -> y[16 j + r] = v[r]
 #pragma unroll
   for (int r = 0; r < 16; r++) tile[j * kT256Pitch + r] = v[r];
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int r = 0; r < 16; r++) v[r] = tile[r * kT256Pitch + j];   // y[j + 16 r]
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int r = 1; r < 16; r++) {
     float2 w = t[r - 1];
@@ -42,12 +43,12 @@ __device__ __forceinline__ void fft256_group(float2 (&v)[16], int j, float2* til
 }
 
 // Hermitian completion of a row spectrum held as v[r] = X[j + 16 r]: when
-// `apply`, X[k] = conj(X[256 - k]) for k > 128.  All 256 threads of the block
-// call this together (block barriers); `tile` is the group's 16 x 17 scratch.
+// `apply`, X[k] = conj(X[256 - k]) for k > 128; `tile` is the group's 16 x 17
+// scratch.
 __device__ __forceinline__ void hermitian_upper256(float2 (&v)[16], int j, float2* tile, bool apply) {
 #pragma unroll
   for (int r = 0; r < 16; r++) tile[r * kT256Pitch + j] = v[r];
-  __syncthreads();
+  wave_sync();
   if (apply) {
 #pragma unroll
     for (int r = 8; r < 16; r++) {
@@ -59,7 +60,7 @@ __device__ __forceinline__ void hermitian_upper256(float2 (&v)[16], int j, float
       }
     }
   }
-  __syncthreads();
+  wave_sync();
 }
 
 // An offset of 0 the compiler cannot see through (an empty asm statement, no
@@ -78,10 +79,10 @@ __device__ __forceinline__ void fft256_group_lds(float2 (&v)[16], int j, float2*
   Dft<16, DIR>::run(v);
 #pragma unroll
   for (int r = 0; r < 16; r++) tile[j * kT256Pitch + r] = v[r];
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int r = 0; r < 16; r++) v[r] = tile[r * kT256Pitch + j];
-  __syncthreads();
+  wave_sync();
   const float2* tj = twl + 15 * j + opaque_zero();
 #pragma unroll
   for (int r = 1; r < 16; r++) {

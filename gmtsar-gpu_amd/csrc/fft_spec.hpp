@@ -43,6 +43,16 @@ __host__ __device__ inline FftPlan plan_for(int n) {
   return p;
 }
 
+// Barrier for LDS data exchanged among the lanes of one wave (every transform
+// group of the register-resident kernels lies within one wave): completes the
+// wave's LDS writes and keeps the compiler from moving LDS accesses across it.
+// Other waves are not waited for.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {

@@ -16,15 +16,6 @@
 
 namespace xc {
 
-// Barrier for LDS data exchanged among the lanes of one wave (every group
-// here lies within one wave): completes the wave's LDS writes and keeps the
-// compiler from moving LDS accesses across it.  Other waves are not waited for.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 template <int N>
 struct RegFft;
 

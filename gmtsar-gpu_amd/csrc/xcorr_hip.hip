@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_stageA256(DevParams P, const vo
         s2 += fxp20(ea[q]) + fxp20(oa[q]);
       }
     }
-    __syncthreads();
+    wave_sync();                                   // the group's two lines are in LDS
     const float* m1 = (const float*)mr;
     const float* a2 = (const float*)tile;
     const bool in_y = row >= P.ys && row < NY - P.ys;
@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_stageA256(DevParams P, const vo
       const bool in = in_y && x >= P.xs && x < 256 - P.xs;
       v[r] = make_float2(m1[x], in ? a2[x] : 0.0f);
     }
-    __syncthreads();
+    wave_sync();                                   // read before the transform reuses the tile
     fft256_group<-1>(v, j, tile, t);
     hermitian_upper256(v, j, tile, !in_y);        // unmasked rows: real input
 #pragma unroll
@@ -736,7 +736,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_stripA256(DevParams P, const vo
       const long long st = s.group_total();
       if (act && j == 0) lsum[2 * l + 1] = st;
     }
-    const bool anyM = __syncthreads_or(wm);
+    wave_sync();                                // the group's two lines are in LDS
     const float* m1 = (const float*)mr;
     const float* a2 = (const float*)tile;
     float2 v[16];
@@ -745,22 +745,22 @@ __global__ void __launch_bounds__(kThreads, 2) k_stripA256(DevParams P, const vo
       const int x = j + 16 * r;
       v[r] = make_float2(m1[x], (r >= 4 && r < 12) ? a2[x] : 0.0f);
     }
-    const bool anyU = __syncthreads_or(wu);
-    if (anyM) {
+    wave_sync();                                // read before the transforms reuse the tile
+    // wm / wu are uniform over the group's 16 lanes (one line per group)
+    if (wm) {
       fft256_group_lds<-1>(v, j, tile, twl);
-      if (wm)
 #pragma unroll
-        for (int r = 0; r < 16; r++) Zm[l * 256 + j + 16 * r] = v[r];
+      for (int r = 0; r < 16; r++) Zm[l * 256 + j + 16 * r] = v[r];
     }
-    if (anyU) {
+    if (wu) {
 #pragma unroll
       for (int r = 0; r < 16; r++) v[r] = make_float2(m1[j + 16 * r], 0.0f);
       fft256_group_lds<-1>(v, j, tile, twl);
-      if (wu)                                   // bins 0..128; the rest are their conjugate mirrors
 #pragma unroll
-        for (int r = 0; r <= 8; r++)
-          if (r < 8 || j == 0) Zu[l * 256 + j + 16 * r] = v[r];
+      for (int r = 0; r <= 8; r++)              // bins 0..128; the rest are their conjugate mirrors
+        if (r < 8 || j == 0) Zu[l * 256 + j + 16 * r] = v[r];
     }
+    wave_sync();                                // the next line's loads into mr / tile follow
   }
 }
 
@@ -908,6 +908,7 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
   __syncthreads();
   fft256_group<-1>(vT, j, T + g * kT256Tile, t);
   fft256_group<-1>(vM, j, T + g * kT256Tile, t);
+  __syncthreads();                                           // every group is done with its tile
   // T products need the M spectra at the mirrored rows
 #pragma unroll
   for (int r = 0; r < 16; r++) T[(j + 16 * r) * kT256Pitch + g] = vM[r];
@@ -941,6 +942,7 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
   __syncthreads();
   fft256_group<+1>(pT, j, T + g * kT256Tile, t);
   if (p == 0) fft256_group<+1>(vM, j, T + g * kT256Tile, t);
+  __syncthreads();                                           // every group is done with its tile
   // lag rows 64..191 (r = 4..11)
 #pragma unroll
   for (int q = 0; q < 8; q++) T[(j + 16 * q) * kT256Pitch + g] = pT[q + 4];
@@ -975,7 +977,7 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
   const float sc = 1.0f / (float)P.N;
   unsigned long long best = 0ull;
   double sum = 0.0;
-  const int iters = (P.nyc + 15) >> 4;          // uniform trip count: the transform has block barriers
+  const int iters = (P.nyc + 15) >> 4;
   // row loads run one iteration ahead of the transform
   auto load_row = [&](int row, float2 (&u)[16]) {
 #pragma unroll
