@@ -848,9 +848,11 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
                                                          float2* __restrict__ D, const float2* __restrict__ tabs) {
   __shared__ float2 T[256 * kT256Pitch];
   __shared__ float2 sx[256];                                 // spectrum of a self-mirrored column
+  __shared__ float2 twl[240];                                // pass-2 twiddles w256^(r j), twl[15 j + r - 1]
   const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
   const int NX = P.NX;
   const int pairs = NX >> 5;
+  for (int k = threadIdx.x; k < 240; k += kThreads) twl[k] = tabs[P.tw_ny + (k % 15 + 1) * (k / 15)];
   const int p = blockIdx.x % pairs, b = blockIdx.x / pairs;
   const size_t row0 = rb ? (size_t)rb[b] : (size_t)b * 256;
   auto zrow = [&](int y) -> const float2* {
@@ -860,8 +862,6 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
   const int kxT = 16 * p + g;
   const int kxM = (p == 0 && g == 0) ? NX / 2 : NX - 16 * p - g;
   const bool self = (p == 0 && g == 0);                      // columns 0 and npx/2 mirror onto themselves
-  float2 t[15];
-  load_tw256(tabs + P.tw_ny, j, t);
   const float m1NX = (float)patch_mean(sums[2 * b], P.N) * (float)NX;
   const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
   float2 vT[16], vM[16];
@@ -906,8 +906,8 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
 #pragma unroll
   for (int r = 0; r < 16; r++) vM[r] = T[(j + 16 * r) * kT256Pitch + g];
   __syncthreads();
-  fft256_group<-1>(vT, j, T + g * kT256Tile, t);
-  fft256_group<-1>(vM, j, T + g * kT256Tile, t);
+  fft256_group_lds<-1>(vT, j, T + g * kT256Tile, twl);
+  fft256_group_lds<-1>(vM, j, T + g * kT256Tile, twl);
   __syncthreads();                                           // every group is done with its tile
   // T products need the M spectra at the mirrored rows
 #pragma unroll
@@ -940,8 +940,8 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
       }
   }
   __syncthreads();
-  fft256_group<+1>(pT, j, T + g * kT256Tile, t);
-  if (p == 0) fft256_group<+1>(vM, j, T + g * kT256Tile, t);
+  fft256_group_lds<+1>(pT, j, T + g * kT256Tile, twl);
+  if (p == 0) fft256_group_lds<+1>(vM, j, T + g * kT256Tile, twl);
   __syncthreads();                                           // every group is done with its tile
   // lag rows 64..191 (r = 4..11)
 #pragma unroll
@@ -965,6 +965,7 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
                                                               double* __restrict__ partial,
                                                               const float2* __restrict__ tabs) {
   __shared__ float2 tiles[16 * kT256Tile];
+  __shared__ float2 twl[240];                   // pass-2 twiddles w256^(r j), twl[15 j + r - 1]
   __shared__ unsigned long long redk[kThreads / 64];
   __shared__ double redd[kThreads / 64];
   const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
@@ -972,8 +973,8 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
   const float2* db = D + (size_t)b * P.nyc * 256;
   float* sb = surf + (size_t)b * P.nyc * 128;
   float2* tile = tiles + g * kT256Tile;
-  float2 t[15];
-  load_tw256(tabs + P.tw_nx, j, t);
+  for (int k = threadIdx.x; k < 240; k += kThreads) twl[k] = tabs[P.tw_nx + (k % 15 + 1) * (k / 15)];
+  __syncthreads();
   const float sc = 1.0f / (float)P.N;
   unsigned long long best = 0ull;
   double sum = 0.0;
@@ -987,19 +988,15 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
       if (row < P.nyc) u[r] = db[(size_t)row * 256 + (x <= 128 ? x : 256 - x)];
     }
   };
-  float2 nx[16];
-  load_row(g, nx);
   for (int it = 0; it < iters; it++) {
     const int row = it * 16 + g;
     const bool active = row < P.nyc;
     float2 v[16];
+    load_row(row, v);
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-      v[r] = nx[r];
+    for (int r = 0; r < 16; r++)
       if (j + 16 * r > 128) v[r].y = -v[r].y;      // upper half: conjugate mirror
-    }
-    if (it + 1 < iters) load_row(row + 16, nx);
-    fft256_group<+1>(v, j, tile, t);
+    fft256_group_lds<+1>(v, j, tile, twl);
     if (!active) continue;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
