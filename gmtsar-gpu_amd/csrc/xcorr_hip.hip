@@ -629,21 +629,26 @@ __global__ void k_need_mark(DevParams P, const int* __restrict__ rb, unsigned ch
   }
 }
 
-__device__ __forceinline__ long long group16_sum_i64(long long v) {
-  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o);
+// Sums over a group of G consecutive lanes (G a power of two <= 64).
+template <int G>
+__device__ __forceinline__ long long group_sum_i64(long long v) {
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
-__device__ __forceinline__ double group16_sum_f64(double v) {
-  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o);
+template <int G>
+__device__ __forceinline__ double group_sum_f64(double v) {
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
 
-// Exact fixed-point sum of the 16 amplitudes a lane holds for one line, as
-// sum_k trunc(a_k 2^20).  Complex int16 samples give a < 2^16, so every term
-// is an integer below 2^36 and a 256-term line sum stays below 2^53: FP64
-// accumulation is exact in any order.  Real float samples are unbounded and
-// take the int64 route.
-template <int FMT>
+// Exact fixed-point sum of the amplitudes a group of G lanes holds for one
+// line, as sum_k trunc(a_k 2^20).  Complex int16 samples have |x| < 46341;
+// their half-sample interpolants exceed that by at most the Lebesgue constant
+// of trigonometric interpolation ((2/pi) ln N + 1.1 < 5.1 for N <= 512), so
+// a < 2^18, every term is an integer below 2^38 and a line sum of up to 512
+// terms stays below 2^53: FP64 accumulation is exact in any order.  Real
+// float samples are unbounded and take the int64 route.
+template <int FMT, int G = 16>
 struct LineSum {
   double d = 0.0;
   long long i = 0;
@@ -652,8 +657,8 @@ struct LineSum {
     else i += fxp20(a);
   }
   __device__ __forceinline__ long long group_total() {
-    if (FMT == XCORR_HIP_CINT16) return (long long)group16_sum_f64(d);
-    return group16_sum_i64(i);
+    if (FMT == XCORR_HIP_CINT16) return (long long)group_sum_f64<G>(d);
+    return group_sum_i64<G>(i);
   }
 };
 
@@ -761,6 +766,161 @@ __global__ void __launch_bounds__(kThreads, 2) k_stripA256(DevParams P, const vo
         if (r < 8 || j == 0) Zu[l * 256 + j + 16 * r] = v[r];
     }
     wave_sync();                                // the next line's loads into mr / tile follow
+  }
+}
+
+// Raw samples x0 + j + G r (r < V) of line gy for a group of G lanes, as
+// load_line256: the address is clamped into the held lines [lo, hi) and the
+// image columns, and samples outside the image are zeroed afterwards.
+template <int G, int V, bool EDGE>
+__device__ __forceinline__ void load_line_g(const unsigned* __restrict__ im, int inx, int iny, int lo, int hi, int gy,
+                                            int x0, int j, unsigned (&u)[V]) {
+  const bool oky = (unsigned)gy < (unsigned)iny;
+  const unsigned* line = im + (ptrdiff_t)min(max(gy, lo), hi - 1) * inx;
+  if (!EDGE) {
+    const unsigned* q = line + x0 + j;
+#pragma unroll
+    for (int r = 0; r < V; r++) {
+      const unsigned w = q[G * r];
+      u[r] = oky ? w : 0u;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < V; r++) {
+      const int gx = x0 + j + G * r;
+      const unsigned w = line[min(max(gx, 0), inx - 1)];
+      u[r] = (oky && (unsigned)gx < (unsigned)inx) ? w : 0u;
+    }
+  }
+}
+
+// Range interpolation of one held line for npx = N in {128, 512} (range_row
+// after the gather): amplitudes ea / oa of output columns 2c, 2c + 1,
+// c = j + G q.
+template <int N, int FMT>
+__device__ __forceinline__ void interp_line(const unsigned (&u)[RegFft<N>::V], int j, float2* tile,
+                                            const float2 (&t)[RegFft<N>::TW], const float2* sph,
+                                            float (&ea)[RegFft<N>::V / 2], float (&oa)[RegFft<N>::V / 2]) {
+  using F = RegFft<N>;
+  constexpr int G = F::G, V = F::V;
+  float2 v[V];
+#pragma unroll
+  for (int r = 0; r < V; r++)
+    v[r] = FMT == XCORR_HIP_CINT16 ? make_float2((float)(short)(u[r] & 0xffffu), (float)(short)(u[r] >> 16))
+                                   : make_float2(__uint_as_float(u[r]), 0.0f);
+#pragma unroll
+  for (int q = 0; q < V / 2; q++) ea[q] = ampl(v[q + V / 4]);
+  F::template run<-1>(v, j, tile, t);
+  const float2* phj = sph + j + opaque_zero();
+#pragma unroll
+  for (int r = 0; r < V; r++) v[r] = cmul(v[r], phj[G * r]);
+  F::template run<+1>(v, j, tile, t);
+  const float sc = 1.0f / (float)N;
+#pragma unroll
+  for (int q = 0; q < V / 2; q++) {
+    float2 o = v[q + V / 4];
+    o.x = o.x * sc;
+    o.y = o.y * sc;
+    oa[q] = ampl(o);
+  }
+}
+
+// Strip lines for npx = npy = N in {128, 512} (the outputs of k_stripA256):
+// block b + blk_off covers kStripBlockLines lines from global line
+// blk[2b + 1] of strip blk[2b], one line per group of G lanes and iteration;
+// the groups never wait for each other.  Zu holds bins 0 .. N/2 of the
+// unmasked rows' spectra (the rest are their conjugate mirrors).
+template <int N, int FMT, bool EDGE>
+__global__ void __launch_bounds__(kThreads) k_stripA(DevParams P, const void* __restrict__ master,
+                                                     const void* __restrict__ slave,
+                                                     const StripDesc* __restrict__ strips,
+                                                     const int* __restrict__ blk, int blk_off,
+                                                     const unsigned char* __restrict__ needM,
+                                                     const unsigned char* __restrict__ needU,
+                                                     float* __restrict__ amp1, float* __restrict__ amp2,
+                                                     long long* __restrict__ lsum, float2* __restrict__ Zm,
+                                                     float2* __restrict__ Zu, const float2* __restrict__ tabs) {
+  using F = RegFft<N>;
+  constexpr int G = F::G, V = F::V, GPB = kThreads / G;
+  __shared__ float2 tiles[GPB * F::TILE];
+  __shared__ float2 mrow[GPB * (N / 2)];           // each group's master amplitude line (column pairs)
+  __shared__ float2 sph[N];                         // range-interpolation phase factors
+  const int g = threadIdx.x / G, j = threadIdx.x % G;
+  const int bb = blockIdx.x + blk_off;
+  const StripDesc S = strips[blk[2 * bb]];
+  float2* tile = tiles + g * F::TILE;
+  float2* mr = mrow + g * (N / 2);
+  float2 t[F::TW];
+  F::load_tw(tabs + P.tw_nx, j, t);
+  for (int k = threadIdx.x; k < N; k += kThreads) sph[k] = tabs[P.phi + k];
+  __syncthreads();
+  const unsigned* im_m = (const unsigned*)master;
+  const unsigned* im_s = (const unsigned*)slave;
+  const int first = blk[2 * bb + 1] - S.line0;                   // strip-relative line of this block
+  const int end = min(first + kStripBlockLines, S.nlines);
+  const int c_lo = P.xs / 2, c_hi = (P.xs + P.nx_corr) / 2;     // template column pairs
+  for (int i = first + g; i < end; i += GPB) {
+    const size_t l = (size_t)(S.line0 + i);
+    const bool wm = needM[l], wu = needU[l];                      // uniform over the group
+    unsigned u[V];
+    float ea[V / 2], oa[V / 2];
+    load_line_g<G, V, EDGE>(im_m, P.m_nx, P.m_ny, P.m_lo, P.m_hi, S.my0 + i, S.mx0, j, u);
+    interp_line<N, FMT>(u, j, tile, t, sph, ea, oa);
+    {
+      LineSum<FMT, G> s;
+      float2* o = (float2*)(amp1 + l * N);
+#pragma unroll
+      for (int q = 0; q < V / 2; q++) {
+        const float2 a = make_float2(ea[q], oa[q]);
+        o[j + G * q] = a;
+        mr[j + G * q] = a;
+        s.add(ea[q]);
+        s.add(oa[q]);
+      }
+      const long long st = s.group_total();
+      if (j == 0) lsum[2 * l] = st;
+    }
+    load_line_g<G, V, EDGE>(im_s, P.s_nx, P.s_ny, P.s_lo, P.s_hi, S.sy0 + i, S.sx0, j, u);
+    interp_line<N, FMT>(u, j, tile, t, sph, ea, oa);
+    {
+      LineSum<FMT, G> s;
+      float2* o = (float2*)(amp2 + l * N);
+#pragma unroll
+      for (int q = 0; q < V / 2; q++) {
+        const float2 a = make_float2(ea[q], oa[q]);
+        const int c = j + G * q;
+        if (wm && c >= c_lo && c < c_hi) o[c] = a;
+        tile[c] = a;                               // the tile doubles as the slave line buffer
+        s.add(ea[q]);
+        s.add(oa[q]);
+      }
+      const long long st = s.group_total();
+      if (j == 0) lsum[2 * l + 1] = st;
+    }
+    wave_sync();                                   // the group's two lines are in LDS
+    const float* m1 = (const float*)mr;
+    const float* a2 = (const float*)tile;
+    float2 v[V];
+    if (wm) {
+#pragma unroll
+      for (int r = 0; r < V; r++) {
+        const int x = j + G * r;
+        v[r] = make_float2(m1[x], (x >= P.xs && x < N - P.xs) ? a2[x] : 0.0f);
+      }
+      wave_sync();                                 // read before the transform reuses the tile
+      F::template run<-1>(v, j, tile, t);
+#pragma unroll
+      for (int r = 0; r < V; r++) Zm[l * N + j + G * r] = v[r];
+    }
+    if (wu) {
+#pragma unroll
+      for (int r = 0; r < V; r++) v[r] = make_float2(m1[j + G * r], 0.0f);
+      F::template run<-1>(v, j, tile, t);
+#pragma unroll
+      for (int r = 0; r < V; r++)
+        if (j + G * r <= N / 2) Zu[l * N + j + G * r] = v[r];
+    }
+    wave_sync();                                   // the next line rewrites mr and the tile
   }
 }
 
@@ -1032,10 +1192,21 @@ struct ColCfg {
   static constexpr int PITCH = TC + 1;
 };
 
+// Row spectra come from Z[b] (rb null) or, for strips, from line rb[b] + y of
+// Zm (masked rows) or Zu (bins 0 .. N/2 of unmasked rows; the others are the
+// conjugates of their mirrors, as the Hermitian completion makes them).
+// Work items w = (patch w / tiles, column tile w % tiles), w < nwork.
+// Workgroups go to the 8 XCDs round-robin in blockIdx order; block i takes
+// item (i mod 8) (grid / 8) + i / 8 (the grid is a multiple of 8), so each XCD
+// runs a contiguous range of items in order: the tiles of a patch, whose
+// column segments share cache lines, and neighbouring patches of a strip,
+// which share lines, meet in one L2.
 template <int N>
 __global__ void __launch_bounds__(kThreads) k_colpass(DevParams P, const float2* __restrict__ Z,
+                                                      const float2* __restrict__ Zu, const int* __restrict__ rb,
                                                       const unsigned long long* __restrict__ sums,
-                                                      float2* __restrict__ D, const float2* __restrict__ tabs) {
+                                                      float2* __restrict__ D, const float2* __restrict__ tabs,
+                                                      int nwork) {
   using F = RegFft<N>;
   using C = ColCfg<N>;
   constexpr int G = F::G, V = F::V, CPG = C::CPG, TC = C::TC, PITCH = C::PITCH;
@@ -1046,23 +1217,43 @@ __global__ void __launch_bounds__(kThreads) k_colpass(DevParams P, const float2*
   __shared__ float2 sx[N];
   float2* const ftiles = T;
   const int tiles = N / (2 * TC);
-  const int p = blockIdx.x % tiles, b = blockIdx.x / tiles;
+  const int wi = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+  if (wi >= nwork) return;
+  const int p = wi % tiles, b = wi / tiles;
   const int g = threadIdx.x / G, j = threadIdx.x % G;
   float2* ftile = ftiles + g * F::TILE;
   float2 t[F::TW];
   F::load_tw(tabs + P.tw_ny, j, t);
   const float m1NX = (float)patch_mean(sums[2 * b], P.N) * (float)N;
   const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
-  const float2* zb = Z + (size_t)b * P.N;
+  const size_t row0 = rb ? (size_t)rb[b] : (size_t)b * N;
   auto kx_of = [&](int i, bool mirror) { return !mirror ? TC * p + i : ((p == 0 && i == 0) ? N / 2 : N - TC * p - i); };
   float2 vT[CPG][V], vM[CPG][V];
+  // Thread (y0, i) loads column i of the tile at rows y0 + LS k: all LK loads
+  // are in flight together, then the demeaned values go through LDS.  Rows of
+  // strips come from Zm (masked) or Zu (bins <= N/2, the others conjugated).
+  static_assert(kThreads % TC == 0 && (N * TC) % kThreads == 0, "column tile");
+  constexpr int LS = kThreads / TC, LK = N * TC / kThreads;
+  const int i0 = threadIdx.x % TC, y0 = threadIdx.x / TC;
   for (int half = 0; half < 2; half++) {
+    const int kx = kx_of(i0, half == 1);
+    const int ku = kx <= N / 2 ? kx : N - kx;
+    const float2 mk = tabs[P.mk_x + kx];
+    float2 zr[LK];
+#pragma unroll
+    for (int k = 0; k < LK; k++) {
+      const int y = y0 + LS * k;
+      const bool ym = y >= P.ys && y < N - P.ys;
+      const bool u = rb && !ym;
+      const float2* base = u ? Zu : Z;
+      zr[k] = base[(row0 + y) * N + (u ? ku : kx)];
+      if (u && kx > N / 2) zr[k].y = -zr[k].y;
+    }
     __syncthreads();
-    for (int e = threadIdx.x; e < N * TC; e += kThreads) {
-      const int y = e / TC, i = e - (e / TC) * TC;
-      const int kx = kx_of(i, half == 1);
-      T[y * PITCH + i] = demean_row_bin(zb[(size_t)y * N + kx], kx, y >= P.ys && y < N - P.ys, m1NX, m2f,
-                                        tabs[P.mk_x + kx]);
+#pragma unroll
+    for (int k = 0; k < LK; k++) {
+      const int y = y0 + LS * k;
+      T[y * PITCH + i0] = demean_row_bin(zr[k], kx, y >= P.ys && y < N - P.ys, m1NX, m2f, mk);
     }
     __syncthreads();
 #pragma unroll
@@ -2152,6 +2343,7 @@ struct xcorr_hip_ctx {
   float* amp2s = nullptr;
   long long* lsum = nullptr;
   float2* Zu = nullptr;                   // unmasked row spectra (masked ones in Z)
+  size_t z_elems = 0;                     // capacity of Z (float2)
   unsigned char* need = nullptr;          // needM [max_lines] then needU [max_lines]
   int* d_plan = nullptr;                  // rb | perm | blk_strip | strips, per batch
   void* h_stage[2] = {nullptr, nullptr};  // pinned staging of the plan, two batches in flight
@@ -2263,6 +2455,7 @@ static void free_scratch(xcorr_hip_ctx* c) {
   c->surfd = nullptr; c->peak = nullptr; c->partial = nullptr; c->tpeak = nullptr; c->tcave = nullptr;
   c->d_win = nullptr; c->d_out = nullptr; c->tabs = nullptr; c->amp1s = nullptr; c->amp2s = nullptr;
   c->lsum = nullptr; c->Zu = nullptr; c->need = nullptr; c->d_plan = nullptr; c->tcaa = nullptr;
+  c->z_elems = 0;
 }
 
 // Frequency-mode square 128 / 512 windows with ri 2 and the default radix
@@ -2280,6 +2473,27 @@ static int groups4_for(const xcorr_hip_ctx* c) {
   const DevParams& d = c->dp;
   if ((d.NX == 256 && same_plan(d.plx, plan_for(256))) || reg_shape(c)) return 1;
   return d.nyc / d.G4;
+}
+
+// Strip contexts (square 128 / 256 / 512 windows, ri 2, frequency mode) run
+// the strips with the default radix plans; a caller-set plan runs the
+// per-patch kernels, whose buffers are allocated when such a plan is first set.
+static bool use_strips(const xcorr_hip_ctx* c) {
+  const DevParams& d = c->dp;
+  return c->strips && same_plan(d.plx, plan_for(d.NX)) && same_plan(d.ply, plan_for(d.NY));
+}
+static int ensure_patch_buffers(xcorr_hip_ctx* c) {
+  const size_t B = (size_t)c->max_batch, N = (size_t)c->dp.N;
+  if (!c->amp && hipMalloc((void**)&c->amp, B * 2 * N * sizeof(float)) != hipSuccess) return XCORR_HIP_ENOMEM;
+  if (c->z_elems < B * N) {
+    float2* z = nullptr;
+    if (hipMalloc((void**)&z, B * N * sizeof(float2)) != hipSuccess) return XCORR_HIP_ENOMEM;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return XCORR_HIP_EDEVICE;
+    (void)hipFree(c->Z);
+    c->Z = z;
+    c->z_elems = B * N;
+  }
+  return 0;
 }
 
 // Plan layout (ints): rb[max_batch] | perm[max_batch] | blk[2 max_lines / 64] | StripDesc[max_batch]
@@ -2392,7 +2606,8 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   auto alloc = [&](void** ptr, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(ptr, bytes ? bytes : 16);
   };
-  c->strips = d.NX == 256 && d.NY == 256 && d.ri == 2 && p->corr_flag == XCORR_HIP_FREQ && phi_sym;
+  c->strips = d.NX == d.NY && (d.NX == 128 || d.NX == 256 || d.NX == 512) && d.ri == 2 &&
+              p->corr_flag == XCORR_HIP_FREQ && phi_sym;
   if (c->strips) {
     // half the lines of unshared windows: overlapping windows need far fewer
     c->max_lines = std::max(max_batch * (d.NY / 2), d.NY + kStripBlockLines);
@@ -2414,7 +2629,8 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   }
   alloc((void**)&c->sums, B * 2 * sizeof(unsigned long long));
   if (p->corr_flag == XCORR_HIP_FREQ) {
-    alloc((void**)&c->Z, (c->strips ? (size_t)c->max_lines * d.NX : B * N) * sizeof(float2));
+    c->z_elems = c->strips ? (size_t)c->max_lines * d.NX : B * N;
+    alloc((void**)&c->Z, c->z_elems * sizeof(float2));
     alloc((void**)&c->D, B * d.nyc * d.NX * sizeof(float2));
     alloc((void**)&c->surf, B * ns * sizeof(float));
     // room for the generic row path's groups too (a tuned radix plan can select it for npx = 256)
@@ -2565,7 +2781,7 @@ extern "C" int xcorr_hip_correlate(xcorr_hip_ctx* c, const xcorr_hip_window* win
   const bool timing = (flags & XCORR_HIP_TIMING) != 0;
   if (timing)
     for (auto& t : c->timers) { t.ms = 0.0f; t.launches = 0; }
-  if (c->strips) {
+  if (use_strips(c)) {
     const xcorr_hip_window* hw = win;
     if (dev_win && n > 0) {     // the plan is built on the host
       c->hwin.resize(n);
@@ -2694,6 +2910,15 @@ static size_t strip_plan_bytes(const xcorr_hip_ctx* c, const StripBatch& sb) {
          (size_t)sb.ns * sizeof(StripDesc);
 }
 
+// k_colpass<N> over nb patches (rows of Z per patch, or strip lines from the
+// row bases rb) on stream st.
+template <int N>
+static void launch_colpass(xcorr_hip_ctx* c, const DevParams& d, hipStream_t st, const int* rb, int nb) {
+  const int nwork = nb * (N / (2 * ColCfg<N>::TC));
+  hipLaunchKernelGGL(k_colpass<N>, dim3((nwork + 7) & ~7), dim3(kThreads), 0, st, d, (const float2*)c->Z,
+                     (const float2*)c->Zu, rb, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs, nwork);
+}
+
 // Kernels of one strip batch whose plan is in device memory at d_plan, on
 // stream st with images (master, slave) and image dims in d.  Results go to
 // out[perm[b]] (dev_out) or to c->d_out[b].
@@ -2713,10 +2938,21 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
     hipLaunchKernelGGL(k_need_mark, dim3(nb), dim3(256), 0, st, d, d_rb, needM, needU, nb);
   });
   if (rc) return rc;
-  rc = launch_timed(c, timing, "k_stripA256", [&] {
-    const bool ci = d.format == XCORR_HIP_CINT16;
-    auto kin = ci ? k_stripA256<XCORR_HIP_CINT16, false> : k_stripA256<XCORR_HIP_FLOAT32, false>;
-    auto kedge = ci ? k_stripA256<XCORR_HIP_CINT16, true> : k_stripA256<XCORR_HIP_FLOAT32, true>;
+  const bool ci = d.format == XCORR_HIP_CINT16;
+  rc = launch_timed(c, timing, d.NX == 256 ? "k_stripA256" : "k_stripA", [&] {
+    using KA = void (*)(DevParams, const void*, const void*, const StripDesc*, const int*, int, const unsigned char*,
+                        const unsigned char*, float*, float*, long long*, float2*, float2*, const float2*);
+    KA kin, kedge;
+    if (d.NX == 256) {
+      kin = ci ? k_stripA256<XCORR_HIP_CINT16, false> : k_stripA256<XCORR_HIP_FLOAT32, false>;
+      kedge = ci ? k_stripA256<XCORR_HIP_CINT16, true> : k_stripA256<XCORR_HIP_FLOAT32, true>;
+    } else if (d.NX == 128) {
+      kin = ci ? k_stripA<128, XCORR_HIP_CINT16, false> : k_stripA<128, XCORR_HIP_FLOAT32, false>;
+      kedge = ci ? k_stripA<128, XCORR_HIP_CINT16, true> : k_stripA<128, XCORR_HIP_FLOAT32, true>;
+    } else {
+      kin = ci ? k_stripA<512, XCORR_HIP_CINT16, false> : k_stripA<512, XCORR_HIP_FLOAT32, false>;
+      kedge = ci ? k_stripA<512, XCORR_HIP_CINT16, true> : k_stripA<512, XCORR_HIP_FLOAT32, true>;
+    }
     if (nblk_in > 0)
       hipLaunchKernelGGL(kin, dim3(nblk_in), dim3(kThreads), 0, st, d, master, slave, d_st, d_bs, 0,
                          (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s, c->amp2s, c->lsum, c->Z,
@@ -2732,14 +2968,26 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
                        c->sums, nb);
   });
   if (rc) return rc;
-  rc = launch_timed(c, timing, "k_colpass256", [&] {
-    hipLaunchKernelGGL(k_colpass256, dim3(nb * (d.NX / 32)), dim3(kThreads), 0, st, d, (const float2*)c->Z,
-                       (const float2*)c->Zu, d_rb, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs);
+  rc = launch_timed(c, timing, d.NX == 256 ? "k_colpass256" : "k_colpass", [&] {
+    if (d.NX == 256)
+      hipLaunchKernelGGL(k_colpass256, dim3(nb * (d.NX / 32)), dim3(kThreads), 0, st, d, (const float2*)c->Z,
+                         (const float2*)c->Zu, d_rb, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs);
+    else if (d.NX == 128)
+      launch_colpass<128>(c, d, st, d_rb, nb);
+    else
+      launch_colpass<512>(c, d, st, d_rb, nb);
   });
   if (rc) return rc;
-  rc = launch_timed(c, timing, "k_rowifft_surf256", [&] {
-    hipLaunchKernelGGL(k_rowifft_surf256, dim3(nb), dim3(kThreads), 0, st, d, (const float2*)c->D, c->surf, c->peak,
-                       c->partial, (const float2*)c->tabs);
+  rc = launch_timed(c, timing, d.NX == 256 ? "k_rowifft_surf256" : "k_rowifft_surf_reg", [&] {
+    if (d.NX == 256)
+      hipLaunchKernelGGL(k_rowifft_surf256, dim3(nb), dim3(kThreads), 0, st, d, (const float2*)c->D, c->surf,
+                         c->peak, c->partial, (const float2*)c->tabs);
+    else if (d.NX == 128)
+      hipLaunchKernelGGL(k_rowifft_surf_reg<128>, dim3(nb), dim3(kThreads), 0, st, d, (const float2*)c->D, c->surf,
+                         c->peak, c->partial, (const float2*)c->tabs);
+    else
+      hipLaunchKernelGGL(k_rowifft_surf_reg<512>, dim3(nb), dim3(kThreads), 0, st, d, (const float2*)c->D, c->surf,
+                         c->peak, c->partial, (const float2*)c->tabs);
   });
   if (rc) return rc;
   const size_t sh5 = d.interp_flag ? ((size_t)d.Lx * 16 + (size_t)d.Ly * 8) * sizeof(float2) : 16;
@@ -2856,11 +3104,9 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
     if (regA) {
       rc = launch_timed(c, timing, "k_colpass", [&] {
         if (d.NX == 128)
-          hipLaunchKernelGGL(k_colpass<128>, dim3(nb * (128 / (2 * ColCfg<128>::TC))), dim3(kThreads), 0, c->stream,
-                             d, (const float2*)c->Z, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs);
+          launch_colpass<128>(c, d, c->stream, nullptr, nb);
         else
-          hipLaunchKernelGGL(k_colpass<512>, dim3(nb * (512 / (2 * ColCfg<512>::TC))), dim3(kThreads), 0, c->stream,
-                             d, (const float2*)c->Z, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs);
+          launch_colpass<512>(c, d, c->stream, nullptr, nb);
       });
     } else if (col256 && d.NX >= 32) {
       rc = launch_timed(c, timing, "k_colpass256", [&] {
@@ -3166,8 +3412,8 @@ extern "C" int xcorr_hip_set_tuning(xcorr_hip_ctx* c, const xcorr_hip_tuning* t)
     return fail(c, XCORR_HIP_EINVAL, "set_tuning: a radix plan (2/4/8/16, up to 4 passes) must multiply to its length");
   if (d.NX == d.NY && !same_plan(px, py))
     return fail(c, XCORR_HIP_EINVAL, "set_tuning: transforms of equal length share one radix plan");
-  if (c->strips && (!same_plan(px, plan_for(d.NX)) || !same_plan(py, plan_for(d.NY))))
-    return fail(c, XCORR_HIP_EINVAL, "set_tuning: the 256-point strip path runs the default radix-16 plan only");
+  if (c->strips && (!same_plan(px, plan_for(d.NX)) || !same_plan(py, plan_for(d.NY))) && ensure_patch_buffers(c))
+    return fail(c, XCORR_HIP_ENOMEM, "set_tuning: per-patch buffers for a non-default radix plan");
   c->tune_batch = t->batch;
   c->nt_rows = t->threads;
   d.G1 = G;
@@ -3265,7 +3511,7 @@ extern "C" int xcorr_hip_autotune(xcorr_hip_ctx* c, const xcorr_hip_window* win,
     return 0;
   };
   // coordinate descent: threads, row tile, column tile, radix plans, batch
-  const bool generic = !c->strips;
+  const bool generic = !c->strips;         // strip contexts: batch only (the default plans run the strips)
   if (!rc && generic)
     for (int th : {128, 256, 512}) {
       xcorr_hip_tuning t = best;
@@ -3441,7 +3687,7 @@ static int multi_build(xcorr_hip_multi* m, unsigned flags) {
     for (int i = 0; i < np; i++) {
       if (m->pair_class[i] != k || m->pairs[i].n == 0) continue;
       const xcorr_hip_pair& q = m->pairs[i];
-      if (c->strips) {
+      if (use_strips(c)) {
         bind_pair(c, q);
         std::vector<int> ord;
         strip_order(q.win, q.n, ord);
@@ -3469,10 +3715,10 @@ static int multi_build(xcorr_hip_multi* m, unsigned flags) {
         }
       }
     }
-    const size_t bytes = c->strips ? host.size() * sizeof(int) : hw.size() * sizeof(xcorr_hip_window);
+    const size_t bytes = use_strips(c) ? host.size() * sizeof(int) : hw.size() * sizeof(xcorr_hip_window);
     if (bytes) {
       if (hipMalloc(&m->dbuf[k], bytes) != hipSuccess) return mfail(m, XCORR_HIP_ENOMEM, "multi: plan allocation failed");
-      MHIPCK(m, hipMemcpy(m->dbuf[k], c->strips ? (const void*)host.data() : (const void*)hw.data(), bytes,
+      MHIPCK(m, hipMemcpy(m->dbuf[k], use_strips(c) ? (const void*)host.data() : (const void*)hw.data(), bytes,
                           hipMemcpyHostToDevice));
     }
   }
@@ -3495,7 +3741,7 @@ static int multi_build(xcorr_hip_multi* m, unsigned flags) {
       if (e != hipSuccess || rc) break;
       const xcorr_hip_pair& q = m->pairs[b.pair];
       bind_pair(c, q);
-      if (c->strips)
+      if (use_strips(c))
         rc = launch_strip_batch(c, c->dp, q.master, q.slave, c->stream, (const int*)m->dbuf[k] + b.off, b.sb, q.out,
                                 true, false);
       else
@@ -3614,7 +3860,7 @@ extern "C" long xcorr_hip_dump(xcorr_hip_ctx* c, int what, int slot, void* host,
   switch (what) {
     case 0:
     case 1:
-      if (c->strips)   // npy consecutive strip lines from the row base
+      if (!c->last_rb.empty())   // npy consecutive strip lines from the row base
         src = (what ? c->amp2s : c->amp1s) + (size_t)c->last_rb[slot] * d.NX;
       else
         src = c->amp + ((size_t)slot * 2 + what) * d.N;

@@ -134,15 +134,16 @@ def test_set_tuning_rejects_invalid(xg, pairs):
                    dict(radix_x=[8, 16], radix_y=[16, 8]), dict(radix_x=[32, 4], radix_y=[32, 4])):
             with pytest.raises(xg.XcorrError):
                 eng.set_tuning(**kw)
-    with xg.Engine(xg.make_params(64, 64), max_batch=8) as eng:     # strip path: default plan only
-        with pytest.raises(xg.XcorrError):
-            eng.set_tuning(radix_x=[4, 4, 16], radix_y=[4, 4, 16])
+    with xg.Engine(xg.make_params(64, 64), max_batch=8) as eng:     # strip shape: any valid plan, any batch
+        eng.set_tuning(radix_x=[4, 4, 16], radix_y=[4, 4, 16])
         eng.set_tuning(batch=4)
+        with pytest.raises(xg.XcorrError):
+            eng.set_tuning(batch=9)
 
 
 def test_autotune_keeps_results(xg, pairs):
     m, s = pairs["b"]
-    p = xg.make_params(32, 32, interp_factor=8)
+    p = xg.make_params(16, 32, interp_factor=8)      # not a strip shape: every launch setting is tuned
     w = _windows(xg, p, 1024, 1280, 10, 8)
     with xg.Engine(p, max_batch=64) as eng:
         eng.set_images(m, s)
@@ -152,9 +153,18 @@ def test_autotune_keeps_results(xg, pairs):
         print("autotune:", d)
         assert d["candidates"] >= 5 and d["ms_per_batch"] > 0
         assert 1 <= d["batch"] <= 64 and d["threads"] % 64 == 0
-        assert d["radix_x"] == [16, 8] and d["radix_y"] == [16, 8]
+        assert d["radix_x"] == [8, 8] and d["radix_y"] == [16, 8]
         got = eng.correlate(w)
     _same(got, base, "autotuned")
+    p = xg.make_params(32, 32, interp_factor=8)      # strip shape: the batch is tuned
+    w = _windows(xg, p, 1024, 1280, 10, 8)
+    with xg.Engine(p, max_batch=64) as eng:
+        eng.set_images(m, s)
+        base = eng.correlate(w)
+        d = eng.autotune(w, radix=True).as_dict()
+        assert d["radix_x"] == [16, 8] and d["radix_y"] == [16, 8] and d["candidates"] >= 1
+        got = eng.correlate(w)
+    _same(got, base, "autotuned strips")
 
 
 def _multi_workload(xg, torch, pairs):

@@ -1,11 +1,13 @@
-"""GPU parity of the strip path (-xsearch 64 -ysearch 64, ri 2, frequency mode).
+"""GPU parity of the strip path (square windows of 128, 256 and 512 samples:
+-xsearch = -ysearch = 32, 64, 128; ri 2, frequency mode).
 
 Windows that share the master column, the slave column and the slave line
 offset are computed from shared strip lines.  These cases exercise the
 planner: overlapping, touching, duplicate and isolated windows in one column,
 mixed slave offsets, unsorted input, windows across the image edges, batches
 that split a strip, and the three pointer modes.  Every result must equal the
-FP32 CPU restatement bit for bit, in input order.
+FP32 CPU restatement bit for bit, in input order.  A caller-set radix plan on
+a strip-shaped context runs the per-patch kernels, also bit-exact.
 """
 import numpy as np
 import pytest
@@ -15,15 +17,23 @@ from test_gpu_parity import _assert_bit_exact, _locs, _oracle_params
 
 pytestmark = pytest.mark.gpu
 
+SIZES = [32, 64, 128]        # xsearch = ysearch; windows 4 xsearch square
+
 
 @pytest.fixture(scope="module")
-def pair(synth):
+def pairs(synth):
     if not gpu_present():
         pytest.skip("no GPU")
-    return synth.constant_shift_pair(768, 1536, -2.63, 5.41, 0.8)
+    out = {}
+    for xs in SIZES:
+        k = xs / 64.0
+        out[xs] = synth.constant_shift_pair(int(768 * k), int(1536 * k), -2.63 * k, 5.41 * k, 0.8)
+    return out
 
 
-def _strip_windows(xg):
+def _strip_windows(xg, xs):
+    """The 256-window layout scaled to windows of 4 xs samples."""
+    k = xs / 64.0
     rng = np.random.default_rng(7)
     rows = []
     # column A: dense overlapping run, a duplicate, a touching window, a gap
@@ -38,28 +48,49 @@ def _strip_windows(xg):
     # edges: left/right/top/bottom overhang, fully outside
     rows += [(-60, 20, -57, 18), (600, 1400, 603, 1395), (-100, -200, -98, -190), (700, -30, 705, -25),
              (-300, 500, -297, 495), (900, 900, 905, 900)]
-    w = np.array(rows, np.int32)
+    w = np.rint(np.array(rows, np.float64) * k).astype(np.int32)
     perm = rng.permutation(len(w))                 # unsorted input order
     return np.ascontiguousarray(w[perm]).view(xg.WINDOW_DTYPE).reshape(-1)
 
 
-def test_strip_planner_bit_exact(xg, orc, pair):
-    m, s = pair
-    p = xg.make_params(64, 64)
-    w = _strip_windows(xg)
+@pytest.mark.parametrize("xs", SIZES)
+def test_strip_planner_bit_exact(xg, orc, pairs, xs):
+    m, s = pairs[xs]
+    p = xg.make_params(xs, xs)
+    w = _strip_windows(xg, xs)
     ref = orc.run(_oracle_params(orc, p), m, s, _locs(w), precision=32)
     for mb in (7, 64):                              # 7: strips split across batches
         with xg.Engine(p, max_batch=mb) as eng:
             eng.set_images(m, s)
             got = eng.correlate(w)
-        _assert_bit_exact(got, ref, "max_batch %d" % mb)
+        _assert_bit_exact(got, ref, "xs %d max_batch %d" % (xs, mb))
 
 
-def test_strip_pointer_modes(xg, pair):
+@pytest.mark.parametrize("xs", SIZES)
+def test_strip_shape_with_radix_plan_runs_per_patch(xg, orc, pairs, xs):
+    m, s = pairs[xs]
+    p = xg.make_params(xs, xs)
+    w = _strip_windows(xg, xs)
+    plan = {32: [8, 16], 64: [4, 4, 16], 128: [16, 16, 2]}[xs]
+    with xg.Engine(p, max_batch=16) as eng:
+        eng.set_images(m, s)
+        eng.set_tuning(radix_x=plan, radix_y=plan)
+        got = eng.correlate(w)
+        default = {32: [16, 8], 64: [16, 16], 128: [8, 8, 8]}[xs]
+        eng.set_tuning(radix_x=default, radix_y=default)   # back to the default plans: strips again
+        again = eng.correlate(w)
+    with orc.fft_plans({4 * xs: plan}):
+        ref = orc.run(_oracle_params(orc, p), m, s, _locs(w), precision=32)
+    _assert_bit_exact(got, ref, "xs %d radix %s" % (xs, plan))
+    _assert_bit_exact(again, orc.run(_oracle_params(orc, p), m, s, _locs(w), precision=32), "xs %d default" % xs)
+
+
+@pytest.mark.parametrize("xs", SIZES)
+def test_strip_pointer_modes(xg, pairs, xs):
     import torch
-    m, s = pair
-    p = xg.make_params(64, 64)
-    w = _strip_windows(xg)
+    m, s = pairs[xs]
+    p = xg.make_params(xs, xs)
+    w = _strip_windows(xg, xs)
     dm, ds = torch.from_numpy(m).cuda(), torch.from_numpy(s).cuda()
     dw = torch.from_numpy(w.view(np.int32).reshape(-1, 4).copy()).cuda()
     nbytes = len(w) * xg.RESULT_DTYPE.itemsize
@@ -76,10 +107,13 @@ def test_strip_pointer_modes(xg, pair):
         _assert_bit_exact(got, host, label)
 
 
-def test_strip_dumps_in_shared_strip(xg, orc, pair):
-    m, s = pair
-    p = xg.make_params(64, 64)
-    w = np.array([(300, my0, 297, my0 + 3) for my0 in (400, 450, 520, 560)], np.int32).view(
+@pytest.mark.parametrize("xs", SIZES)
+def test_strip_dumps_in_shared_strip(xg, orc, pairs, xs):
+    m, s = pairs[xs]
+    n = 4 * xs
+    k = xs / 64.0
+    p = xg.make_params(xs, xs)
+    w = np.rint(np.array([(300, my0, 297, my0 + 3) for my0 in (400, 450, 520, 560)]) * k).astype(np.int32).view(
         xg.WINDOW_DTYPE).reshape(-1)
     with xg.Engine(p) as eng:
         eng.set_images(m, s)
@@ -87,11 +121,11 @@ def test_strip_dumps_in_shared_strip(xg, orc, pair):
         for slot in (1, 2):
             amp1, amp2 = eng.dump(0, slot), eng.dump(1, slot)
             means, surf = eng.dump(3, slot), eng.dump(2, slot)
-            mw = m[w["my0"][slot]:w["my0"][slot] + 256, w["mx0"][slot]:w["mx0"][slot] + 256].astype(np.float32)
-            sw = s[w["sy0"][slot]:w["sy0"][slot] + 256, w["sx0"][slot]:w["sx0"][slot] + 256].astype(np.float32)
+            mw = m[w["my0"][slot]:w["my0"][slot] + n, w["mx0"][slot]:w["mx0"][slot] + n].astype(np.float32)
+            sw = s[w["sy0"][slot]:w["sy0"][slot] + n, w["sx0"][slot]:w["sx0"][slot] + n].astype(np.float32)
             _, d = orc.patch(_oracle_params(orc, p), mw[..., 0] + 1j * mw[..., 1], sw[..., 0] + 1j * sw[..., 1],
                              precision=32, dump=True)
-            t = (slice(64, 192), slice(64, 192))
+            t = (slice(n // 4, 3 * n // 4), slice(n // 4, 3 * n // 4))
             assert np.array_equal(amp1.view(np.uint32), d["amp1"].view(np.uint32)), "amp master, slot %d" % slot
             assert np.array_equal(amp2[t].view(np.uint32), d["amp2"][t].view(np.uint32)), "amp slave, slot %d" % slot
             assert np.array_equal(means, d["mean"]), "means, slot %d" % slot
