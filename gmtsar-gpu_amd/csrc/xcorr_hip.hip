@@ -1982,34 +1982,41 @@ typedef int v4i_t __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const float* __restrict__ amp,
                                                             const unsigned long long* __restrict__ sums,
                                                             double* __restrict__ aa) {
-  __shared__ int Ai[128 * 128];
   __shared__ long long E[127 * 64];
   const int b = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float m1f = (float)patch_mean(sums[2 * b], P.N);
   const float* a1 = amp + (size_t)(2 * b) * P.N;
-  for (int e = threadIdx.x; e < 128 * 128; e += kThreads) Ai[e] = (int)(a1[e] - m1f);
-  __syncthreads();
-  if (threadIdx.x < 127) {                        // row energies E[r][dx] = sum_{x<64} A[r][x + dx]^2
-    const int* row = Ai + threadIdx.x * 128;
-    long long s = 0;
-    for (int x = 0; x < 64; x++) s += (long long)row[x] * row[x];
-    E[threadIdx.x * 64] = s;
-    for (int dx = 1; dx < 64; dx++) {
-      s += (long long)row[dx + 63] * row[dx + 63] - (long long)row[dx - 1] * row[dx - 1];
-      E[threadIdx.x * 64 + dx] = s;
+  // row energies E[r][dx] = sum_{x < 64} A[r][x + dx]^2, one row per wave and
+  // step: lane l holds A[r][l] and A[r][64 + l], and
+  // E[r][l] = (sum_{x >= l} A[r][x]^2) + (sum_{x < l} A[r][64 + x]^2),
+  // a suffix and an exclusive prefix sum over the wave (exact in int64)
+  for (int r = wave; r < 127; r += kThreads / 64) {
+    const long long v0 = (int)(a1[r * 128 + lane] - m1f);
+    const long long v1 = (int)(a1[r * 128 + 64 + lane] - m1f);
+    const long long q0 = v0 * v0, q1 = v1 * v1;
+    long long p0 = q0, p1 = q1;                   // inclusive prefix sums over the lanes
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long t0 = __shfl_up(p0, o), t1 = __shfl_up(p1, o);
+      if (lane >= o) {
+        p0 += t0;
+        p1 += t1;
+      }
     }
+    const long long tot0 = __shfl(p0, 63);
+    E[r * 64 + lane] = (tot0 - p0 + q0) + (p1 - q1);
   }
   __syncthreads();
-  if (threadIdx.x < 64) {                         // column sums over 64 rows, slid down the lags
-    const int dx = threadIdx.x;
-    long long s = 0;
-    for (int y = 0; y < 64; y++) s += E[y * 64 + dx];
-    double* ob = aa + (size_t)b * 64 * 64;
-    ob[dx] = (double)s;
-    for (int dy = 1; dy < 64; dy++) {
-      s += E[(dy + 63) * 64 + dx] - E[(dy - 1) * 64 + dx];
-      ob[dy * 64 + dx] = (double)s;
-    }
+  // aa[dy][dx] = sum_{y < 64} E[dy + y][dx]: thread (dx, d0) slides down lags d0 .. d0 + 15
+  const int dx = lane, d0 = 16 * wave;
+  long long s = 0;
+  for (int y = 0; y < 64; y++) s += E[(d0 + y) * 64 + dx];
+  double* ob = aa + (size_t)b * 64 * 64;
+  ob[d0 * 64 + dx] = (double)s;
+  for (int dy = d0 + 1; dy < d0 + 16; dy++) {
+    s += E[(dy + 63) * 64 + dx] - E[(dy - 1) * 64 + dx];
+    ob[dy * 64 + dx] = (double)s;
   }
 }
 
@@ -2022,28 +2029,57 @@ __device__ __forceinline__ void limbs7(int v, signed char* l) {
   l[2] = (signed char)((v1 - l1) >> 7);
 }
 
-// 16 bytes of a limb row starting at byte offset o (any alignment): two
-// aligned 16-byte LDS loads and a byte funnel shift.
+// 16 bytes of a limb row starting at byte offset o (any alignment): the five
+// dwords that cover them and one byte-align per output dword
+// (v_alignbyte_b32: bytes sh .. sh + 3 of the pair {hi, lo}).
 __device__ __forceinline__ v4i_t ld16_unaligned(const signed char* row, int o) {
-  const int base = o & ~15, sh = o & 15;
-  const uint4 lo = *(const uint4*)(row + base);
-  const uint4 hi = *(const uint4*)(row + base + 16);
-  const unsigned w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-  const int q = sh >> 2, bsh = (sh & 3) * 8;
+  const unsigned* wp = (const unsigned*)(row + (o & ~3));
+  const unsigned sh = (unsigned)(o & 3);
+  unsigned w[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) w[k] = wp[k];
   v4i_t r;
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    // select words q + k and q + k + 1 (q is 0..3) without dynamic register indexing
-    unsigned a = w[k], c = w[k + 1];
-    if (q == 1) { a = w[k + 1]; c = w[k + 2]; }
-    if (q == 2) { a = w[k + 2]; c = w[k + 3]; }
-    if (q == 3) { a = w[k + 3]; c = w[k + 4]; }
-    r[k] = (int)(bsh ? ((a >> bsh) | (c << (32 - bsh))) : a);
-  }
+  for (int k = 0; k < 4; k++) r[k] = (int)__builtin_amdgcn_alignbyte(w[k + 1], w[k], sh);
   return r;
 }
 
 constexpr int kLimbPitch = 144;                   // bytes per limb row of the window (128 + 16 for the tail loads)
+
+// The wave's 2 x 2 tiles of 16 x 16 lags (dx0 + 16 i + .., dy0 + 16 j + ..):
+// for every master row r that meets them, the Toeplitz rows A[r][x + dx]
+// times the template rows, one 16 x 16 x 64 int8 product per limb pair
+// (ka < KA, kb < KB), accumulated in acc[i][j][ka + kb].
+template <int KA, int KB>
+__device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed char* Bl, int dx0, int dy0, int m,
+                                             int g, v4i_t (&acc)[2][2][5]) {
+  const v4i_t zero = v4i_t{0, 0, 0, 0};
+  for (int r = dy0; r < dy0 + 32 + 63; r++) {     // master rows that meet the wave's lag rows
+    v4i_t A[2][KA], B[2][KB];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int k = 0; k < KA; k++)                // Toeplitz rows: A[r][x + dx], dx = dx0 + 16 i + m, x = 16 g ..
+        A[i][k] = ld16_unaligned(Al + (k * 128 + r) * kLimbPitch, dx0 + 16 * i + m + 16 * g);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int y = r - (dy0 + 16 * j + m);       // template row of lag row dy = dy0 + 16 j + m
+      const bool ok = y >= 0 && y < 64;
+#pragma unroll
+      for (int k = 0; k < KB; k++)
+        B[j][k] = ok ? *(const v4i_t*)(Bl + (k * 64 + (ok ? y : 0)) * 64 + 16 * g) : zero;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int ka = 0; ka < KA; ka++)
+#pragma unroll
+          for (int kb = 0; kb < KB; kb++)
+            acc[i][j][ka + kb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[i][ka], B[j][kb], acc[i][j][ka + kb], 0, 0, 0);
+  }
+}
 
 __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, const float* __restrict__ amp,
                                                              const unsigned long long* __restrict__ sums,
@@ -2053,15 +2089,19 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
   __shared__ __attribute__((aligned(16))) signed char Al[3 * 128 * kLimbPitch];
   __shared__ __attribute__((aligned(16))) signed char Bl[3 * 64 * 64];
   __shared__ long long redl[kThreads / 64];
-  __shared__ int sbig;
+  __shared__ int sbig, shi[2];                    // values beyond the limbs; limb 2 nonzero (master, template)
   const int b = blockIdx.x;
   const float m1f = (float)patch_mean(sums[2 * b], P.N);
   const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
   const float* a1 = amp + (size_t)(2 * b) * P.N;
   const float* a2 = a1 + P.N;
-  if (threadIdx.x == 0) sbig = 0;
+  if (threadIdx.x == 0) {
+    sbig = 0;
+    shi[0] = 0;
+    shi[1] = 0;
+  }
   __syncthreads();
-  int big = 0;
+  int big = 0, hia = 0, hib = 0;
   for (int e = threadIdx.x; e < 128 * kLimbPitch; e += kThreads) {
     const int r = e / kLimbPitch, x = e - r * kLimbPitch;
     int v = 0;
@@ -2069,6 +2109,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
     big |= (v >= (1 << 20) || v <= -(1 << 20));
     signed char l[3];
     limbs7(v, l);
+    hia |= l[2];
 #pragma unroll
     for (int k = 0; k < 3; k++) Al[(k * 128 + r) * kLimbPitch + x] = l[k];
   }
@@ -2080,10 +2121,13 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
     b2 += (long long)v * v;
     signed char l[3];
     limbs7(v, l);
+    hib |= l[2];
 #pragma unroll
     for (int k = 0; k < 3; k++) Bl[(k * 64 + y) * 64 + x] = l[k];
   }
   if (big) atomicOr(&sbig, 1);
+  if (hia) atomicOr(&shi[0], 1);
+  if (hib) atomicOr(&shi[1], 1);
   const double d2d = (double)block_sum_i64(b2, redl);     // (its barriers also publish the limbs)
   double* sd = surfd + (size_t)b * 64 * 64;
   const double* ab_aa = aa + (size_t)b * 64 * 64;
@@ -2115,32 +2159,12 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
     for (int j = 0; j < 2; j++)
 #pragma unroll
       for (int w = 0; w < 5; w++) acc[i][j][w] = v4i_t{0, 0, 0, 0};
-  const v4i_t zero = v4i_t{0, 0, 0, 0};
-  for (int r = dy0; r < dy0 + 32 + 63; r++) {     // master rows that meet the wave's lag rows
-    v4i_t A[2][3], B[2][3];
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int k = 0; k < 3; k++)                 // Toeplitz rows: A[r][x + dx], dx = dx0 + 16 i + m, x = 16 g ..
-        A[i][k] = ld16_unaligned(Al + (k * 128 + r) * kLimbPitch, dx0 + 16 * i + m + 16 * g);
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int y = r - (dy0 + 16 * j + m);       // template row of lag row dy = dy0 + 16 j + m
-      const bool ok = y >= 0 && y < 64;
-#pragma unroll
-      for (int k = 0; k < 3; k++)
-        B[j][k] = ok ? *(const v4i_t*)(Bl + (k * 64 + (ok ? y : 0)) * 64 + 16 * g) : zero;
-    }
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 2; j++)
-#pragma unroll
-        for (int ka = 0; ka < 3; ka++)
-#pragma unroll
-          for (int kb = 0; kb < 3; kb++)
-            acc[i][j][ka + kb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[i][ka], B[j][kb], acc[i][j][ka + kb], 0, 0, 0);
-  }
+  // limb 2 is zero throughout a window whose values all lie in [-8256, 8127]:
+  // its products are skipped (9, 6 or 4 limb products per term instead of 9)
+  if (shi[0] && shi[1]) i8_lag_tiles<3, 3>(Al, Bl, dx0, dy0, m, g, acc);
+  else if (shi[0]) i8_lag_tiles<3, 2>(Al, Bl, dx0, dy0, m, g, acc);
+  else if (shi[1]) i8_lag_tiles<2, 3>(Al, Bl, dx0, dy0, m, g, acc);
+  else i8_lag_tiles<2, 2>(Al, Bl, dx0, dy0, m, g, acc);
   __syncthreads();                                // every wave is done with the limbs
   // lane holds lags dx = dx0 + 16 i + 4 g + q, dy = dy0 + 16 j + m
 #pragma unroll

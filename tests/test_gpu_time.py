@@ -91,3 +91,41 @@ def test_time_surface_xs32_matches_int64_bruteforce(xg, orc, pair):
     den = np.sqrt(d1.astype(np.float64) * float(d2))
     brute = np.where(den == 0, 0.0, 100.0 * np.abs(num / np.where(den == 0, 1.0, den)))
     assert np.array_equal(surf, brute)
+
+
+def _scaled(pair, fm, fs, left_only=False):
+    m, s = pair
+    out = []
+    for img, f in ((m, fm), (s, fs)):
+        v = img.astype(np.int64)
+        if left_only:
+            v[:, : v.shape[1] // 2] *= f
+        else:
+            v *= f
+        out.append(np.clip(v, -32768, 32767).astype(np.int16))
+    return tuple(out)
+
+
+@pytest.mark.parametrize("fm,fs,left", [(1, 1, False), (5, 1, False), (1, 5, False), (5, 5, False), (5, 5, True)],
+                         ids=["small", "master_big", "template_big", "both_big", "mixed"])
+def test_time_xs32_limb_counts(xg, orc, pair, fm, fs, left):
+    """Windows whose integer values leave [-8256, 8127] need the third 7-bit
+    limb; the others skip its products.  Every combination (and a batch that
+    mixes them) must stay bit-exact, and the surface must equal the int64
+    correlation."""
+    from numpy.lib.stride_tricks import sliding_window_view
+    sp = _scaled(pair, fm, fs, left)
+    p, got, ref, surf, (a1, a2, means) = _run(xg, orc, sp, 32, 32, 0, nxl=4, nyl=2)
+    _exact(got, ref)
+    m1f, m2f = np.float32(means[0]), np.float32(means[1])
+    i1 = (a1 - m1f).astype(np.int64)
+    i2 = (a2 - m2f).astype(np.int64)
+    print("max |v| master %d template %d" % (np.abs(i1).max(), np.abs(i2[32:96, 32:96]).max()))
+    b = i2[32:96, 32:96]
+    win = sliding_window_view(i1[:127, :127], (64, 64))
+    num = np.einsum("ijkl,kl->ij", win, b)
+    d1 = np.einsum("ijkl,ijkl->ij", win, win)
+    d2 = int((b * b).sum())
+    den = np.sqrt(d1.astype(np.float64) * float(d2))
+    brute = np.where(den == 0, 0.0, 100.0 * np.abs(num / np.where(den == 0, 1.0, den)))
+    assert np.array_equal(surf, brute)
