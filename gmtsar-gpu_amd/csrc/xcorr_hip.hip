@@ -1991,9 +1991,20 @@ __global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const f
   // step: lane l holds A[r][l] and A[r][64 + l], and
   // E[r][l] = (sum_{x >= l} A[r][x]^2) + (sum_{x < l} A[r][64 + x]^2),
   // a suffix and an exclusive prefix sum over the wave (exact in int64)
-  for (int r = wave; r < 127; r += kThreads / 64) {
-    const long long v0 = (int)(a1[r * 128 + lane] - m1f);
-    const long long v1 = (int)(a1[r * 128 + 64 + lane] - m1f);
+  static_assert(kThreads == 256, "four waves, 32 rows each");
+  float f0[32], f1[32];                           // all loads of the wave's rows in flight together
+#pragma unroll
+  for (int k = 0; k < 32; k++) {
+    const int r = min(wave + 4 * k, 126);
+    f0[k] = a1[r * 128 + lane];
+    f1[k] = a1[r * 128 + 64 + lane];
+  }
+#pragma unroll
+  for (int k = 0; k < 32; k++) {
+    const int r = wave + 4 * k;
+    if (r >= 127) break;
+    const long long v0 = (int)(f0[k] - m1f);
+    const long long v1 = (int)(f1[k] - m1f);
     const long long q0 = v0 * v0, q1 = v1 * v1;
     long long p0 = q0, p1 = q1;                   // inclusive prefix sums over the lanes
 #pragma unroll
@@ -2054,8 +2065,7 @@ template <int KA, int KB>
 __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed char* Bl, int dx0, int dy0, int m,
                                              int g, v4i_t (&acc)[2][2][5]) {
   const v4i_t zero = v4i_t{0, 0, 0, 0};
-  for (int r = dy0; r < dy0 + 32 + 63; r++) {     // master rows that meet the wave's lag rows
-    v4i_t A[2][KA], B[2][KB];
+  auto load = [&](int r, v4i_t (&A)[2][KA], v4i_t (&B)[2][KB]) {
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -2069,6 +2079,8 @@ __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed
       for (int k = 0; k < KB; k++)
         B[j][k] = ok ? *(const v4i_t*)(Bl + (k * 64 + (ok ? y : 0)) * 64 + 16 * g) : zero;
     }
+  };
+  auto mma = [&](const v4i_t (&A)[2][KA], const v4i_t (&B)[2][KB]) {
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -2078,7 +2090,20 @@ __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed
 #pragma unroll
           for (int kb = 0; kb < KB; kb++)
             acc[i][j][ka + kb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[i][ka], B[j][kb], acc[i][j][ka + kb], 0, 0, 0);
+  };
+  // master rows r = dy0 .. dy0 + 94 meet the wave's lag rows; the operands of
+  // row r + 1 load while the products of row r run
+  const int r_end = dy0 + 32 + 63;
+  v4i_t A0[2][KA], B0[2][KB], A1[2][KA], B1[2][KB];
+  load(dy0, A0, B0);
+  int r = dy0;
+  for (; r + 1 < r_end; r += 2) {
+    load(r + 1, A1, B1);
+    mma(A0, B0);
+    if (r + 2 < r_end) load(r + 2, A0, B0);
+    mma(A1, B1);
   }
+  if (r < r_end) mma(A0, B0);
 }
 
 __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, const float* __restrict__ amp,
