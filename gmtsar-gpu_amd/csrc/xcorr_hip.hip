@@ -825,12 +825,13 @@ __device__ __forceinline__ void interp_line(const unsigned (&u)[RegFft<N>::V], i
   }
 }
 
-// Strip lines for npx = npy = N in {128, 512} (the outputs of k_stripA256):
+// Strip lines for npx = npy = N in {128, 512} (the outputs of k_stripA256;
+// without ROWFFT, for the time modes, amplitudes and line sums only):
 // block b + blk_off covers kStripBlockLines lines from global line
 // blk[2b + 1] of strip blk[2b], one line per group of G lanes and iteration;
 // the groups never wait for each other.  Zu holds bins 0 .. N/2 of the
 // unmasked rows' spectra (the rest are their conjugate mirrors).
-template <int N, int FMT, bool EDGE>
+template <int N, int FMT, bool EDGE, bool ROWFFT = true>
 __global__ void __launch_bounds__(kThreads) k_stripA(DevParams P, const void* __restrict__ master,
                                                      const void* __restrict__ slave,
                                                      const StripDesc* __restrict__ strips,
@@ -873,7 +874,7 @@ __global__ void __launch_bounds__(kThreads) k_stripA(DevParams P, const void* __
       for (int q = 0; q < V / 2; q++) {
         const float2 a = make_float2(ea[q], oa[q]);
         o[j + G * q] = a;
-        mr[j + G * q] = a;
+        if (ROWFFT) mr[j + G * q] = a;
         s.add(ea[q]);
         s.add(oa[q]);
       }
@@ -890,13 +891,14 @@ __global__ void __launch_bounds__(kThreads) k_stripA(DevParams P, const void* __
         const float2 a = make_float2(ea[q], oa[q]);
         const int c = j + G * q;
         if (wm && c >= c_lo && c < c_hi) o[c] = a;
-        tile[c] = a;                               // the tile doubles as the slave line buffer
+        if (ROWFFT) tile[c] = a;                   // the tile doubles as the slave line buffer
         s.add(ea[q]);
         s.add(oa[q]);
       }
       const long long st = s.group_total();
       if (j == 0) lsum[2 * l + 1] = st;
     }
+    if (!ROWFFT) continue;                         // time modes: amplitudes and sums only
     wave_sync();                                   // the group's two lines are in LDS
     const float* m1 = (const float*)mr;
     const float* a2 = (const float*)tile;
@@ -1979,14 +1981,15 @@ typedef int v4i_t __attribute__((ext_vector_type(4)));
 
 // Exact master energies sum_{y, x < 64} A[y + dy][x + dx]^2 of every lag
 // (int64 sliding sums), as doubles (exact: < 2^53).  One block per patch.
-__global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const float* __restrict__ amp,
+__global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const float* __restrict__ A1,
+                                                            const int* __restrict__ rb,
                                                             const unsigned long long* __restrict__ sums,
                                                             double* __restrict__ aa) {
   __shared__ long long E[127 * 64];
   const int b = blockIdx.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float m1f = (float)patch_mean(sums[2 * b], P.N);
-  const float* a1 = amp + (size_t)(2 * b) * P.N;
+  const float* a1 = rb ? A1 + (size_t)rb[b] * 128 : A1 + (size_t)(2 * b) * P.N;
   // row energies E[r][dx] = sum_{x < 64} A[r][x + dx]^2, one row per wave and
   // step: lane l holds A[r][l] and A[r][64 + l], and
   // E[r][l] = (sum_{x >= l} A[r][x]^2) + (sum_{x < l} A[r][64 + x]^2),
@@ -2106,7 +2109,9 @@ __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed
   if (r < r_end) mma(A0, B0);
 }
 
-__global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, const float* __restrict__ amp,
+__global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, const float* __restrict__ A1,
+                                                             const float* __restrict__ A2,
+                                                             const int* __restrict__ rb,
                                                              const unsigned long long* __restrict__ sums,
                                                              const double* __restrict__ aa,
                                                              double* __restrict__ surfd, int* __restrict__ tpeak,
@@ -2118,8 +2123,9 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
   const int b = blockIdx.x;
   const float m1f = (float)patch_mean(sums[2 * b], P.N);
   const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
-  const float* a1 = amp + (size_t)(2 * b) * P.N;
-  const float* a2 = a1 + P.N;
+  // amplitudes of patch b: rows of 128 from row base rb[b] of the strip lines, or per patch
+  const float* a1 = rb ? A1 + (size_t)rb[b] * 128 : A1 + (size_t)(2 * b) * P.N;
+  const float* a2 = rb ? A2 + (size_t)rb[b] * 128 : a1 + P.N;
   if (threadIdx.x == 0) {
     sbig = 0;
     shi[0] = 0;
@@ -2534,7 +2540,7 @@ static bool use_strips(const xcorr_hip_ctx* c) {
 static int ensure_patch_buffers(xcorr_hip_ctx* c) {
   const size_t B = (size_t)c->max_batch, N = (size_t)c->dp.N;
   if (!c->amp && hipMalloc((void**)&c->amp, B * 2 * N * sizeof(float)) != hipSuccess) return XCORR_HIP_ENOMEM;
-  if (c->z_elems < B * N) {
+  if (c->dp.corr_flag == XCORR_HIP_FREQ && c->z_elems < B * N) {
     float2* z = nullptr;
     if (hipMalloc((void**)&z, B * N * sizeof(float2)) != hipSuccess) return XCORR_HIP_ENOMEM;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return XCORR_HIP_EDEVICE;
@@ -2655,8 +2661,11 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   auto alloc = [&](void** ptr, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(ptr, bytes ? bytes : 16);
   };
-  c->strips = d.NX == d.NY && (d.NX == 128 || d.NX == 256 || d.NX == 512) && d.ri == 2 &&
-              p->corr_flag == XCORR_HIP_FREQ && phi_sym;
+  // strips: square 128 / 256 / 512 windows in frequency mode, and -time on
+  // 128 x 128 windows (xs = ys = 32, the int8 search); ri 2
+  c->strips = d.NX == d.NY && d.ri == 2 && phi_sym &&
+              ((p->corr_flag == XCORR_HIP_FREQ && (d.NX == 128 || d.NX == 256 || d.NX == 512)) ||
+               (p->corr_flag == XCORR_HIP_TIME && p->xsearch == 32 && p->ysearch == 32));
   if (c->strips) {
     // half the lines of unshared windows: overlapping windows need far fewer
     c->max_lines = std::max(max_batch * (d.NY / 2), d.NY + kStripBlockLines);
@@ -2665,7 +2674,7 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
     alloc((void**)&c->amp1s, L * d.NX * sizeof(float));
     alloc((void**)&c->amp2s, L * d.NX * sizeof(float));
     alloc((void**)&c->lsum, L * 2 * sizeof(long long));
-    alloc((void**)&c->Zu, L * d.NX * sizeof(float2));
+    if (p->corr_flag == XCORR_HIP_FREQ) alloc((void**)&c->Zu, L * d.NX * sizeof(float2));
     alloc((void**)&c->need, L * 2);
     c->plan_bytes = plan_ints(c) * sizeof(int);
     alloc((void**)&c->d_plan, c->plan_bytes);
@@ -2968,6 +2977,17 @@ static void launch_colpass(xcorr_hip_ctx* c, const DevParams& d, hipStream_t st,
                      (const float2*)c->Zu, rb, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs, nwork);
 }
 
+// The -time search of 128 x 128 windows (xs = ys = 32) on the int8 matrix
+// cores: sliding energies, then the search with its peak scan.  Amplitudes
+// per patch (rb null: A1 = A2 = amp) or on strip lines from row bases rb.
+static void launch_time_i8(xcorr_hip_ctx* c, const DevParams& d, hipStream_t st, const float* A1, const float* A2,
+                           const int* rb, int nb) {
+  hipLaunchKernelGGL(k_time_energy32, dim3(nb), dim3(kThreads), 0, st, d, A1, rb, (const unsigned long long*)c->sums,
+                     c->tcaa);
+  hipLaunchKernelGGL(k_time_search_i8, dim3(nb), dim3(kThreads), 0, st, d, A1, A2, rb,
+                     (const unsigned long long*)c->sums, (const double*)c->tcaa, c->surfd, c->tpeak, c->tcave);
+}
+
 // Kernels of one strip batch whose plan is in device memory at d_plan, on
 // stream st with images (master, slave) and image dims in d.  Results go to
 // out[perm[b]] (dev_out) or to c->d_out[b].
@@ -2988,6 +3008,7 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   });
   if (rc) return rc;
   const bool ci = d.format == XCORR_HIP_CINT16;
+  const bool freq = d.corr_flag == XCORR_HIP_FREQ;    // else -time on 128 x 128 windows
   rc = launch_timed(c, timing, d.NX == 256 ? "k_stripA256" : "k_stripA", [&] {
     using KA = void (*)(DevParams, const void*, const void*, const StripDesc*, const int*, int, const unsigned char*,
                         const unsigned char*, float*, float*, long long*, float2*, float2*, const float2*);
@@ -2995,6 +3016,9 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
     if (d.NX == 256) {
       kin = ci ? k_stripA256<XCORR_HIP_CINT16, false> : k_stripA256<XCORR_HIP_FLOAT32, false>;
       kedge = ci ? k_stripA256<XCORR_HIP_CINT16, true> : k_stripA256<XCORR_HIP_FLOAT32, true>;
+    } else if (d.NX == 128 && !freq) {
+      kin = ci ? k_stripA<128, XCORR_HIP_CINT16, false, false> : k_stripA<128, XCORR_HIP_FLOAT32, false, false>;
+      kedge = ci ? k_stripA<128, XCORR_HIP_CINT16, true, false> : k_stripA<128, XCORR_HIP_FLOAT32, true, false>;
     } else if (d.NX == 128) {
       kin = ci ? k_stripA<128, XCORR_HIP_CINT16, false> : k_stripA<128, XCORR_HIP_FLOAT32, false>;
       kedge = ci ? k_stripA<128, XCORR_HIP_CINT16, true> : k_stripA<128, XCORR_HIP_FLOAT32, true>;
@@ -3017,6 +3041,11 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
                        c->sums, nb);
   });
   if (rc) return rc;
+  if (!freq) {
+    rc = launch_timed(c, timing, "k_time_search", [&] {
+      launch_time_i8(c, d, st, (const float*)c->amp1s, (const float*)c->amp2s, d_rb, nb);
+    });
+  } else {
   rc = launch_timed(c, timing, d.NX == 256 ? "k_colpass256" : "k_colpass", [&] {
     if (d.NX == 256)
       hipLaunchKernelGGL(k_colpass256, dim3(nb * (d.NX / 32)), dim3(kThreads), 0, st, d, (const float2*)c->Z,
@@ -3038,14 +3067,15 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
       hipLaunchKernelGGL(k_rowifft_surf_reg<512>, dim3(nb), dim3(kThreads), 0, st, d, (const float2*)c->D, c->surf,
                          c->peak, c->partial, (const float2*)c->tabs);
   });
+  }
   if (rc) return rc;
   const size_t sh5 = d.interp_flag ? ((size_t)d.Lx * 16 + (size_t)d.Ly * 8) * sizeof(float2) : 16;
   return launch_timed(c, timing, "k_finish", [&] {
     hipLaunchKernelGGL(k_finish, dim3(nb), dim3(kThreads), sh5, st, d, (const float*)c->amp1s,
                        (const float*)c->amp2s, d_rb, 0LL, dev_out ? d_perm : (const int*)nullptr,
-                       (const unsigned long long*)c->sums, (const float*)c->surf, (const double*)nullptr,
-                       (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)nullptr,
-                       (const double*)nullptr, dev_out ? out : c->d_out, (const float2*)c->tabs);
+                       (const unsigned long long*)c->sums, (const float*)c->surf, (const double*)c->surfd,
+                       (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)c->tpeak,
+                       (const double*)c->tcave, dev_out ? out : c->d_out, (const float2*)c->tabs);
   });
 }
 
@@ -3224,11 +3254,7 @@ static int time_search(xcorr_hip_ctx* c, const xcorr_hip_window*, int nb, bool t
   const bool i8 = d.corr_flag == XCORR_HIP_TIME && d.xs == 32 && d.ys == 32 && c->tcaa;   // peak found in-kernel
   if (i8)
     rc = launch_timed(c, timing, "k_time_search", [&] {
-      hipLaunchKernelGGL(k_time_energy32, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float*)c->amp,
-                         (const unsigned long long*)c->sums, c->tcaa);
-      hipLaunchKernelGGL(k_time_search_i8, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float*)c->amp,
-                         (const unsigned long long*)c->sums, (const double*)c->tcaa, c->surfd, c->tpeak,
-                         c->tcave);
+      launch_time_i8(c, d, c->stream, (const float*)c->amp, (const float*)c->amp, nullptr, nb);
     });
   else if (d.corr_flag == XCORR_HIP_TIME && d.nxc % TL == 0 && RCb >= 1)
     rc = launch_timed(c, timing, "k_time_search", [&] {

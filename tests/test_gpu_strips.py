@@ -130,3 +130,17 @@ def test_strip_dumps_in_shared_strip(xg, orc, pairs, xs):
             assert np.array_equal(amp2[t].view(np.uint32), d["amp2"][t].view(np.uint32)), "amp slave, slot %d" % slot
             assert np.array_equal(means, d["mean"]), "means, slot %d" % slot
             assert np.array_equal(surf.view(np.uint32), d["surf"].view(np.uint32)), "surface, slot %d" % slot
+
+
+def test_time_strips_bit_exact(xg, orc, pairs):
+    """-time on 128 x 128 windows (xs = ys = 32) also runs on strip lines: the
+    planner cases above, against the restatement's full integer search."""
+    m, s = pairs[32]
+    p = xg.make_params(32, 32, corr_flag=xg.TIME, interp_factor=8)
+    w = _strip_windows(xg, 32)
+    ref = orc.run(_oracle_params(orc, p), m, s, _locs(w), precision=32)
+    for mb in (7, 64):
+        with xg.Engine(p, max_batch=mb) as eng:
+            eng.set_images(m, s)
+            got = eng.correlate(w)
+        _assert_bit_exact(got, ref, "time max_batch %d" % mb)

@@ -25,9 +25,10 @@ def _run(xg, orc, pair, xs, ys, corr_flag, ri=2, ifc=16, interp=1, nxl=4, nyl=3)
     with xg.Engine(p, max_batch=5) as eng:
         eng.set_images(m, s)
         got = eng.correlate(w)
-        first = (len(w) - 1) // 5 * 5              # dumps reach the last batch only
-        surf0 = eng.dump(2, first)
-        amp = (eng.dump(0, first), eng.dump(1, first), eng.dump(3, first))
+        first = (len(w) - 1) // 5 * 5
+        eng.correlate(w[first:first + 1])         # dumps reach the last call's last batch
+        surf0 = eng.dump(2, 0)
+        amp = (eng.dump(0, 0), eng.dump(1, 0), eng.dump(3, 0))
     op = orc.make_params(xs, ys, ri=ri, interp_factor=ifc, interp_flag=interp, corr_flag=corr_flag)
     locs = np.stack([w["mx0"], w["my0"], w["sx0"], w["sy0"]], 1).astype(np.int32)
     ref = orc.run(op, m, s, locs, precision=32)
