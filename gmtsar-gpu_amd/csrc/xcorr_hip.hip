@@ -524,7 +524,8 @@ __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __
 }
 
 // --------------------------------------------------------------------------
-// Strips (npx = npy = 256, ri = 2, frequency mode).  Windows that share the
+// Strips (square windows of 128, 256, 512 samples, ri = 2; frequency mode,
+// and -time on 128 x 128 without the row transforms).  Windows that share the
 // master column, the slave column and the slave line offset see the same
 // range-interpolated lines, so a strip computes each line once: amplitudes,
 // their exact fixed-point sum, and the row transforms of z = amp1 + i amp2

@@ -141,8 +141,10 @@ int xcorr_hip_correlate_stream(xcorr_hip_ctx *ctx, xcorr_hip_line_reader reader,
  * its order).  A radix plan other than the default (npx/npy = 128: 16 x 8,
  * 256: 16 x 16, 512: 8 x 8 x 8, ...) is a different FP32 rounding of the
  * transforms of that length; the CPU restatement accepts the same plan.
- * Radix plans apply to the generic LDS kernels; the 256 x 256 strip path
- * (xsearch = ysearch = 64, ri 2, frequency) runs the default plan only.
+ * Radix plans apply to the generic LDS kernels.  Square 128 / 256 / 512
+ * windows (ri 2, frequency mode; -time on 128 x 128) run shared strip lines
+ * with the default plans; setting another plan on such a context switches it
+ * to the per-patch kernels (their buffers are allocated at that call).
  * No counterpart in the reference (cuFFT picks its own plans, xcorr.cu:161-230
  * sizes batches but is unused). */
 typedef struct xcorr_hip_tuning {
