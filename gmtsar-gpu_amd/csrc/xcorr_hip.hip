@@ -2134,28 +2134,53 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
   }
   __syncthreads();
   int big = 0, hia = 0, hib = 0;
-  for (int e = threadIdx.x; e < 128 * kLimbPitch; e += kThreads) {
-    const int r = e / kLimbPitch, x = e - r * kLimbPitch;
-    int v = 0;
-    if (x < 128) v = (int)(a1[r * 128 + x] - m1f);
-    big |= (v >= (1 << 20) || v <= -(1 << 20));
-    signed char l[3];
-    limbs7(v, l);
-    hia |= l[2];
+  // window integers and their limbs, four columns per thread and step: all
+  // float4 loads of the window (16 per thread) and of the template (4) are
+  // issued before the first use; each limb row takes one dword per 4 columns
+  static_assert(kThreads == 256, "16 window and 4 template float4 per thread");
+  auto pack4 = [&](const float4 f, float mf, unsigned (&w)[3], int& hi, long long* e2) {
+    const int v[4] = {(int)(f.x - mf), (int)(f.y - mf), (int)(f.z - mf), (int)(f.w - mf)};
+    w[0] = w[1] = w[2] = 0u;
 #pragma unroll
-    for (int k = 0; k < 3; k++) Al[(k * 128 + r) * kLimbPitch + x] = l[k];
+    for (int q = 0; q < 4; q++) {
+      big |= (v[q] >= (1 << 20) || v[q] <= -(1 << 20));
+      if (e2) *e2 += (long long)v[q] * v[q];
+      signed char l[3];
+      limbs7(v[q], l);
+      hi |= l[2];
+#pragma unroll
+      for (int k = 0; k < 3; k++) w[k] |= (unsigned)(unsigned char)l[k] << (8 * q);
+    }
+  };
+  float4 fa[16], fb[4];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const int idx = threadIdx.x + kThreads * k;   // row idx / 32, columns 4 (idx % 32) ..
+    fa[k] = ((const float4*)(a1 + (idx >> 5) * 128))[idx & 31];
   }
-  long long b2 = 0;
-  for (int e = threadIdx.x; e < 64 * 64; e += kThreads) {
-    const int y = e >> 6, x = e & 63;
-    const int v = (int)(a2[(size_t)(P.ys + y) * 128 + P.xs + x] - m2f);
-    big |= (v >= (1 << 20) || v <= -(1 << 20));
-    b2 += (long long)v * v;
-    signed char l[3];
-    limbs7(v, l);
-    hib |= l[2];
 #pragma unroll
-    for (int k = 0; k < 3; k++) Bl[(k * 64 + y) * 64 + x] = l[k];
+  for (int k = 0; k < 4; k++) {
+    const int idx = threadIdx.x + kThreads * k;   // template row idx / 16, columns 4 (idx % 16) ..
+    fb[k] = ((const float4*)(a2 + (size_t)(P.ys + (idx >> 4)) * 128 + P.xs))[idx & 15];
+  }
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const int idx = threadIdx.x + kThreads * k, r = idx >> 5, x = 4 * (idx & 31);
+    unsigned w[3];
+    pack4(fa[k], m1f, w, hia, nullptr);
+#pragma unroll
+    for (int l = 0; l < 3; l++) *(unsigned*)(Al + (l * 128 + r) * kLimbPitch + x) = w[l];
+  }
+  for (int e = threadIdx.x; e < 3 * 128 * 4; e += kThreads)      // columns 128 .. 143 of every limb row
+    *(unsigned*)(Al + (e >> 2) * kLimbPitch + 128 + 4 * (e & 3)) = 0u;
+  long long b2 = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int idx = threadIdx.x + kThreads * k, y = idx >> 4, x = 4 * (idx & 15);
+    unsigned w[3];
+    pack4(fb[k], m2f, w, hib, &b2);
+#pragma unroll
+    for (int l = 0; l < 3; l++) *(unsigned*)(Bl + (l * 64 + y) * 64 + x) = w[l];
   }
   if (big) atomicOr(&sbig, 1);
   if (hia) atomicOr(&shi[0], 1);
