@@ -1982,6 +1982,30 @@ typedef int v4i_t __attribute__((ext_vector_type(4)));
 
 // Exact master energies sum_{y, x < 64} A[y + dy][x + dx]^2 of every lag
 // (int64 sliding sums), as doubles (exact: < 2^53).  One block per patch.
+// Inclusive prefix sum of an int64 over the 64 lanes of a wave with DPP lane
+// moves (row shifts by 1, 2, 4, 8 within rows of 16, then the row broadcasts
+// of lanes 15 and 31); exact in any order.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ long long dpp_i64(long long v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)v, CTRL, ROWS, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, ROWS, 0xf, true);
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ long long wave_incl_scan_i64(long long x) {
+  x += dpp_i64<0x111, 0xf>(x);                   // row_shr:1
+  x += dpp_i64<0x112, 0xf>(x);                   // row_shr:2
+  x += dpp_i64<0x114, 0xf>(x);                   // row_shr:4
+  x += dpp_i64<0x118, 0xf>(x);                   // row_shr:8
+  x += dpp_i64<0x142, 0xa>(x);                   // row_bcast:15 into rows 1 and 3
+  x += dpp_i64<0x143, 0xc>(x);                   // row_bcast:31 into rows 2 and 3
+  return x;
+}
+__device__ __forceinline__ long long wave_lane63_i64(long long x) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(x >> 32), 63);
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
 __global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const float* __restrict__ A1,
                                                             const int* __restrict__ rb,
                                                             const unsigned long long* __restrict__ sums,
@@ -2010,16 +2034,8 @@ __global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const f
     const long long v0 = (int)(f0[k] - m1f);
     const long long v1 = (int)(f1[k] - m1f);
     const long long q0 = v0 * v0, q1 = v1 * v1;
-    long long p0 = q0, p1 = q1;                   // inclusive prefix sums over the lanes
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const long long t0 = __shfl_up(p0, o), t1 = __shfl_up(p1, o);
-      if (lane >= o) {
-        p0 += t0;
-        p1 += t1;
-      }
-    }
-    const long long tot0 = __shfl(p0, 63);
+    const long long p0 = wave_incl_scan_i64(q0), p1 = wave_incl_scan_i64(q1);
+    const long long tot0 = wave_lane63_i64(p0);
     E[r * 64 + lane] = (tot0 - p0 + q0) + (p1 - q1);
   }
   __syncthreads();
@@ -2060,6 +2076,7 @@ __device__ __forceinline__ v4i_t ld16_unaligned(const signed char* row, int o) {
 }
 
 constexpr int kLimbPitch = 144;                   // bytes per limb row of the window (128 + 16 for the tail loads)
+constexpr int kBPad = 32, kBRows = 128;           // template limb rows: 32 zero rows, 64 rows, 32 zero rows
 
 // The wave's 2 x 2 tiles of 16 x 16 lags (dx0 + 16 i + .., dy0 + 16 j + ..):
 // for every master row r that meets them, the Toeplitz rows A[r][x + dx]
@@ -2068,7 +2085,6 @@ constexpr int kLimbPitch = 144;                   // bytes per limb row of the w
 template <int KA, int KB>
 __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed char* Bl, int dx0, int dy0, int m,
                                              int g, v4i_t (&acc)[2][2][5]) {
-  const v4i_t zero = v4i_t{0, 0, 0, 0};
   auto load = [&](int r, v4i_t (&A)[2][KA], v4i_t (&B)[2][KB]) {
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -2077,23 +2093,28 @@ __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed
         A[i][k] = ld16_unaligned(Al + (k * 128 + r) * kLimbPitch, dx0 + 16 * i + m + 16 * g);
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-      const int y = r - (dy0 + 16 * j + m);       // template row of lag row dy = dy0 + 16 j + m
-      const bool ok = y >= 0 && y < 64;
+      // template row of lag row dy = dy0 + 16 j + m: y = r - dy lies in [-31, 94];
+      // the rows outside [0, 64) are the zero rows around the template
+      const int y = r - (dy0 + 16 * j + m);
 #pragma unroll
-      for (int k = 0; k < KB; k++)
-        B[j][k] = ok ? *(const v4i_t*)(Bl + (k * 64 + (ok ? y : 0)) * 64 + 16 * g) : zero;
+      for (int k = 0; k < KB; k++) B[j][k] = *(const v4i_t*)(Bl + (k * kBRows + kBPad + y) * 64 + 16 * g);
     }
   };
-  auto mma = [&](const v4i_t (&A)[2][KA], const v4i_t (&B)[2][KB]) {
+  // lag-row tile j meets master rows dy0 + 16 j .. dy0 + 16 j + 78 only: the
+  // products of a row outside that range are zero for every lane and skipped
+  // (the bounds are wave-uniform)
+  auto mma = [&](int r, const v4i_t (&A)[2][KA], const v4i_t (&B)[2][KB]) {
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) {
+      if (r < dy0 + 16 * j || r >= dy0 + 16 * j + 79) continue;
 #pragma unroll
-      for (int j = 0; j < 2; j++)
+      for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int ka = 0; ka < KA; ka++)
 #pragma unroll
           for (int kb = 0; kb < KB; kb++)
             acc[i][j][ka + kb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[i][ka], B[j][kb], acc[i][j][ka + kb], 0, 0, 0);
+    }
   };
   // master rows r = dy0 .. dy0 + 94 meet the wave's lag rows; the operands of
   // row r + 1 load while the products of row r run
@@ -2103,11 +2124,11 @@ __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed
   int r = dy0;
   for (; r + 1 < r_end; r += 2) {
     load(r + 1, A1, B1);
-    mma(A0, B0);
+    mma(r, A0, B0);
     if (r + 2 < r_end) load(r + 2, A0, B0);
-    mma(A1, B1);
+    mma(r + 1, A1, B1);
   }
-  if (r < r_end) mma(A0, B0);
+  if (r < r_end) mma(r, A0, B0);
 }
 
 __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, const float* __restrict__ A1,
@@ -2118,7 +2139,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
                                                              double* __restrict__ surfd, int* __restrict__ tpeak,
                                                              double* __restrict__ tcave) {
   __shared__ __attribute__((aligned(16))) signed char Al[3 * 128 * kLimbPitch];
-  __shared__ __attribute__((aligned(16))) signed char Bl[3 * 64 * 64];
+  __shared__ __attribute__((aligned(16))) signed char Bl[3 * kBRows * 64];
   __shared__ long long redl[kThreads / 64];
   __shared__ int sbig, shi[2];                    // values beyond the limbs; limb 2 nonzero (master, template)
   const int b = blockIdx.x;
@@ -2180,7 +2201,11 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
     unsigned w[3];
     pack4(fb[k], m2f, w, hib, &b2);
 #pragma unroll
-    for (int l = 0; l < 3; l++) *(unsigned*)(Bl + (l * 64 + y) * 64 + x) = w[l];
+    for (int l = 0; l < 3; l++) *(unsigned*)(Bl + (l * kBRows + kBPad + y) * 64 + x) = w[l];
+  }
+  for (int e = threadIdx.x; e < 3 * 64 * 4; e += kThreads) {       // the zero rows, 16 bytes a store
+    const int l = e / 256, q = e % 256, row = q < 128 ? q >> 2 : kBPad + 64 + ((q - 128) >> 2);
+    *(uint4*)(Bl + (l * kBRows + row) * 64 + 16 * (q & 3)) = make_uint4(0u, 0u, 0u, 0u);
   }
   if (big) atomicOr(&sbig, 1);
   if (hia) atomicOr(&shi[0], 1);
@@ -2208,7 +2233,9 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
     }
   } else {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
-  const int dx0 = 32 * (wave & 1), dy0 = 32 * (wave >> 1);   // the wave's 2 x 2 tiles of 16 x 16 lags
+  // the wave's 2 x 2 tiles of 16 x 16 lags (wave-uniform corners)
+  const int dx0 = __builtin_amdgcn_readfirstlane(32 * (wave & 1));
+  const int dy0 = __builtin_amdgcn_readfirstlane(32 * (wave >> 1));
   v4i_t acc[2][2][5];
 #pragma unroll
   for (int i = 0; i < 2; i++)
