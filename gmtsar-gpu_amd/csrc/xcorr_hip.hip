@@ -670,7 +670,7 @@ struct LineSum {
 // Block b + blk_off covers kStripBlockLines lines from global line blk[2b + 1]
 // of strip blk[2b].  EDGE: the strips' windows may cross the image sides.
 template <int FMT, bool EDGE>
-__global__ void __launch_bounds__(kThreads, 2) k_stripA256(DevParams P, const void* __restrict__ master,
+__global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams P, const void* __restrict__ master,
                                                         const void* __restrict__ slave,
                                                         const StripDesc* __restrict__ strips,
                                                         const int* __restrict__ blk, int blk_off,
@@ -695,24 +695,20 @@ __global__ void __launch_bounds__(kThreads, 2) k_stripA256(DevParams P, const vo
   const unsigned* im_s = (const unsigned*)slave;
   const int first = blk[2 * bb + 1] - S.line0;                   // strip-relative line of this block
   const int nit = min(kStripBlockLines, S.nlines - first + 15) >> 4;
-  // Loads run ahead of the transforms with one line in flight at a time: the
-  // slave line is loaded before the master transforms, the next master line
-  // before the slave and z transforms.
-  unsigned um[16];
-  load_line256<EDGE>(im_m, P.m_nx, P.m_ny, P.m_lo, P.m_hi, S.my0 + first + g, S.mx0, j, um);
+  // Both lines of an iteration are loaded together, before the master
+  // transforms (3 waves per SIMD hide the rest of the latency).
   __syncthreads();
   for (int it = 0; it < nit; it++) {
     const int i = first + 16 * it + g;                            // strip-relative line of this group
     const bool act = i < S.nlines;
-    const bool more = it + 1 < nit;
     const size_t l = (size_t)(S.line0 + i);
     const bool wm = act && needM[l], wu = act && needU[l];
-    unsigned us[16];
+    unsigned um[16], us[16];
+    load_line256<EDGE>(im_m, P.m_nx, P.m_ny, P.m_lo, P.m_hi, S.my0 + i, S.mx0, j, um);
     load_line256<EDGE>(im_s, P.s_nx, P.s_ny, P.s_lo, P.s_hi, S.sy0 + i, S.sx0, j, us);
     {
       float ea[8], oa[8];
       interp_line256<FMT>(um, j, tile, twl, sph, ea, oa);
-      if (more) load_line256<EDGE>(im_m, P.m_nx, P.m_ny, P.m_lo, P.m_hi, S.my0 + i + 16, S.mx0, j, um);
       LineSum<FMT> s;
       float2* o = (float2*)(amp1 + l * 256);
 #pragma unroll
