@@ -32,8 +32,10 @@ def _shuffled_with_edges(xg, w, p):
     return np.ascontiguousarray(allw[rng.permutation(len(allw))]).view(xg.WINDOW_DTYPE).reshape(-1)
 
 
-@pytest.mark.parametrize("xs,ys,corr,band", [(64, 64, 2, 300), (64, 64, 2, 700), (32, 32, 2, 150), (16, 16, 0, 80)],
-                         ids=["strips_b300", "strips_b700", "freq32_b150", "time16_b80"])
+@pytest.mark.parametrize("xs,ys,corr,band", [(64, 64, 2, 300), (64, 64, 2, 700), (32, 32, 2, 150), (16, 16, 0, 80),
+                                             (128, 128, 2, 600), (32, 32, 0, 150)],
+                         ids=["strips_b300", "strips_b700", "strips32_b150", "time16_b80", "strips128_b600",
+                              "time_strips32_b150"])
 def test_stream_equals_resident(xg, pair, xs, ys, corr, band):
     m, s = pair
     p = xg.make_params(xs, ys, corr_flag=corr)
