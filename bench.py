@@ -147,15 +147,24 @@ def main():
     torch.cuda.synchronize()
     log("warmup done")
 
+    # HIP events on the engine's stream between the steps give the per-step spread
+    est = torch.cuda.ExternalStream(eng.stream())
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     t1 = time.perf_counter()
-    for _ in range(args.steps):
+    evs[0].record(est)
+    for k in range(args.steps):
         eng.correlate(w, out=dout, flags=xg.NO_SYNC)
+        evs[k + 1].record(est)
     eng.sync()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     step_s = (t2 - t1) / args.steps
     value = n / step_s
-    log("timed %d steps: %.3f ms/step, %.0f patches/s" % (args.steps, step_s * 1e3, value))
+    per_step = sorted(evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps))
+    step_stats = {"median_ms": round(per_step[len(per_step) // 2], 4), "min_ms": round(per_step[0], 4),
+                  "max_ms": round(per_step[-1], 4), "source": "HIP events on the engine stream"}
+    log("timed %d steps: %.3f ms/step, %.0f patches/s (median step %.3f ms)"
+        % (args.steps, step_s * 1e3, value, step_stats["median_ms"]))
 
     # per-kernel HIP-event timing on the engine's stream (separate, untimed step)
     eng.correlate(w, out=dout, flags=xg.TIMING)
@@ -171,15 +180,45 @@ def main():
     elif not args.small:
         tr = committed_traffic()
     traffic = tr.get(dom_name)
+    # measured device-copy bandwidth (the practical HBM ceiling) and the host-to-device
+    # rate of the pair itself (the PCIe-inclusive cost of handing over host buffers)
+    cbytes = 1 << 30
+    src = torch.empty(cbytes, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    copy_gbs = 2 * cbytes * 5 / (time.perf_counter() - tc) / 1e9
+    del src, dst
+    pair_bytes = 2 * m.numel() * m.element_size()
+    hbuf = torch.empty(m.numel() * m.element_size(), dtype=torch.uint8, pin_memory=True)
+    gbuf = torch.empty_like(hbuf, device=dev)
+    gbuf.copy_(hbuf, non_blocking=True)
+    torch.cuda.synchronize()
+    th = time.perf_counter()
+    for _ in range(2):
+        gbuf.copy_(hbuf, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d_s = (time.perf_counter() - th) / 2 * 2          # both images
+    del hbuf, gbuf
+    h2d = {"bytes": pair_bytes, "ms": round(h2d_s * 1e3, 3), "GBs": round(pair_bytes / h2d_s / 1e9, 2),
+           "pcie_inclusive_patches_per_s": round(n / (step_s + h2d_s), 2)}
+    log("device copy %.0f GB/s; pair upload %.1f ms (%.1f GB/s)" % (copy_gbs, h2d_s * 1e3, h2d["GBs"]))
+
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/hbm_traffic_xcorr_cfg2.md"
                                    if traffic and not args.traffic else None),
                 "kernel": dom_name, "launch_ms": round(per_launch_ms, 4),
-                "patches_per_launch": patches_per_launch}
+                "patches_per_launch": patches_per_launch, "copy_peak_GBs": round(copy_gbs, 1),
+                "frac_of_copy": round(achieved / copy_gbs, 5)}
     kernels = {k: {"ms_total": round(v[0], 4), "launches": v[1]} for k, v in kt.items()}
     path = {"hbm_GBs": round(BYTES_PER_PATCH * value / 1e9, 2),
             "hbm_frac": round(BYTES_PER_PATCH * value / 1e9 / HBM_PEAK_GBS, 5),
+            "hbm_frac_of_copy": round(BYTES_PER_PATCH * value / 1e9 / copy_gbs, 5),
             "fp32_TFLOPs": round(FLOP_PER_PATCH * value / 1e12, 3),
             "fp32_frac": round(FLOP_PER_PATCH * value / 1e12 / FP32_PEAK_TFLOPS, 5)}
     log("kernels:", json.dumps(kernels))
@@ -229,7 +268,7 @@ def main():
            "config": {"workload": workload, "patches": n, "xsearch": 64, "ysearch": 64, "ri": 2,
                       "interp_factor": 16, "batch": min(args.batch, n)},
            "roofline": roofline, "cpu_baseline": cpu, "path_roofline": path, "kernels": kernels,
-           "parity": parity}
+           "step_stats": step_stats, "h2d": h2d, "parity": parity}
     print(json.dumps(out), flush=True)
     eng.close()
     return 0
