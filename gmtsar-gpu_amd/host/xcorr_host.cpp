@@ -142,7 +142,7 @@ extern "C" int xh_read_prm(const char* path, xh_prm* prm) {
     if (sp != std::string::npos) value = value.substr(0, sp);
     if (name.find_first_of(" \t") != std::string::npos) name = name.substr(name.find_last_of(" \t") + 1);
     if (name == "SLC_file") {
-      strncpy(prm->SLC_file, value.c_str(), sizeof(prm->SLC_file) - 1);
+      snprintf(prm->SLC_file, sizeof(prm->SLC_file), "%s", value.c_str());
     } else if (name == "num_rng_bins") {
       prm->num_rng_bins = atoi(value.c_str());
     } else if (name == "num_patches") {
@@ -171,7 +171,7 @@ extern "C" int xh_handle_prm(const char* mprm, const char* sprm, xh_config* xc, 
       return 1;
     }
     if (i == 0) {
-      strncpy(xc->data1_name, r[0].SLC_file, sizeof(xc->data1_name) - 1);
+      snprintf(xc->data1_name, sizeof(xc->data1_name), "%s", r[0].SLC_file);
       FILE* f = fopen(xc->data1_name, "r");
       if (!f) {
         snprintf(err, errlen, " %s %s \n", "Cannot open SLC_file", xc->data1_name);
@@ -181,7 +181,7 @@ extern "C" int xh_handle_prm(const char* mprm, const char* sprm, xh_config* xc, 
       xc->m_nx = r[0].num_rng_bins;
       xc->m_ny = r[0].num_patches * r[0].num_valid_az;
     } else {
-      strncpy(xc->data2_name, r[1].SLC_file, sizeof(xc->data2_name) - 1);
+      snprintf(xc->data2_name, sizeof(xc->data2_name), "%s", r[1].SLC_file);
       FILE* f = fopen(xc->data2_name, "r");
       if (!f) {
         snprintf(err, errlen, " %s %s \n", "Cannot open SLC_file", xc->data2_name);
