@@ -971,15 +971,20 @@ __global__ void __launch_bounds__(kThreads) k_rowfft256(DevParams P, const float
   }
 }
 
-// C3 at one bin from A = Z[ky][kx] and Zm = Z[-ky][-kx] (Z = FFT2(c1 + i c2)).
+// C3 at one bin from A = Z[ky][kx] and Zm = Z[-ky][-kx] (Z = FFT2(c1 + i c2)):
+// C1 = (A + conj(Zm)) / 2, conj(C2) = i conj(A - conj(Zm)) / 2 written as the
+// swapped pair (d.y, d.x) / 2, C3 = C1 conj(C2) (-1)^(kx+ky).  The halvings are
+// applied as one factor of +-1/4 after the complex product: scaling by a power
+// of two commutes with every rounding of cmul (the values are far from the
+// overflow and subnormal ranges), so the result is bit-identical to halving
+// the operands first.
 __device__ __forceinline__ float2 c3_bin(float2 a, float2 zm, int kx, int ky) {
   const float2 bb = make_float2(zm.x, -zm.y);
-  const float2 c1 = make_float2((a.x + bb.x) * 0.5f, (a.y + bb.y) * 0.5f);
+  const float2 c1 = make_float2(a.x + bb.x, a.y + bb.y);
   const float2 d = csub(a, bb);
-  const float2 c2c = make_float2(0.5f * d.y, 0.5f * d.x);
-  float2 p = cmul(c1, c2c);
-  if ((kx + ky) & 1) { p.x = -p.x; p.y = -p.y; }
-  return p;
+  const float2 p = cmul(c1, make_float2(d.y, d.x));
+  const float f = ((kx + ky) & 1) ? -0.25f : 0.25f;
+  return make_float2(p.x * f, p.y * f);
 }
 
 // Demean in the row spectrum: the row transform of z = amp1 + i amp2 mask minus
