@@ -45,6 +45,7 @@ struct DevParams {
   int groups4;                    // k_rowifft_surf blocks per patch
   int tLR, tRC;                   // k_time_search: surface rows per block, template rows per LDS chunk
   int tw_nx, tw_ny, tw_lx, tw_ly, phi;   // offsets into the table buffer
+  int e8;                         // high-res row factors e^{2 pi i k~ s / Ly}, [Ly][8]
   int mk_x, mk_y;                 // spectra of the 1-D mask boxes (npx, npy entries)
   int m_nx, m_ny, s_nx, s_ny;
   // lines held in device memory (all of each image, or a streamed band):
@@ -1566,7 +1567,7 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
   extern __shared__ float2 smem[];
   __shared__ double redd[kThreads / 64];
   __shared__ unsigned long long redk[kThreads / 64];
-  __shared__ float2 sE[512];                 // e^{+2 pi i m/Lx} (m < Lx), then e^{+2 pi i m/Ly}
+  __shared__ float2 sE[512];                 // e^{+2 pi i m/Lx} (m < Lx)
   __shared__ float md[64];
   __shared__ float2 Xr[64];
   __shared__ double s_msc;
@@ -1676,9 +1677,7 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
     }
     __syncthreads();
     const float2* sEx = sE;
-    const float2* sEy = sE + Lx;
     for (int m = threadIdx.x; m < Lx; m += kThreads) sE[m] = tabs[P.tw_lx + m];
-    for (int m = threadIdx.x; m < Ly; m += kThreads) sE[Lx + m] = tabs[P.tw_ly + m];
     __syncthreads();
     const float sx = 1.0f / 8.0f, sy = 1.0f / 8.0f;
     for (int e = threadIdx.x; e < 8 * Lx; e += kThreads) {
@@ -1705,24 +1704,17 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
       for (int k = 0; k < 8; k++) Xc[t * 8 + k] = v[k];
     }
     __syncthreads();
-    // E8[s][k] = e^{2 pi i kt s / Ly}: the row factors of output row s
-    float2* E8 = Xc + Lx * 8;
-    for (int e = threadIdx.x; e < Ly * 8; e += kThreads) {
-      const int s = e >> 3, k = e & 7;
-      E8[e] = sEy[((k < 4 ? k : k - 8) * s) & (Ly - 1)];
-    }
-    __syncthreads();
-    // each thread owns one column t (its 8 spectrum values in registers) and
-    // walks the rows s in increasing order; lanes of a wave share s, so the E8
-    // reads broadcast.  The running maximum keeps the first index (strict >)
+    // E8[s][k] = e^{2 pi i kt s / Ly}, the row factors of output row s, come
+    // from the table in global memory.  Each thread owns one column t (its 8
+    // spectrum values in registers) and walks the rows s in increasing order;
+    // when Lx is a multiple of 64 the lanes of a wave share s and the factors
+    // are scalar loads.  The running maximum keeps the first index (strict >)
     // and skips values <= 0 and NaN, as the peak key does.
+    const float2* E8 = tabs + P.e8;
     float bestv = 0.0f;
     unsigned besti = 0u;
     const int per = kThreads >= Lx ? kThreads / Lx : 1;
-    for (int t0 = 0; t0 < Lx; t0 += kThreads) {
-      const int t = t0 + (per > 1 ? (int)threadIdx.x % Lx : (int)threadIdx.x);
-      const int s0 = per > 1 ? (int)threadIdx.x / Lx : 0;
-      if (t >= Lx) continue;
+    auto scan = [&](int s0, int t) {
       float2 X[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) X[k] = Xc[t * 8 + k];
@@ -1740,6 +1732,13 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
           besti = (unsigned)(s * Lx + t);
         }
       }
+    };
+    for (int t0 = 0; t0 < Lx; t0 += kThreads) {
+      const int t = t0 + (per > 1 ? (int)threadIdx.x % Lx : (int)threadIdx.x);
+      if (t >= Lx) continue;
+      const int s0 = per > 1 ? (int)threadIdx.x / Lx : 0;
+      if (per == 1 || Lx % 64 == 0) scan(__builtin_amdgcn_readfirstlane(s0), t);   // s uniform over the wave
+      else scan(s0, t);
     }
     unsigned long long best = block_max_u64(peak_key(bestv, besti), redk);
     float bv;
@@ -2683,6 +2682,9 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   };
   d.mk_x = add_mask(d.NX, d.xs);
   d.mk_y = add_mask(d.NY, d.ys);
+  d.e8 = (int)tab.size();                 // E8[s][k] = e^{2 pi i k~ s / Ly}, k~ = k (k < 4) or k - 8
+  for (int s = 0; s < d.Ly; s++)
+    for (int k = 0; k < 8; k++) tab.push_back(twf(d.Ly, (long)(((k < 4 ? k : k - 8) * s) & (d.Ly - 1))));
   d.phi = (int)tab.size();
   const int M = d.ri * d.NX;
   // the strip kernel stores phi[0..npx/2] only and rebuilds phi[k] = conj(phi[npx - k])
@@ -3123,7 +3125,7 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   });
   }
   if (rc) return rc;
-  const size_t sh5 = d.interp_flag ? ((size_t)d.Lx * 16 + (size_t)d.Ly * 8) * sizeof(float2) : 16;
+  const size_t sh5 = d.interp_flag ? (size_t)d.Lx * 16 * sizeof(float2) : 16;
   return launch_timed(c, timing, "k_finish", [&] {
     hipLaunchKernelGGL(k_finish, dim3(nb), dim3(kThreads), sh5, st, d, (const float*)c->amp1s,
                        (const float*)c->amp2s, d_rb, 0LL, dev_out ? d_perm : (const int*)nullptr,
@@ -3284,7 +3286,7 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
     rc = time_search(c, d_win, nb, timing);
     if (rc) return rc;
   }
-  const size_t sh5 = d.interp_flag ? ((size_t)d.Lx * 16 + (size_t)d.Ly * 8) * sizeof(float2) : 16;
+  const size_t sh5 = d.interp_flag ? (size_t)d.Lx * 16 * sizeof(float2) : 16;
   rc = launch_timed(c, timing, "k_finish", [&] {
     hipLaunchKernelGGL(k_finish, dim3(nb), dim3(kThreads), sh5, c->stream, d, (const float*)c->amp,
                        (const float*)c->amp + d.N, (const int*)nullptr, (long long)2 * d.N, (const int*)nullptr,
