@@ -65,6 +65,8 @@ CASES = [
     (8, 64, 2, 16, 1),
     (64, 64, 4, 16, 1),
     (64, 64, 1, 16, 0),
+    (32, 32, 2, 2, 1),          # high-res grid of 16 columns (rows not uniform over a wave)
+    (64, 64, 2, 32, 1),         # high-res grid of 256 columns (one column per thread)
 ]
 
 
