@@ -162,11 +162,13 @@ def test_windows_outside_image_read_zero(xg, orc, pair):
     _assert_bit_exact(got, ref, "edge windows")
 
 
-def test_real_float_format(xg, orc, pair):
+@pytest.mark.parametrize("xs,corr", [(32, 2), (64, 2), (128, 2), (32, 0), (16, 2)],
+                         ids=["strips128", "strips256", "strips512", "time_strips128", "per_patch64"])
+def test_real_float_format(xg, orc, pair, xs, corr):
     m, s = pair
     mf = np.ascontiguousarray(m[..., 0].astype(np.float32))
     sf = np.ascontiguousarray(s[..., 0].astype(np.float32))
-    p = xg.make_params(32, 32, format=xg.FLOAT32)
+    p = xg.make_params(xs, xs, format=xg.FLOAT32, corr_flag=corr)
     w, _, _ = _windows(xg, p, 4, 4)
     with xg.Engine(p) as eng:
         eng.set_images(mf, sf)
