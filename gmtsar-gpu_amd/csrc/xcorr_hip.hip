@@ -1020,10 +1020,10 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
   for (int k = threadIdx.x; k < 240; k += kThreads) twl[k] = tabs[P.tw_ny + (k % 15 + 1) * (k / 15)];
   const int p = blockIdx.x % pairs, b = blockIdx.x / pairs;
   const size_t row0 = rb ? (size_t)rb[b] : (size_t)b * 256;
-  auto zrow = [&](int y) -> const float2* {
-    const float2* base = (rb && !(y >= P.ys && y < 256 - P.ys)) ? Zu : Z;
-    return base + (row0 + y) * NX;
-  };
+  // npy = 256 gives ys = 64: row y0 + 16 k (y0 < 16) is masked iff 4 <= k < 12
+  const float2* zm0 = Z + (row0 + (threadIdx.x >> 4)) * NX;
+  const float2* zu0 = (rb ? Zu : Z) + (row0 + (threadIdx.x >> 4)) * NX;
+  auto zrow = [&](int k) -> const float2* { return (k >= 4 && k < 12 ? zm0 : zu0) + (size_t)(16 * k) * NX; };
   const int kxT = 16 * p + g;
   const int kxM = (p == 0 && g == 0) ? NX / 2 : NX - 16 * p - g;
   const bool self = (p == 0 && g == 0);                      // columns 0 and npx/2 mirror onto themselves
@@ -1041,7 +1041,7 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
     float2 rT[16], rM[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-      const float2* zr = zrow(y0 + 16 * k);
+      const float2* zr = zrow(k);
       rT[k] = zr[kxt];
       // rows y0 + 16 k outside [64, 192) are unmasked: their spectrum is
       // Hermitian, so column npx - kx is conj(column kx) (npx/2 is stored)
@@ -1055,7 +1055,7 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
 #pragma unroll
     for (int k = 0; k < 16; k++) {
       const int y = y0 + 16 * k;
-      T[y * kT256Pitch + c] = demean_row_bin(rT[k], kxt, y >= P.ys && y < 256 - P.ys, m1NX, m2f, mkt);
+      T[y * kT256Pitch + c] = demean_row_bin(rT[k], kxt, k >= 4 && k < 12, m1NX, m2f, mkt);
     }
     __syncthreads();
 #pragma unroll
@@ -1064,7 +1064,7 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
 #pragma unroll
     for (int k = 0; k < 16; k++) {
       const int y = y0 + 16 * k;
-      T[y * kT256Pitch + c] = demean_row_bin(rM[k], kxm, y >= P.ys && y < 256 - P.ys, m1NX, m2f, mkm);
+      T[y * kT256Pitch + c] = demean_row_bin(rM[k], kxm, k >= 4 && k < 12, m1NX, m2f, mkm);
     }
   }
   __syncthreads();
