@@ -1240,25 +1240,28 @@ __global__ void __launch_bounds__(kThreads) k_colpass(DevParams P, const float2*
   static_assert(kThreads % TC == 0 && (N * TC) % kThreads == 0, "column tile");
   constexpr int LS = kThreads / TC, LK = N * TC / kThreads;
   const int i0 = threadIdx.x % TC, y0 = threadIdx.x / TC;
+  // square windows: ys = N / 4, so row y0 + LS k (y0 < LS) is masked iff
+  // LK / 4 <= k < 3 LK / 4 (known at compile time)
+  static_assert(LS * LK == N, "column tile rows");
+  const float2* zm0 = Z + (row0 + y0) * N;
+  const float2* zu0 = (rb ? Zu : Z) + (row0 + y0) * N;
   for (int half = 0; half < 2; half++) {
     const int kx = kx_of(i0, half == 1);
-    const int ku = kx <= N / 2 ? kx : N - kx;
+    const int ku = rb && kx > N / 2 ? N - kx : kx;      // unmasked strip rows: bins <= N/2 only
+    const bool cj = rb && kx > N / 2;
     const float2 mk = tabs[P.mk_x + kx];
     float2 zr[LK];
 #pragma unroll
     for (int k = 0; k < LK; k++) {
-      const int y = y0 + LS * k;
-      const bool ym = y >= P.ys && y < N - P.ys;
-      const bool u = rb && !ym;
-      const float2* base = u ? Zu : Z;
-      zr[k] = base[(row0 + y) * N + (u ? ku : kx)];
-      if (u && kx > N / 2) zr[k].y = -zr[k].y;
+      const bool ym = k >= LK / 4 && k < 3 * LK / 4;
+      zr[k] = ym ? zm0[(size_t)(LS * k) * N + kx] : zu0[(size_t)(LS * k) * N + ku];
+      if (!ym && cj) zr[k].y = -zr[k].y;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < LK; k++) {
       const int y = y0 + LS * k;
-      T[y * PITCH + i0] = demean_row_bin(zr[k], kx, y >= P.ys && y < N - P.ys, m1NX, m2f, mk);
+      T[y * PITCH + i0] = demean_row_bin(zr[k], kx, k >= LK / 4 && k < 3 * LK / 4, m1NX, m2f, mk);
     }
     __syncthreads();
 #pragma unroll
