@@ -279,3 +279,38 @@ def test_zero_windows_are_defined(orc):
     for prec in (32, 64):
         rec, _ = orc.patch(p, z, z, precision=prec)
         assert rec["corr"] == 0.0 and rec["ip"] == 0 and rec["jp"] == 0
+
+
+def test_integer_peaks_fp32_vs_fp64_outside_near_ties(orc, synth):
+    """The FP32 surface (the GPU's arithmetic) stays within a stated error
+    bound of the FP64 surface, and wherever the FP64 surface's top two values
+    are separated by more than twice that bound the integer peaks agree; a
+    flip is only possible on a near-tie.  Low coherence (0.25) makes near-ties
+    common enough to show up; their count is printed."""
+    xs = ys = 16
+    npx, npy = 4 * xs, 4 * ys
+    m, s = synth.constant_shift_pair(512, 640, 1.7, -2.9, 0.25, seed=41, noise_seed=42)
+    p = orc.make_params(xs, ys)
+    bound = 2.0 ** -16                             # |surf32 - surf64| / max(surf64)
+    worst, near, flips = 0.0, 0, 0
+    for my0 in range(0, 640 - npy, 37):
+        for mx0 in range(0, 512 - npx, 41):
+            c = lambda a: (a[my0:my0 + npy, mx0:mx0 + npx, 0].astype(np.float32)
+                           + 1j * a[my0:my0 + npy, mx0:mx0 + npx, 1].astype(np.float32))
+            mw, sw = c(m), c(s)
+            r32, d32 = orc.patch(p, mw, sw, precision=32, dump=True)
+            r64, d64 = orc.patch(p, mw, sw, precision=64, dump=True)
+            s32, s64 = d32["surf"].astype(np.float64), d64["surf"]
+            top = s64.max()
+            worst = max(worst, np.abs(s32 - s64).max() / top)
+            v = np.sort(s64.ravel())
+            margin = (v[-1] - v[-2]) / top
+            same = (r32["ip"], r32["jp"]) == (r64["ip"], r64["jp"])
+            if margin <= 2 * bound:
+                near += 1
+                flips += not same
+            else:
+                assert same, "peak differs outside a near-tie at (%d, %d), margin %.3g" % (mx0, my0, margin)
+    print("max relative surface error %.3g (bound %.3g); near-ties %d, flips among them %d" % (worst, bound, near,
+                                                                                           flips))
+    assert worst < bound
