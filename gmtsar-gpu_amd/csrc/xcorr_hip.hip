@@ -2080,6 +2080,10 @@ __device__ __forceinline__ v4i_t ld16_unaligned(const signed char* row, int o) {
 
 constexpr int kLimbPitch = 144;                   // bytes per limb row of the window (128 + 16 for the tail loads)
 constexpr int kBPad = 32, kBRows = 128;           // template limb rows: 32 zero rows, 64 rows, 32 zero rows
+// Byte offset of 16-byte chunk c of template limb row `row`: the chunks of a
+// row are rotated by row / 4, so the 16 lanes of a group, which read chunk c
+// of 16 consecutive rows, hit 16 different 16-byte bank groups.
+__device__ __forceinline__ int bl_chunk(int row, int c) { return 16 * ((c + (row >> 2)) & 3); }
 
 // The wave's 2 x 2 tiles of 16 x 16 lags (dx0 + 16 i + .., dy0 + 16 j + ..):
 // for every master row r that meets them, the Toeplitz rows A[r][x + dx]
@@ -2100,7 +2104,10 @@ __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed
       // the rows outside [0, 64) are the zero rows around the template
       const int y = r - (dy0 + 16 * j + m);
 #pragma unroll
-      for (int k = 0; k < KB; k++) B[j][k] = *(const v4i_t*)(Bl + (k * kBRows + kBPad + y) * 64 + 16 * g);
+      for (int k = 0; k < KB; k++) {
+        const int row = k * kBRows + kBPad + y;
+        B[j][k] = *(const v4i_t*)(Bl + row * 64 + bl_chunk(row, g));
+      }
     }
   };
   // lag-row tile j meets master rows dy0 + 16 j .. dy0 + 16 j + 78 only: the
@@ -2204,7 +2211,10 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
     unsigned w[3];
     pack4(fb[k], m2f, w, hib, &b2);
 #pragma unroll
-    for (int l = 0; l < 3; l++) *(unsigned*)(Bl + (l * kBRows + kBPad + y) * 64 + x) = w[l];
+    for (int l = 0; l < 3; l++) {
+      const int row = l * kBRows + kBPad + y;
+      *(unsigned*)(Bl + row * 64 + bl_chunk(row, x >> 4) + (x & 15)) = w[l];
+    }
   }
   for (int e = threadIdx.x; e < 3 * 64 * 4; e += kThreads) {       // the zero rows, 16 bytes a store
     const int l = e / 256, q = e % 256, row = q < 128 ? q >> 2 : kBPad + 64 + ((q - 128) >> 2);
