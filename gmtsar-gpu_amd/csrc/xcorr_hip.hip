@@ -3654,29 +3654,34 @@ extern "C" int xcorr_hip_autotune(xcorr_hip_ctx* c, const xcorr_hip_window* win,
     }
     return 0;
   };
-  // coordinate descent: threads, row tile, column tile, radix plans, batch
-  const bool generic = !c->strips;         // strip contexts: batch only (the default plans run the strips)
-  if (!rc && generic)
-    for (int th : {128, 256, 512}) {
-      xcorr_hip_tuning t = best;
-      t.threads = th;
-      if (th != best.threads && (rc = consider(t))) break;
-    }
-  if (!rc && generic)
-    for (int G = 1; G <= d.NY && !rc; G *= 2) {
-      xcorr_hip_tuning t = best;
-      t.rows_per_block = G;
-      const size_t nel = (size_t)G * d.NX;
-      if (G != best.rows_per_block && nel >= 256 && lds_rows(d, G) <= kMaxDynLds) rc = consider(t);
-    }
-  if (!rc && generic)
-    for (int C = 1; C <= d.NX && !rc; C *= 2) {
-      xcorr_hip_tuning t = best;
-      t.cols_per_block = C;
-      const size_t nel = (size_t)C * d.NY;
-      if (C != best.cols_per_block && nel >= 256 && lds_cols(d, C) <= kMaxDynLds) rc = consider(t);
-    }
-  if (!rc && generic && d.corr_flag == XCORR_HIP_FREQ && (flags & XCORR_HIP_TUNE_RADIX)) {
+  // coordinate descent: threads, row tile, column tile, radix plans, batch.
+  // Strip contexts start from the strips (default plans); a radix plan
+  // candidate runs the per-patch kernels, and only if one of them wins are the
+  // launch settings of those kernels tuned.
+  auto tune_launch = [&]() {
+    if (!rc)
+      for (int th : {128, 256, 512}) {
+        xcorr_hip_tuning t = best;
+        t.threads = th;
+        if (th != best.threads && (rc = consider(t))) break;
+      }
+    if (!rc)
+      for (int G = 1; G <= d.NY && !rc; G *= 2) {
+        xcorr_hip_tuning t = best;
+        t.rows_per_block = G;
+        const size_t nel = (size_t)G * d.NX;
+        if (G != best.rows_per_block && nel >= 256 && lds_rows(d, G) <= kMaxDynLds) rc = consider(t);
+      }
+    if (!rc)
+      for (int C = 1; C <= d.NX && !rc; C *= 2) {
+        xcorr_hip_tuning t = best;
+        t.cols_per_block = C;
+        const size_t nel = (size_t)C * d.NY;
+        if (C != best.cols_per_block && nel >= 256 && lds_cols(d, C) <= kMaxDynLds) rc = consider(t);
+      }
+  };
+  auto tune_radix = [&]() {
+    if (rc || d.corr_flag != XCORR_HIP_FREQ || !(flags & XCORR_HIP_TUNE_RADIX)) return;
     std::vector<FftPlan> cand;
     plan_candidates(d.NX, cand);
     for (const FftPlan& pl : cand) {
@@ -3695,6 +3700,14 @@ extern "C" int xcorr_hip_autotune(xcorr_hip_ctx* c, const xcorr_hip_window* win,
         if (!same_plan(pl, ints_to_plan(best.radix_y))) rc = consider(t);
       }
     }
+  };
+  if (!c->strips) {
+    tune_launch();
+    tune_radix();
+  } else {
+    tune_radix();
+    if (!same_plan(ints_to_plan(best.radix_x), plan_for(d.NX)) || !same_plan(ints_to_plan(best.radix_y), plan_for(d.NY)))
+      tune_launch();
   }
   if (!rc)
     for (int b = c->max_batch / 2; b >= 512 && !rc; b /= 2) {
