@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: xcorr correlation patches/s on one MI355X.
 
-Workload: a 10800 (range) x 27648 (azimuth) complex-int16
+Workload (the headline strip workload): a 10800 (range) x 27648 (azimuth) complex-int16
 SLC pair generated on the GPU (speckle model, offset field dx = 2.1 + 1e-4 x,
 dy = -7.6 + 3e-4 y, coherence per 512 x 512 block uniform in [0.3, 0.95]),
 locations from get_locations with -nx 100 -ny 400 -xsearch 64 -ysearch 64
@@ -47,7 +47,7 @@ def committed_traffic():
     """Per-launch HBM bytes per kernel from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (the JSON line under "Per-launch
     traffic"), or {} when it is absent."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_xcorr_cfg2.md")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_xcorr_strip.md")
     try:
         lines = open(path).read().splitlines()
     except OSError:
@@ -68,7 +68,7 @@ def main():
     ap.add_argument("--batch", type=int, default=8192, help="locations per engine batch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--small", action="store_true", help="config 1 (1024 x 2048, 1000 patches) instead")
+    ap.add_argument("--small", action="store_true", help="the small workload (1024 x 2048, 1000 patches) instead")
     ap.add_argument("--workload", default="strip", choices=("strip", "small", "batched", "proposal", "dense"),
                     help="strip: 10800 x 27648 pair, 40,000 patches (default); small: 1024 x 2048, 1000 patches; "
                          "batched: 64 uneven pairs with mixed patch sizes in one graph launch; proposal: the "
@@ -210,7 +210,7 @@ def main():
 
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/hbm_traffic_xcorr_cfg2.md"
+                "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/hbm_traffic_xcorr_strip.md"
                                    if traffic and not args.traffic else None),
                 "kernel": dom_name, "launch_ms": round(per_launch_ms, 4),
                 "patches_per_launch": patches_per_launch, "copy_peak_GBs": round(copy_gbs, 1),

@@ -40,8 +40,8 @@ def to_cint16(z, scale):
 
 
 def constant_shift_pair(n_range, n_azimuth, dx, dy, coherence, seed=1, noise_seed=2):
-    """Master/aligned pair with a constant (dx, dy) shift (BASELINE config 1 uses
-    1024 x 2048, dx = 3.37, dy = -12.81, coherence 0.8)."""
+    """Master/aligned pair with a constant (dx, dy) shift (the small parity
+    workload uses 1024 x 2048, dx = 3.37, dy = -12.81, coherence 0.8)."""
     rng = np.random.default_rng(seed)
     m = speckle((n_azimuth, n_range), rng)
     s = fourier_shift(m, dx, dy)
@@ -71,7 +71,7 @@ def write_prm(path, slc_file, num_rng_bins, num_lines, rshift=0, ashift=0, prf=1
 
 def strip_pair_torch(n_range, n_azimuth, dx_fn, dy_fn, coh_range=(0.3, 0.95), block=512, margin=64,
                      seed=7, device="cuda"):
-    """Full-strip pair generated on the GPU (10800 x 27648,
+    """Full-strip pair generated on the GPU (the strip workload: 10800 x 27648,
     dx = 2.1 + 1e-4 x, dy = -7.6 + 3e-4 y, coherence per 512 x 512 block uniform
     in [0.3, 0.95]).  Each block is Fourier-shifted by the offset at its centre
     (per-block Fourier resampling with a `margin`-sample apron).  Returns

@@ -127,7 +127,7 @@ def test_tolerance_vs_fp64_oracle(xg, orc, pair):
 
 
 def test_full_config1_recovers_shift(xg, orc, pair):
-    """1024 x 2048, -nx 20 -ny 50 -xsearch 64 -ysearch 64 (1000 patches)."""
+    """The small workload: 1024 x 2048, -nx 20 -ny 50 -xsearch 64 -ysearch 64 (1000 patches)."""
     m, s = pair
     p = xg.make_params(64, 64)
     w, x, y = _windows(xg, p, 20, 50)
@@ -144,7 +144,7 @@ def test_full_config1_recovers_shift(xg, orc, pair):
     # bit-exact against the FP32 restatement on a sample of 40 patches
     idx = np.arange(0, 1000, 25)
     ref = orc.run(_oracle_params(orc, p), m, s, _locs(w[idx]), precision=32)
-    _assert_bit_exact(got[idx], ref, "config 1 sample")
+    _assert_bit_exact(got[idx], ref, "small workload sample")
 
 
 def test_windows_outside_image_read_zero(xg, orc, pair):

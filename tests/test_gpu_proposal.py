@@ -82,7 +82,7 @@ def test_proposal_paths_are_plain_xcorr_runs(xg, synth, env):
             assert np.array_equal(cdx, want)
         assert np.array_equal(dx[k0], pdx[k0]) and np.array_equal(dy[k0], pdy[k0])
         err = np.sqrt(np.mean(np.concatenate([dx - pdx, dy - pdy]) ** 2))
-        print("config-1 pair: shrunken %.1f %%, rmse vs plain %.4f px" % (100 * (path > 0).mean(), err))
+        print("1024 x 2048 pair: shrunken %.1f %%, rmse vs plain %.4f px" % (100 * (path > 0).mean(), err))
         # verification: windows whose shrunken peak lies on the lag-window border
         # or whose correlation is low are redone by the plain search (path 3)
         pr.verify, pr.corr_min = True, float(np.median(res["corr"][path > 0]))
