@@ -127,6 +127,16 @@ __device__ __forceinline__ double patch_mean(unsigned long long s, int N) {
   return (double)(long long)s * (1.0 / 1048576.0) / (double)N;
 }
 
+// Work item of this block for a grid whose size is a multiple of 8 (items
+// beyond the real count are padding the caller skips).  Workgroups go to the
+// 8 XCDs round-robin in blockIdx order; block i takes item
+// (i mod 8) (grid / 8) + i / 8, so each XCD runs one contiguous range of
+// items in order, and neighbouring items (patches of one strip, whose rows
+// overlap) meet in one L2.
+__device__ __forceinline__ int xcd_item() {
+  return (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+}
+
 // --------------------------------------------------------------------------
 // stage (a): gather + range interpolation + amplitude + fixed-point sums
 // xcorr.cu:237-292, fft_interpolate_routines.cu:76-113, conv_ampl_demean.cu:98-108, 703-726
@@ -1106,7 +1116,9 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
   }
   __syncthreads();
   fft256_group_lds<+1>(pT, j, T + g * kT256Tile, twl);
-  if (p == 0) fft256_group_lds<+1>(vM, j, T + g * kT256Tile, twl);
+  // the M column npx/2 alone (group 0 of block 0): the other lanes of its wave
+  // stay masked and waves 1..3 skip the transform
+  if (self) fft256_group_lds<+1>(vM, j, T + g * kT256Tile, twl);
   __syncthreads();                                           // every group is done with its tile
   // lag rows 64..191 (r = 4..11)
 #pragma unroll
@@ -1222,7 +1234,7 @@ __global__ void __launch_bounds__(kThreads) k_colpass(DevParams P, const float2*
   __shared__ float2 sx[N];
   float2* const ftiles = T;
   const int tiles = N / (2 * TC);
-  const int wi = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+  const int wi = xcd_item();
   if (wi >= nwork) return;
   const int p = wi % tiles, b = wi / tiles;
   const int g = threadIdx.x / G, j = threadIdx.x % G;
@@ -1318,7 +1330,7 @@ __global__ void __launch_bounds__(kThreads) k_colpass(DevParams P, const float2*
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < CPG; c++) F::template run<+1>(pT[c], j, ftile, t);
-  if (p == 0) F::template run<+1>(vM[0], j, ftile, t);
+  if (holder) F::template run<+1>(vM[0], j, ftile, t);   // column N/2 alone; the other groups skip it
   __syncthreads();                                 // transform tiles (aliasing T) are done
   // lag rows N/4 .. 3N/4 - 1 (r = V/4 .. 3V/4 - 1) through LDS, written as rows of D
 #pragma unroll
@@ -1555,6 +1567,8 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf(DevParams P, const fl
 // --------------------------------------------------------------------------
 // Amplitudes of patch b: rows y of A1 / A2 at (rb[b] + y) npx (strips) or
 // b pstride + y npx.  The result goes to out[perm[b]] (or out[b]).
+// Patches are mapped to blocks by xcd_item: neighbours in a strip, whose
+// amplitude rows overlap, meet in one L2.
 __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* __restrict__ A1,
                                                      const float* __restrict__ A2, const int* __restrict__ rb,
                                                      long long pstride, const int* __restrict__ perm,
@@ -1566,7 +1580,7 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
                                                      const int* __restrict__ tpeak,
                                                      const double* __restrict__ tcave,
                                                      xcorr_hip_result* __restrict__ out,
-                                                     const float2* __restrict__ tabs) {
+                                                     const float2* __restrict__ tabs, int nb) {
   extern __shared__ float2 smem[];
   __shared__ double redd[kThreads / 64];
   __shared__ unsigned long long redk[kThreads / 64];
@@ -1574,7 +1588,8 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
   __shared__ float md[64];
   __shared__ float2 Xr[64];
   __shared__ double s_msc;
-  const int b = blockIdx.x;
+  const int b = xcd_item();
+  if (b >= nb) return;
   const int NX = P.NX, nxc = P.nxc, nyc = P.nyc;
   const bool freq = P.corr_flag == XCORR_HIP_FREQ;
   xcorr_hip_result res;
@@ -2012,9 +2027,10 @@ __device__ __forceinline__ long long wave_lane63_i64(long long x) {
 __global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const float* __restrict__ A1,
                                                             const int* __restrict__ rb,
                                                             const unsigned long long* __restrict__ sums,
-                                                            double* __restrict__ aa) {
+                                                            double* __restrict__ aa, int nb) {
   __shared__ long long E[127 * 64];
-  const int b = blockIdx.x;
+  const int b = xcd_item();                     // neighbours in a strip share an XCD (and its L2)
+  if (b >= nb) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float m1f = (float)patch_mean(sums[2 * b], P.N);
   const float* a1 = rb ? A1 + (size_t)rb[b] * 128 : A1 + (size_t)(2 * b) * P.N;
@@ -2147,12 +2163,13 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
                                                              const unsigned long long* __restrict__ sums,
                                                              const double* __restrict__ aa,
                                                              double* __restrict__ surfd, int* __restrict__ tpeak,
-                                                             double* __restrict__ tcave) {
+                                                             double* __restrict__ tcave, int nb) {
   __shared__ __attribute__((aligned(16))) signed char Al[3 * 128 * kLimbPitch];
   __shared__ __attribute__((aligned(16))) signed char Bl[3 * kBRows * 64];
   __shared__ long long redl[kThreads / 64];
   __shared__ int sbig, shi[2];                    // values beyond the limbs; limb 2 nonzero (master, template)
-  const int b = blockIdx.x;
+  const int b = xcd_item();                     // neighbours in a strip share an XCD (and its L2)
+  if (b >= nb) return;
   const float m1f = (float)patch_mean(sums[2 * b], P.N);
   const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
   // amplitudes of patch b: rows of 128 from row base rb[b] of the strip lines, or per patch
@@ -3051,10 +3068,10 @@ static void launch_colpass(xcorr_hip_ctx* c, const DevParams& d, hipStream_t st,
 // per patch (rb null: A1 = A2 = amp) or on strip lines from row bases rb.
 static void launch_time_i8(xcorr_hip_ctx* c, const DevParams& d, hipStream_t st, const float* A1, const float* A2,
                            const int* rb, int nb) {
-  hipLaunchKernelGGL(k_time_energy32, dim3(nb), dim3(kThreads), 0, st, d, A1, rb, (const unsigned long long*)c->sums,
-                     c->tcaa);
-  hipLaunchKernelGGL(k_time_search_i8, dim3(nb), dim3(kThreads), 0, st, d, A1, A2, rb,
-                     (const unsigned long long*)c->sums, (const double*)c->tcaa, c->surfd, c->tpeak, c->tcave);
+  hipLaunchKernelGGL(k_time_energy32, dim3((nb + 7) & ~7), dim3(kThreads), 0, st, d, A1, rb,
+                     (const unsigned long long*)c->sums, c->tcaa, nb);
+  hipLaunchKernelGGL(k_time_search_i8, dim3((nb + 7) & ~7), dim3(kThreads), 0, st, d, A1, A2, rb,
+                     (const unsigned long long*)c->sums, (const double*)c->tcaa, c->surfd, c->tpeak, c->tcave, nb);
 }
 
 // Kernels of one strip batch whose plan is in device memory at d_plan, on
@@ -3140,11 +3157,11 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   if (rc) return rc;
   const size_t sh5 = d.interp_flag ? (size_t)d.Lx * 16 * sizeof(float2) : 16;
   return launch_timed(c, timing, "k_finish", [&] {
-    hipLaunchKernelGGL(k_finish, dim3(nb), dim3(kThreads), sh5, st, d, (const float*)c->amp1s,
+    hipLaunchKernelGGL(k_finish, dim3((nb + 7) & ~7), dim3(kThreads), sh5, st, d, (const float*)c->amp1s,
                        (const float*)c->amp2s, d_rb, 0LL, dev_out ? d_perm : (const int*)nullptr,
                        (const unsigned long long*)c->sums, (const float*)c->surf, (const double*)c->surfd,
                        (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)c->tpeak,
-                       (const double*)c->tcave, dev_out ? out : c->d_out, (const float2*)c->tabs);
+                       (const double*)c->tcave, dev_out ? out : c->d_out, (const float2*)c->tabs, nb);
   });
 }
 
@@ -3301,11 +3318,11 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
   }
   const size_t sh5 = d.interp_flag ? (size_t)d.Lx * 16 * sizeof(float2) : 16;
   rc = launch_timed(c, timing, "k_finish", [&] {
-    hipLaunchKernelGGL(k_finish, dim3(nb), dim3(kThreads), sh5, c->stream, d, (const float*)c->amp,
+    hipLaunchKernelGGL(k_finish, dim3((nb + 7) & ~7), dim3(kThreads), sh5, c->stream, d, (const float*)c->amp,
                        (const float*)c->amp + d.N, (const int*)nullptr, (long long)2 * d.N, (const int*)nullptr,
                        (const unsigned long long*)c->sums, (const float*)c->surf, (const double*)c->surfd,
                        (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)c->tpeak,
-                       (const double*)c->tcave, d_out, (const float2*)c->tabs);
+                       (const double*)c->tcave, d_out, (const float2*)c->tabs, nb);
   });
   return rc;
 }
