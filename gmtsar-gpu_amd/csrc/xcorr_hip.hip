@@ -1020,7 +1020,8 @@ __device__ __forceinline__ float2 demean_row_bin(float2 v, int kx, bool ym, floa
 __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const float2* __restrict__ Z,
                                                          const float2* __restrict__ Zu, const int* __restrict__ rb,
                                                          const unsigned long long* __restrict__ sums,
-                                                         float2* __restrict__ D, const float2* __restrict__ tabs) {
+                                                         float2* __restrict__ D, const float2* __restrict__ tabs,
+                                                         int nwork) {
   __shared__ float2 T[256 * kT256Pitch];
   __shared__ float2 sx[256];                                 // spectrum of a self-mirrored column
   __shared__ float2 twl[240];                                // pass-2 twiddles w256^(r j), twl[15 j + r - 1]
@@ -1028,7 +1029,12 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
   const int NX = P.NX;
   const int pairs = NX >> 5;
   for (int k = threadIdx.x; k < 240; k += kThreads) twl[k] = tabs[P.tw_ny + (k % 15 + 1) * (k / 15)];
-  const int p = blockIdx.x % pairs, b = blockIdx.x / pairs;
+  // work items (patch, column tile) in xcd_item order keep the tiles of a
+  // patch, whose mirrored column segments straddle each other's cache lines,
+  // and the next patches of a strip, which share rows, in one L2
+  const int wi = xcd_item();
+  if (wi >= nwork) return;
+  const int p = wi % pairs, b = wi / pairs;
   const size_t row0 = rb ? (size_t)rb[b] : (size_t)b * 256;
   // npy = 256 gives ys = 64: row y0 + 16 k (y0 < 16) is masked iff 4 <= k < 12
   const float2* zm0 = Z + (row0 + (threadIdx.x >> 4)) * NX;
@@ -3134,8 +3140,9 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   } else {
   rc = launch_timed(c, timing, d.NX == 256 ? "k_colpass256" : "k_colpass", [&] {
     if (d.NX == 256)
-      hipLaunchKernelGGL(k_colpass256, dim3(nb * (d.NX / 32)), dim3(kThreads), 0, st, d, (const float2*)c->Z,
-                         (const float2*)c->Zu, d_rb, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs);
+      hipLaunchKernelGGL(k_colpass256, dim3((nb * 8 + 7) & ~7), dim3(kThreads), 0, st, d, (const float2*)c->Z,
+                         (const float2*)c->Zu, d_rb, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs,
+                         nb * 8);
     else if (d.NX == 128)
       launch_colpass<128>(c, d, st, d_rb, nb);
     else
@@ -3275,9 +3282,9 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
       });
     } else if (col256 && d.NX >= 32) {
       rc = launch_timed(c, timing, "k_colpass256", [&] {
-        hipLaunchKernelGGL(k_colpass256, dim3(nb * (d.NX / 32)), dim3(kThreads), 0, c->stream, d,
+        hipLaunchKernelGGL(k_colpass256, dim3((nb * (d.NX / 32) + 7) & ~7), dim3(kThreads), 0, c->stream, d,
                            (const float2*)c->Z, (const float2*)nullptr, (const int*)nullptr,
-                           (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs);
+                           (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs, nb * (d.NX / 32));
       });
     } else {
       rc = launch_timed(c, timing, "k_colfft", [&] {
