@@ -641,14 +641,46 @@ __global__ void k_need_mark(DevParams P, const int* __restrict__ rb, unsigned ch
   }
 }
 
-// Sums over a group of G consecutive lanes (G a power of two <= 64).
+// A 64-bit value moved between lanes by DPP control CTRL (rows in ROWS);
+// lanes whose source lies outside their row of 16 get 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ long long dpp_i64(long long v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)v, CTRL, ROWS, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, ROWS, 0xf, true);
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  return __builtin_bit_cast(double, dpp_i64<CTRL, 0xf>(__builtin_bit_cast(long long, v)));
+}
+
+// Sums over a group of G consecutive lanes (G a power of two <= 64); the total
+// is valid in the group's last lane.  A group of 16 is one DPP row: an
+// inclusive scan by row shifts of 1, 2, 4 and 8 (register moves, no LDS round
+// trip) leaves the total in lane 15; other G take an xor butterfly (every
+// lane gets the total).  The summands are integers whose partial sums are
+// exact, so the order does not change the result.
 template <int G>
 __device__ __forceinline__ long long group_sum_i64(long long v) {
+  if (G == 16) {
+    v += dpp_i64<0x111, 0xf>(v);                   // row_shr:1
+    v += dpp_i64<0x112, 0xf>(v);                   // row_shr:2
+    v += dpp_i64<0x114, 0xf>(v);                   // row_shr:4
+    v += dpp_i64<0x118, 0xf>(v);                   // row_shr:8
+    return v;
+  }
   for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
 template <int G>
 __device__ __forceinline__ double group_sum_f64(double v) {
+  if (G == 16) {
+    v += dpp_f64<0x111>(v);
+    v += dpp_f64<0x112>(v);
+    v += dpp_f64<0x114>(v);
+    v += dpp_f64<0x118>(v);
+    return v;
+  }
   for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
@@ -731,7 +763,7 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
         s.add(oa[q]);
       }
       const long long st = s.group_total();
-      if (act && j == 0) lsum[2 * l] = st;
+      if (act && j == 15) lsum[2 * l] = st;
     }
     {
       float ea[8], oa[8];
@@ -747,7 +779,7 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
         s.add(oa[q]);
       }
       const long long st = s.group_total();
-      if (act && j == 0) lsum[2 * l + 1] = st;
+      if (act && j == 15) lsum[2 * l + 1] = st;
     }
     wave_sync();                                // the group's two lines are in LDS
     const float* m1 = (const float*)mr;
@@ -887,7 +919,7 @@ __global__ void __launch_bounds__(kThreads) k_stripA(DevParams P, const void* __
         s.add(oa[q]);
       }
       const long long st = s.group_total();
-      if (j == 0) lsum[2 * l] = st;
+      if (j == G - 1) lsum[2 * l] = st;
     }
     load_line_g<G, V, EDGE>(im_s, P.s_nx, P.s_ny, P.s_lo, P.s_hi, S.sy0 + i, S.sx0, j, u);
     interp_line<N, FMT>(u, j, tile, t, sph, ea, oa);
@@ -904,7 +936,7 @@ __global__ void __launch_bounds__(kThreads) k_stripA(DevParams P, const void* __
         s.add(oa[q]);
       }
       const long long st = s.group_total();
-      if (j == 0) lsum[2 * l + 1] = st;
+      if (j == G - 1) lsum[2 * l + 1] = st;
     }
     if (!ROWFFT) continue;                         // time modes: amplitudes and sums only
     wave_sync();                                   // the group's two lines are in LDS
@@ -2009,12 +2041,6 @@ typedef int v4i_t __attribute__((ext_vector_type(4)));
 // Inclusive prefix sum of an int64 over the 64 lanes of a wave with DPP lane
 // moves (row shifts by 1, 2, 4, 8 within rows of 16, then the row broadcasts
 // of lanes 15 and 31); exact in any order.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ long long dpp_i64(long long v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)v, CTRL, ROWS, 0xf, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, ROWS, 0xf, true);
-  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
 __device__ __forceinline__ long long wave_incl_scan_i64(long long x) {
   x += dpp_i64<0x111, 0xf>(x);                   // row_shr:1
   x += dpp_i64<0x112, 0xf>(x);                   // row_shr:2
