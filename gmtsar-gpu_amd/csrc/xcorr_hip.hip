@@ -545,7 +545,7 @@ __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __
 // lines starting at its row base.  The arithmetic of every value is that of
 // the per-patch path.
 // --------------------------------------------------------------------------
-constexpr int kStripBlockLines = 64;      // strip lines per block (16 groups x 4)
+constexpr int kStripBlockLines = 128;     // strip lines per block (16 groups x 8; 64 and 256 measured slower)
 
 // Raw samples x0 + j + 16 r of line gy (one dword per sample in both formats),
 // zero outside the image.  Branch-free: the address is clamped into the image
