@@ -123,6 +123,9 @@ __device__ __forceinline__ long long fxp20(float a) {
   return (long long)(((unsigned long long)(unsigned)h << 32) + (unsigned long long)(unsigned)l);
 }
 
+// four floats at a 4-byte aligned address (one global_load_dwordx4)
+typedef float f4a4 __attribute__((ext_vector_type(4), aligned(4)));
+
 __device__ __forceinline__ double patch_mean(unsigned long long s, int N) {
   return (double)(long long)s * (1.0 / 1048576.0) / (double)N;
 }
@@ -1656,7 +1659,38 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
     const float* arow = a1 + (size_t)(P.ys + ipeak) * NX + P.xs + jpeak;
     const float* brow = a2 + (size_t)P.ys * NX + P.xs;
     const int lg = __builtin_ctz(ncx);          // nx_corr is a power of two
-    for (int e0 = 0; e0 < nt; e0 += 8 * kThreads) {   // 8 independent loads per thread in flight
+    // rows of nx_corr >= 4 values: quads of consecutive values (one 16-byte
+    // load each, 4-byte aligned), 8 quads of each window per thread in flight
+    const int nq = ncx >= 4 ? nt >> 2 : 0;
+    for (int q0 = 0; q0 < nq; q0 += 8 * kThreads) {
+      f4a4 ra[8], rb8[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int q = q0 + k * kThreads + threadIdx.x;
+        ra[k] = f4a4{0.0f, 0.0f, 0.0f, 0.0f};
+        rb8[k] = f4a4{0.0f, 0.0f, 0.0f, 0.0f};
+        if (q < nq) {
+          const int e = q << 2;
+          const size_t o = (size_t)(e >> lg) * NX + (e & (ncx - 1));
+          ra[k] = *(const f4a4*)(arow + o);
+          rb8[k] = *(const f4a4*)(brow + o);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        if (q0 + k * kThreads + (int)threadIdx.x < nq) {
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            const double av = (double)(int)(ra[k][c] - m1f);
+            const double bv = (double)(int)(rb8[k][c] - m2f);
+            num = __builtin_fma(av, bv, num);
+            d1 = __builtin_fma(av, av, d1);
+            d2 = __builtin_fma(bv, bv, d2);
+          }
+        }
+      }
+    }
+    for (int e0 = nq > 0 ? nt : 0; e0 < nt; e0 += 8 * kThreads) {   // nx_corr 2: 8 loads per thread in flight
       float ra[8], rb8[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) {
