@@ -1197,7 +1197,8 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
   unsigned long long best = 0ull;
   double sum = 0.0;
   const int iters = (P.nyc + 15) >> 4;
-  // row loads run one iteration ahead of the transform
+  // one row of D per group and iteration (mirrored bins from 0 .. 128); a
+  // prefetch of the next row costs the fourth wave per SIMD and was slower
   auto load_row = [&](int row, float2 (&u)[16]) {
 #pragma unroll
     for (int r = 0; r < 16; r++) {
