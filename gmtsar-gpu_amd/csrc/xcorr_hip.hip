@@ -2558,7 +2558,6 @@ struct xcorr_hip_ctx {
   size_t plan_bytes = 0;
   std::vector<int> order;                 // windows sorted by strip key
   std::vector<xcorr_hip_window> hwin;     // host copy of device windows
-  std::vector<xcorr_hip_result> hres;     // batch results before the host scatter
   std::string err;
   std::vector<KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -2579,6 +2578,14 @@ struct xcorr_hip_ctx {
   int nt_rows = kThreads;                 // threads of the generic row / column kernels
   bool reg_paths = true;                  // register-resident kernels where the shape has them
   double* tcaa = nullptr;                 // -time, xsearch = ysearch = 32: master energies of every lag
+  // strip batches alternate between this context and a second one (own
+  // buffers and stream, created on the first call with more than one batch),
+  // so the kernels of one batch overlap those of the next
+  xcorr_hip_ctx* twin = nullptr;
+  bool twin_failed = false;               // allocation failed: one context, no overlap
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  xcorr_hip_result* h_res = nullptr;      // pinned host results of one batch (host-output calls)
+  const xcorr_hip_ctx* last_buf = nullptr;   // context whose buffers hold the last batch
 };
 
 static std::string g_create_err;
@@ -2889,6 +2896,10 @@ extern "C" void xcorr_hip_destroy(xcorr_hip_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->twin) xcorr_hip_destroy(c->twin);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->h_res) (void)hipHostFree(c->h_res);
   free_stream(c);
   free_scratch(c);
   if (c->own_master) (void)hipFree(c->own_master);
@@ -3233,11 +3244,57 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   });
 }
 
+// The second batch context of a strip context (same parameters and batch
+// size: its own line buffers, spectra, surfaces, plan buffer and stream; the
+// images and tables it uses are this context's).  Without the memory for it
+// the strips run on one context.
+static xcorr_hip_ctx* ensure_twin(xcorr_hip_ctx* c) {
+  if (c->twin || c->twin_failed) return c->twin;
+  xcorr_hip_ctx* t = nullptr;
+  if (xcorr_hip_create(&c->p, c->max_batch, c->device, &t) != 0) {
+    c->twin_failed = true;
+    g_create_err.clear();
+  }
+  (void)hipSetDevice(c->device);
+  c->twin = t;
+  return t;
+}
+
+// Strip batches in plan order.  Unless kernels are timed (their events need
+// one stream), batch k runs on context k mod 2 -- this one or its twin, each
+// on its own stream -- so one batch's kernels overlap the next one's.  The
+// twin's stream first waits for the work queued on this context's stream
+// before the call, and this stream waits for the twin's at the end, so callers
+// see one ordered stream.  Host results of a batch are scattered once the
+// next batch is queued (or before its context is used again).
 static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_hip_result* out, bool dev_out,
                       bool timing) {
   strip_order(w, n, c->order);
   const int cap = std::min(c->max_batch, c->tune_batch > 0 ? c->tune_batch : c->max_batch);
-  for (int pos = 0; pos < n;) {
+  if (!timing) {
+    if (!c->ev_fork) HIPCK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    if (!c->ev_join) HIPCK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    HIPCK(c, hipEventRecord(c->ev_fork, c->stream));
+  }
+  xcorr_hip_ctx* x2 = nullptr;                   // the twin, once this call has used it
+  xcorr_hip_ctx* pend = nullptr;                 // context whose batch's host results are not scattered yet
+  std::vector<int> pend_perm;
+  auto scatter = [&]() -> int {
+    if (!pend) return 0;
+    HIPCK(c, hipStreamSynchronize(pend->stream));
+    for (size_t i = 0; i < pend_perm.size(); i++) out[pend_perm[i]] = pend->h_res[i];
+    pend = nullptr;
+    return 0;
+  };
+  for (int pos = 0, k = 0; pos < n; k++) {
+    xcorr_hip_ctx* x = c;
+    if (!timing && (k & 1)) {
+      if (!x2) {
+        x2 = ensure_twin(c);
+        if (x2) HIPCK(c, hipStreamWaitEvent(x2->stream, c->ev_fork, 0));
+      }
+      if (x2) x = x2;
+    }
     const int si = c->stage_next;
     c->stage_next ^= 1;
     if (c->stage_used[si]) HIPCK(c, hipEventSynchronize(c->ev_stage[si]));
@@ -3245,21 +3302,39 @@ static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_
     StripBatch sb;
     if (!plan_strip_batch(c, c->dp, w, c->order, &pos, n, cap, h, &sb))
       return fail(c, XCORR_HIP_EINVAL, "correlate: a window does not fit the line capacity");
-    HIPCK(c, hipMemcpyAsync(c->d_plan, h, strip_plan_bytes(c, sb), hipMemcpyHostToDevice, c->stream));
-    HIPCK(c, hipEventRecord(c->ev_stage[si], c->stream));
+    HIPCK(c, hipMemcpyAsync(x->d_plan, h, strip_plan_bytes(c, sb), hipMemcpyHostToDevice, x->stream));
+    HIPCK(c, hipEventRecord(c->ev_stage[si], x->stream));
     c->stage_used[si] = true;
     c->last_perm.assign(h + c->max_batch, h + c->max_batch + sb.nb);
     c->last_rb.assign(h, h + sb.nb);
-    const int rc = launch_strip_batch(c, c->dp, c->d_master, c->d_slave, c->stream, c->d_plan, sb, out, dev_out,
+    c->last_buf = x;
+    const int rc = launch_strip_batch(x, c->dp, c->d_master, c->d_slave, x->stream, x->d_plan, sb, out, dev_out,
                                       timing);
-    if (rc) return rc;
-    if (!dev_out) {
-      c->hres.resize(sb.nb);
-      HIPCK(c, hipMemcpyAsync(c->hres.data(), c->d_out, sb.nb * sizeof(xcorr_hip_result), hipMemcpyDeviceToHost,
-                              c->stream));
-      HIPCK(c, hipStreamSynchronize(c->stream));
-      for (int i = 0; i < sb.nb; i++) out[c->last_perm[i]] = c->hres[i];
+    if (rc) {
+      if (x != c) c->err = x->err;
+      return rc;
     }
+    if (!dev_out) {
+      if (pend == x) {                           // one context: its pinned results are about to be reused
+        const int rs = scatter();
+        if (rs) return rs;
+      }
+      if (!x->h_res)
+        HIPCK(c, hipHostMalloc((void**)&x->h_res, (size_t)c->max_batch * sizeof(xcorr_hip_result),
+                               hipHostMallocDefault));
+      HIPCK(c, hipMemcpyAsync(x->h_res, x->d_out, sb.nb * sizeof(xcorr_hip_result), hipMemcpyDeviceToHost,
+                              x->stream));
+      const int rs = scatter();
+      if (rs) return rs;
+      pend = x;
+      pend_perm = c->last_perm;
+    }
+  }
+  const int rs = scatter();
+  if (rs) return rs;
+  if (x2) {
+    HIPCK(c, hipEventRecord(c->ev_join, x2->stream));
+    HIPCK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
   }
   return 0;
 }
@@ -4095,6 +4170,8 @@ extern "C" long xcorr_hip_dump(xcorr_hip_ctx* c, int what, int slot, void* host,
   HIPCK(c, hipSetDevice(c->device));
   HIPCK(c, hipStreamSynchronize(c->stream));
   const DevParams& d = c->dp;
+  // strip batches alternate between two contexts: the last one's buffers
+  const xcorr_hip_ctx* bf = (c->last_buf && !c->last_rb.empty()) ? c->last_buf : c;
   const void* src = nullptr;
   size_t nbytes = 0;
   std::vector<unsigned long long> tmp;
@@ -4103,23 +4180,23 @@ extern "C" long xcorr_hip_dump(xcorr_hip_ctx* c, int what, int slot, void* host,
     case 0:
     case 1:
       if (!c->last_rb.empty())   // npy consecutive strip lines from the row base
-        src = (what ? c->amp2s : c->amp1s) + (size_t)c->last_rb[slot] * d.NX;
+        src = (what ? bf->amp2s : bf->amp1s) + (size_t)c->last_rb[slot] * d.NX;
       else
         src = c->amp + ((size_t)slot * 2 + what) * d.N;
       nbytes = (size_t)d.N * sizeof(float);
       break;
     case 2:
       if (d.corr_flag == XCORR_HIP_FREQ) {
-        src = c->surf + (size_t)slot * d.nxc * d.nyc;
+        src = bf->surf + (size_t)slot * d.nxc * d.nyc;
         nbytes = (size_t)d.nxc * d.nyc * sizeof(float);
       } else {
-        src = c->surfd + (size_t)slot * d.nxc * d.nyc;
+        src = bf->surfd + (size_t)slot * d.nxc * d.nyc;
         nbytes = (size_t)d.nxc * d.nyc * sizeof(double);
       }
       break;
     case 3: {
       unsigned long long s[2];
-      HIPCK(c, hipMemcpy(s, c->sums + 2 * slot, sizeof(s), hipMemcpyDeviceToHost));
+      HIPCK(c, hipMemcpy(s, bf->sums + 2 * slot, sizeof(s), hipMemcpyDeviceToHost));
       for (int i = 0; i < 2; i++) means[i] = (double)(long long)s[i] * (1.0 / 1048576.0) / (double)d.N;
       if (bytes < sizeof(means)) return fail(c, XCORR_HIP_EINVAL, "dump: buffer too small");
       memcpy(host, means, sizeof(means));

@@ -144,3 +144,46 @@ def test_time_strips_bit_exact(xg, orc, pairs):
             eng.set_images(m, s)
             got = eng.correlate(w)
         _assert_bit_exact(got, ref, "time max_batch %d" % mb)
+
+
+@pytest.mark.parametrize("corr", ["freq", "time"])
+def test_strip_batches_on_two_contexts(xg, orc, pairs, corr):
+    """With more than one batch, strip batches alternate between the engine's
+    context and a second one (own buffers and stream).  Results in all three
+    pointer modes equal the restatement, and dumps after the call read the
+    buffers of whichever context ran the last batch: they equal the dumps of
+    the same location correlated alone."""
+    import torch
+    xs = 32
+    m, s = pairs[xs]
+    p = xg.make_params(xs, xs, corr_flag=xg.TIME if corr == "time" else xg.FREQ, interp_factor=8)
+    w = _strip_windows(xg, xs)
+    ref = orc.run(_oracle_params(orc, p), m, s, _locs(w), precision=32)
+    nbytes = len(w) * xg.RESULT_DTYPE.itemsize
+    checked = 0
+    for mb in (4, 5, 6, 9):                         # even and odd batch counts
+        with xg.Engine(p, max_batch=mb) as eng:
+            eng.set_images(m, s)
+            got = eng.correlate(w)
+            _assert_bit_exact(got, ref, "%s max_batch %d, host results" % (corr, mb))
+            d = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
+            eng.correlate(w, out=d, flags=xg.NO_SYNC)
+            eng.sync()
+            _assert_bit_exact(d.cpu().numpy().view(xg.RESULT_DTYPE), ref, "%s max_batch %d, device" % (corr, mb))
+            dumps = {}
+            for loc in range(len(w)):
+                try:
+                    dumps[loc] = [eng.dump(k, loc) for k in range(4)]
+                except xg.XcorrError:
+                    continue                        # not in the last batch
+            assert dumps, "no location of the last batch found"
+            t = (slice(xs, 3 * xs), slice(xs, 3 * xs))   # the slave amplitudes are kept on the template only
+            for loc, got_d in dumps.items():
+                eng.correlate(w[loc:loc + 1])       # one batch, on the engine's own context
+                for k in range(4):
+                    one = eng.dump(k, 0)
+                    a, b = (got_d[k][t], one[t]) if k == 1 else (got_d[k], one)
+                    assert np.array_equal(np.ascontiguousarray(a).view(np.uint8),
+                                          np.ascontiguousarray(b).view(np.uint8)), (corr, mb, loc, k)
+                checked += 1
+    assert checked >= 4
