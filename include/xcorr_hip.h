@@ -113,7 +113,11 @@ int xcorr_hip_set_images(xcorr_hip_ctx *ctx, const void *master, int m_nx, int m
  * memory).  out[i] always belongs to win[i].  For xsearch = ysearch = 64, ri 2,
  * frequency mode, windows that share mx0, sx0 and sy0 - my0 and overlap in
  * lines are computed from shared range-interpolated lines; the windows are then
- * read on the host (device windows are copied back first). */
+ * read on the host (device windows are copied back first).  On those shapes a
+ * call with more than one batch runs alternate batches on a second context
+ * the first such call allocates (the same buffers again: the two batches
+ * overlap on the device); its work is ordered on xcorr_hip_stream(ctx) like
+ * the rest.  If that allocation fails the batches run one after another. */
 int xcorr_hip_correlate(xcorr_hip_ctx *ctx, const xcorr_hip_window *win, int n,
                         xcorr_hip_result *out, unsigned flags);
 
