@@ -2583,6 +2583,7 @@ struct xcorr_hip_ctx {
   // so the kernels of one batch overlap those of the next
   xcorr_hip_ctx* twin = nullptr;
   bool twin_failed = false;               // allocation failed: one context, no overlap
+  bool one_context = false;               // never use a twin (contexts of a multi-pair graph)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   xcorr_hip_result* h_res = nullptr;      // pinned host results of one batch (host-output calls)
   const xcorr_hip_ctx* last_buf = nullptr;   // context whose buffers hold the last batch
@@ -3288,7 +3289,7 @@ static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_
   };
   for (int pos = 0, k = 0; pos < n; k++) {
     xcorr_hip_ctx* x = c;
-    if (!timing && (k & 1)) {
+    if (!timing && !c->one_context && (k & 1)) {
       if (!x2) {
         x2 = ensure_twin(c);
         if (x2) HIPCK(c, hipStreamWaitEvent(x2->stream, c->ev_fork, 0));
@@ -3990,6 +3991,7 @@ static int multi_build(xcorr_hip_multi* m, unsigned flags) {
     int rc = xcorr_hip_create(&p, mb, m->device, &m->ctx[k]);
     if (rc) return mfail(m, rc, std::string("multi: context for a pair class: ") + xcorr_hip_last_error(nullptr));
     xcorr_hip_ctx* c = m->ctx[k];
+    c->one_context = true;         // the graph runs each class on one stream: tune it that way
     if ((flags & XCORR_HIP_TUNE) && biggest >= 0) {
       bind_pair(c, m->pairs[biggest]);
       rc = xcorr_hip_autotune(c, m->pairs[biggest].win, m->pairs[biggest].n, flags, &m->tuning[k]);
