@@ -68,6 +68,9 @@ def main():
     ap.add_argument("--batch", type=int, default=8192, help="locations per engine batch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serial", action="store_true",
+                    help="strip batches one after another on one stream (default: consecutive batches overlap on "
+                         "two contexts); rocprof kernel durations are then those of each kernel alone")
     ap.add_argument("--small", action="store_true", help="the small workload (1024 x 2048, 1000 patches) instead")
     ap.add_argument("--workload", default="strip", choices=("strip", "small", "batched", "proposal", "dense"),
                     help="strip: 10800 x 27648 pair, 40,000 patches (default); small: 1024 x 2048, 1000 patches; "
@@ -141,8 +144,9 @@ def main():
     eng = xg.Engine(p, max_batch=min(args.batch, n))
     eng.set_images(m, s)
 
+    step_flags = xg.NO_SYNC | (xg.SERIAL if args.serial else 0)
     for _ in range(args.warmup):
-        eng.correlate(w, out=dout, flags=xg.NO_SYNC)
+        eng.correlate(w, out=dout, flags=step_flags)
     eng.sync()
     torch.cuda.synchronize()
     log("warmup done")
@@ -153,7 +157,7 @@ def main():
     t1 = time.perf_counter()
     evs[0].record(est)
     for k in range(args.steps):
-        eng.correlate(w, out=dout, flags=xg.NO_SYNC)
+        eng.correlate(w, out=dout, flags=step_flags)
         evs[k + 1].record(est)
     eng.sync()
     torch.cuda.synchronize()
@@ -266,7 +270,8 @@ def main():
            "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 4), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": workload, "patches": n, "xsearch": 64, "ysearch": 64, "ri": 2,
-                      "interp_factor": 16, "batch": min(args.batch, n)},
+                      "interp_factor": 16, "batch": min(args.batch, n),
+                      "batches": "serial" if args.serial else "overlapped on two contexts"},
            "roofline": roofline, "cpu_baseline": cpu, "path_roofline": path, "kernels": kernels,
            "step_stats": step_stats, "h2d": h2d, "parity": parity}
     print(json.dumps(out), flush=True)

@@ -2991,7 +2991,7 @@ static int launch_timed(xcorr_hip_ctx* c, bool timing, const char* name, F&& f) 
 
 static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xcorr_hip_result* d_out, bool timing);
 static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_hip_result* out, bool dev_out,
-                      bool timing);
+                      bool timing, bool serial);
 
 extern "C" int xcorr_hip_correlate(xcorr_hip_ctx* c, const xcorr_hip_window* win, int n, xcorr_hip_result* out,
                                    unsigned flags) {
@@ -3013,7 +3013,7 @@ extern "C" int xcorr_hip_correlate(xcorr_hip_ctx* c, const xcorr_hip_window* win
       HIPCK(c, hipStreamSynchronize(c->stream));
       hw = c->hwin.data();
     }
-    const int rc = run_strips(c, hw, n, out, dev_out, timing);
+    const int rc = run_strips(c, hw, n, out, dev_out, timing, timing || (flags & XCORR_HIP_SERIAL) != 0);
     if (rc) return rc;
   } else {
     const int bcap = c->tune_batch > 0 ? std::min(c->tune_batch, c->max_batch) : c->max_batch;
@@ -3261,18 +3261,18 @@ static xcorr_hip_ctx* ensure_twin(xcorr_hip_ctx* c) {
   return t;
 }
 
-// Strip batches in plan order.  Unless kernels are timed (their events need
-// one stream), batch k runs on context k mod 2 -- this one or its twin, each
+// Strip batches in plan order.  Unless serial (kernel timing, whose events
+// need one stream, or XCORR_HIP_SERIAL), batch k runs on context k mod 2 -- this one or its twin, each
 // on its own stream -- so one batch's kernels overlap the next one's.  The
 // twin's stream first waits for the work queued on this context's stream
 // before the call, and this stream waits for the twin's at the end, so callers
 // see one ordered stream.  Host results of a batch are scattered once the
 // next batch is queued (or before its context is used again).
 static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_hip_result* out, bool dev_out,
-                      bool timing) {
+                      bool timing, bool serial) {
   strip_order(w, n, c->order);
   const int cap = std::min(c->max_batch, c->tune_batch > 0 ? c->tune_batch : c->max_batch);
-  if (!timing) {
+  if (!serial) {
     if (!c->ev_fork) HIPCK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     if (!c->ev_join) HIPCK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     HIPCK(c, hipEventRecord(c->ev_fork, c->stream));
@@ -3289,7 +3289,7 @@ static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_
   };
   for (int pos = 0, k = 0; pos < n; k++) {
     xcorr_hip_ctx* x = c;
-    if (!timing && !c->one_context && (k & 1)) {
+    if (!serial && !c->one_context && (k & 1)) {
       if (!x2) {
         x2 = ensure_twin(c);
         if (x2) HIPCK(c, hipStreamWaitEvent(x2->stream, c->ev_fork, 0));

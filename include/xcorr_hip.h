@@ -51,6 +51,8 @@ extern "C" {
 #define XCORR_HIP_NO_SYNC 2u       /* correlate: return without waiting for the stream (device results only) */
 #define XCORR_HIP_TIMING 4u        /* correlate: record per-kernel HIP events (xcorr_hip_kernel_times) */
 #define XCORR_HIP_HOST_WINDOWS 8u  /* correlate with DEVICE_PTRS: windows are host memory, results device memory */
+#define XCORR_HIP_SERIAL 64u       /* correlate: strip batches one after another on the context's stream (no second
+                                      context; kernel durations then are those of a kernel alone on the device) */
 
 /* Run parameters, named after struct xcorr (xcorr_d.h:23-77).  Sizes follow
  * parse_xcorr_input.c:139-158: nx_corr = nxc = 2*xsearch, npx = 4*xsearch. */

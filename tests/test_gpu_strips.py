@@ -149,10 +149,10 @@ def test_time_strips_bit_exact(xg, orc, pairs):
 @pytest.mark.parametrize("corr", ["freq", "time"])
 def test_strip_batches_on_two_contexts(xg, orc, pairs, corr):
     """With more than one batch, strip batches alternate between the engine's
-    context and a second one (own buffers and stream).  Results in all three
-    pointer modes equal the restatement, and dumps after the call read the
-    buffers of whichever context ran the last batch: they equal the dumps of
-    the same location correlated alone."""
+    context and a second one (own buffers and stream).  Host results, device
+    results and XCORR_HIP_SERIAL results (one stream) equal the restatement,
+    and dumps after the call read the buffers of whichever context ran the
+    last batch: they equal the dumps of the same location correlated alone."""
     import torch
     xs = 32
     m, s = pairs[xs]
@@ -166,6 +166,7 @@ def test_strip_batches_on_two_contexts(xg, orc, pairs, corr):
             eng.set_images(m, s)
             got = eng.correlate(w)
             _assert_bit_exact(got, ref, "%s max_batch %d, host results" % (corr, mb))
+            _assert_bit_exact(eng.correlate(w, flags=xg.SERIAL), ref, "%s max_batch %d, serial" % (corr, mb))
             d = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
             eng.correlate(w, out=d, flags=xg.NO_SYNC)
             eng.sync()
