@@ -126,6 +126,13 @@ __device__ __forceinline__ long long fxp20(float a) {
 // four floats at a 4-byte aligned address (one global_load_dwordx4)
 typedef float f4a4 __attribute__((ext_vector_type(4), aligned(4)));
 
+// A streaming store (nontemporal hint): data the next kernels read long after
+// many other lines have passed through the cache.
+__device__ __forceinline__ void st_nt(float2* p, float2 v) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  __builtin_nontemporal_store(f2v{v.x, v.y}, (f2v*)p);
+}
+
 __device__ __forceinline__ double patch_mean(unsigned long long s, int N) {
   return (double)(long long)s * (1.0 / 1048576.0) / (double)N;
 }
@@ -760,7 +767,7 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
 #pragma unroll
       for (int q = 0; q < 8; q++) {
         const float2 a = make_float2(ea[q], oa[q]);
-        if (act) o[j + 16 * q] = a;
+        if (act) st_nt(o + j + 16 * q, a);
         mr[j + 16 * q] = a;
         s.add(ea[q]);
         s.add(oa[q]);
@@ -776,7 +783,7 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
 #pragma unroll
       for (int q = 0; q < 8; q++) {
         const float2 a = make_float2(ea[q], oa[q]);
-        if (wm && q >= 2 && q < 6) o[j + 16 * q] = a;
+        if (wm && q >= 2 && q < 6) st_nt(o + j + 16 * q, a);
         tile[j + 16 * q] = a;                      // the tile doubles as the slave line buffer
         s.add(ea[q]);
         s.add(oa[q]);
@@ -798,7 +805,7 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
     if (wm) {
       fft256_group_lds<-1>(v, j, tile, twl);
 #pragma unroll
-      for (int r = 0; r < 16; r++) Zm[l * 256 + j + 16 * r] = v[r];
+      for (int r = 0; r < 16; r++) st_nt(Zm + l * 256 + j + 16 * r, v[r]);
     }
     if (wu) {
 #pragma unroll
@@ -806,7 +813,7 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
       fft256_group_lds<-1>(v, j, tile, twl);
 #pragma unroll
       for (int r = 0; r <= 8; r++)              // bins 0..128; the rest are their conjugate mirrors
-        if (r < 8 || j == 0) Zu[l * 256 + j + 16 * r] = v[r];
+        if (r < 8 || j == 0) st_nt(Zu + l * 256 + j + 16 * r, v[r]);
     }
     wave_sync();                                // the next line's loads into mr / tile follow
   }
