@@ -720,13 +720,13 @@ struct LineSum {
 // per group and iteration.  Out: amp1 (all columns), amp2 (template columns of
 // masked lines), lsum (per-line fixed-point sums), Zm / Zu (row spectra of
 // masked / unmasked lines, where needed).
-// Block b + blk_off covers kStripBlockLines lines from global line blk[2b + 1]
-// of strip blk[2b].  EDGE: the strips' windows may cross the image sides.
+// Work item b < blk_n (blocks in xcd_item order) covers kStripBlockLines
+// lines from global line blk[2 (b + blk_off) + 1] of strip blk[2 (b + blk_off)].  EDGE: the strips' windows may cross the image sides.
 template <int FMT, bool EDGE>
 __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams P, const void* __restrict__ master,
                                                         const void* __restrict__ slave,
                                                         const StripDesc* __restrict__ strips,
-                                                        const int* __restrict__ blk, int blk_off,
+                                                        const int* __restrict__ blk, int blk_off, int blk_n,
                                                         const unsigned char* __restrict__ needM,
                                                         const unsigned char* __restrict__ needU,
                                                         float* __restrict__ amp1, float* __restrict__ amp2,
@@ -736,7 +736,9 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
   // pass-2 twiddles w256^(r j) [16][15] | phase factors phi[0..128]
   __shared__ float2 lds[16 * kT256Tile + 16 * 128 + 240 + 129];
   const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
-  const int bb = blockIdx.x + blk_off;
+  const int wi = xcd_item();
+  if (wi >= blk_n) return;
+  const int bb = wi + blk_off;
   const StripDesc S = strips[blk[2 * bb]];
   float2* tile = lds + g * kT256Tile;
   float2* mr = lds + 16 * kT256Tile + g * 128;
@@ -877,15 +879,16 @@ __device__ __forceinline__ void interp_line(const unsigned (&u)[RegFft<N>::V], i
 
 // Strip lines for npx = npy = N in {128, 512} (the outputs of k_stripA256;
 // without ROWFFT, for the time modes, amplitudes and line sums only):
-// block b + blk_off covers kStripBlockLines lines from global line
-// blk[2b + 1] of strip blk[2b], one line per group of G lanes and iteration;
+// work item b < blk_n (xcd_item order) covers kStripBlockLines lines from
+// global line blk[2 (b + blk_off) + 1] of strip blk[2 (b + blk_off)], one line
+// per group of G lanes and iteration;
 // the groups never wait for each other.  Zu holds bins 0 .. N/2 of the
 // unmasked rows' spectra (the rest are their conjugate mirrors).
 template <int N, int FMT, bool EDGE, bool ROWFFT = true>
 __global__ void __launch_bounds__(kThreads) k_stripA(DevParams P, const void* __restrict__ master,
                                                      const void* __restrict__ slave,
                                                      const StripDesc* __restrict__ strips,
-                                                     const int* __restrict__ blk, int blk_off,
+                                                     const int* __restrict__ blk, int blk_off, int blk_n,
                                                      const unsigned char* __restrict__ needM,
                                                      const unsigned char* __restrict__ needU,
                                                      float* __restrict__ amp1, float* __restrict__ amp2,
@@ -897,7 +900,9 @@ __global__ void __launch_bounds__(kThreads) k_stripA(DevParams P, const void* __
   __shared__ float2 mrow[GPB * (N / 2)];           // each group's master amplitude line (column pairs)
   __shared__ float2 sph[N];                         // range-interpolation phase factors
   const int g = threadIdx.x / G, j = threadIdx.x % G;
-  const int bb = blockIdx.x + blk_off;
+  const int wi = xcd_item();
+  if (wi >= blk_n) return;
+  const int bb = wi + blk_off;
   const StripDesc S = strips[blk[2 * bb]];
   float2* tile = tiles + g * F::TILE;
   float2* mr = mrow + g * (N / 2);
@@ -3069,6 +3074,29 @@ static void strip_order(const xcorr_hip_window* w, int n, std::vector<int>& ord)
   });
 }
 
+// Line blocks bs[lo .. hi) in order of their first master line, then of their
+// strip's master column: blocks of neighbouring strips that read the same
+// image lines come next to each other, and with the kernels' xcd_item order
+// run on one XCD, so those lines are read from HBM once into its L2.
+static void order_line_blocks(const StripDesc* st, int* bs, int lo, int hi) {
+  struct E {
+    long long row;
+    int mx0, k, l0;
+  };
+  std::vector<E> v;
+  v.reserve(hi - lo);
+  for (int e = lo; e < hi; e++) {
+    const StripDesc& S = st[bs[2 * e]];
+    v.push_back({(long long)S.my0 + (bs[2 * e + 1] - S.line0), S.mx0, bs[2 * e], bs[2 * e + 1]});
+  }
+  std::stable_sort(v.begin(), v.end(),
+                   [](const E& a, const E& b) { return a.row != b.row ? a.row < b.row : a.mx0 < b.mx0; });
+  for (int e = lo; e < hi; e++) {
+    bs[2 * e] = v[e - lo].k;
+    bs[2 * e + 1] = v[e - lo].l0;
+  }
+}
+
 // Plans the next batch (at most `cap` windows) from ord[*pos] on into h
 // (plan_ints(c) ints); advances *pos.  Returns false if a window does not fit.
 static bool plan_strip_batch(const xcorr_hip_ctx* c, const DevParams& d, const xcorr_hip_window* w,
@@ -3127,6 +3155,8 @@ static bool plan_strip_batch(const xcorr_hip_ctx* c, const DevParams& d, const x
     }
     if (pass == 0) nblk_in = nblk;
   }
+  order_line_blocks(st, bs, 0, nblk_in);
+  order_line_blocks(st, bs, nblk_in, nblk);
   sb->nb = nb;
   sb->ns = ns;
   sb->nl = (int)L;
@@ -3182,8 +3212,9 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   const bool ci = d.format == XCORR_HIP_CINT16;
   const bool freq = d.corr_flag == XCORR_HIP_FREQ;    // else -time on 128 x 128 windows
   rc = launch_timed(c, timing, d.NX == 256 ? "k_stripA256" : "k_stripA", [&] {
-    using KA = void (*)(DevParams, const void*, const void*, const StripDesc*, const int*, int, const unsigned char*,
-                        const unsigned char*, float*, float*, long long*, float2*, float2*, const float2*);
+    using KA = void (*)(DevParams, const void*, const void*, const StripDesc*, const int*, int, int,
+                        const unsigned char*, const unsigned char*, float*, float*, long long*, float2*, float2*,
+                        const float2*);
     KA kin, kedge;
     if (d.NX == 256) {
       kin = ci ? k_stripA256<XCORR_HIP_CINT16, false> : k_stripA256<XCORR_HIP_FLOAT32, false>;
@@ -3199,13 +3230,13 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
       kedge = ci ? k_stripA<512, XCORR_HIP_CINT16, true> : k_stripA<512, XCORR_HIP_FLOAT32, true>;
     }
     if (nblk_in > 0)
-      hipLaunchKernelGGL(kin, dim3(nblk_in), dim3(kThreads), 0, st, d, master, slave, d_st, d_bs, 0,
-                         (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s, c->amp2s, c->lsum, c->Z,
-                         c->Zu, (const float2*)c->tabs);
+      hipLaunchKernelGGL(kin, dim3((nblk_in + 7) & ~7), dim3(kThreads), 0, st, d, master, slave, d_st, d_bs, 0,
+                         nblk_in, (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s, c->amp2s,
+                         c->lsum, c->Z, c->Zu, (const float2*)c->tabs);
     if (nblk > nblk_in)
-      hipLaunchKernelGGL(kedge, dim3(nblk - nblk_in), dim3(kThreads), 0, st, d, master, slave, d_st, d_bs, nblk_in,
-                         (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s, c->amp2s, c->lsum, c->Z,
-                         c->Zu, (const float2*)c->tabs);
+      hipLaunchKernelGGL(kedge, dim3((nblk - nblk_in + 7) & ~7), dim3(kThreads), 0, st, d, master, slave, d_st, d_bs,
+                         nblk_in, nblk - nblk_in, (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s,
+                         c->amp2s, c->lsum, c->Z, c->Zu, (const float2*)c->tabs);
   });
   if (rc) return rc;
   rc = launch_timed(c, timing, "k_patch_sums", [&] {
