@@ -721,7 +721,8 @@ struct LineSum {
 // masked lines), lsum (per-line fixed-point sums), Zm / Zu (row spectra of
 // masked / unmasked lines, where needed).
 // Work item b < blk_n (blocks in xcd_item order) covers kStripBlockLines
-// lines from global line blk[2 (b + blk_off) + 1] of strip blk[2 (b + blk_off)].  EDGE: the strips' windows may cross the image sides.
+// lines from global line blk[2 (b + blk_off) + 1] of strip blk[2 (b + blk_off)].
+// EDGE: the strips' windows may cross the image sides.
 template <int FMT, bool EDGE>
 __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams P, const void* __restrict__ master,
                                                         const void* __restrict__ slave,
