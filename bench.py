@@ -43,6 +43,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """Threads for the CPU baseline and a description of the host: every CPU in
+    this process's affinity set, capped by OMP_NUM_THREADS when the environment
+    sets one (a machine that allots a CPU share per GPU says so there)."""
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(aff, int(omp)) if omp.isdigit() and int(omp) > 0 else aff
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return threads, {"nproc": aff, "os_cpu_count": os.cpu_count(), "omp_num_threads": omp or None,
+                     "cpu_model": model}
+
+
 def committed_traffic():
     """Per-launch HBM bytes per kernel from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (the JSON line under "Per-launch
@@ -233,7 +252,7 @@ def main():
         import oracle
         oracle.lib()
         got = dout.cpu().numpy().view(xg.RESULT_DTYPE)
-        nthreads = min(16, os.cpu_count() or 1)
+        nthreads, host = host_cpu()
         # host copy of the rows the sample needs
         op = oracle.make_params(64, 64)
         probe = min(n, max(nthreads, 16))
@@ -253,9 +272,16 @@ def main():
         tp = time.perf_counter()
         ref = oracle.run(op, mh, shh, locs[:sample], precision=32, nthreads=nthreads)
         cpu_s = time.perf_counter() - tp
+        # one thread on a shorter sample of the same locations
+        one = int(min(sample, max(4, args.cpu_seconds / 4 / (per * nthreads))))
+        tp = time.perf_counter()
+        oracle.run(op, mh, shh, locs[:one], precision=32, nthreads=1)
+        one_s = time.perf_counter() - tp
         cpu = {"value": round(sample / cpu_s, 2), "unit": "patches/s", "cores": nthreads, "kind": "port",
-               "sample": "first %d of the %d locations (FP32 CPU restatement, OpenMP over patches, %.1f s)"
-                         % (sample, n, cpu_s)}
+               "sample": "first %d of the %d locations (FP32 CPU restatement, OpenMP over patches, %.1f s); "
+                         "1 thread: first %d (%.1f s)" % (sample, n, cpu_s, one, one_s),
+               "threads_1_patches_per_s": round(one / one_s, 2),
+               "threads_n_patches_per_s": round(sample / cpu_s, 2), **host}
         same = True
         for f in ("xoff", "yoff", "corr", "xfrac", "yfrac", "ip", "jp"):
             a, b = got[f][:sample], ref[f]
@@ -566,7 +592,7 @@ def batched_cpu_baseline(args, xg, work, classes, pc):
     workload; the sample doubles as a bit-exactness check."""
     import oracle
     oracle.lib()
-    nthreads = min(16, os.cpu_count() or 1)
+    nthreads, host = host_cpu()
     total_cpu_s, total_n, sample_n, same = 0.0, 0, 0, True
     budget = args.cpu_seconds / max(1, len(classes))
     for k, (p, t) in enumerate(classes):
@@ -604,7 +630,7 @@ def batched_cpu_baseline(args, xg, work, classes, pc):
     cpu = {"value": round(total_n / total_cpu_s, 2), "unit": "patches/s", "cores": nthreads, "kind": "port",
            "sample": "%d patches over %d classes (largest pair of each), per-class CPU time weighted by the class's "
                      "share of the %d patches (FP32 CPU restatement, OpenMP over patches)"
-                     % (sample_n, len(classes), total_n)}
+                     % (sample_n, len(classes), total_n), **host}
     log("cpu baseline: %.1f patches/s on %d threads; parity %s" % (cpu["value"], nthreads, same))
     return cpu, {"sample": sample_n, "bit_exact_vs_cpu": same}
 

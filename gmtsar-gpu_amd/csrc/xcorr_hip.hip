@@ -3306,9 +3306,11 @@ static xcorr_hip_ctx* ensure_twin(xcorr_hip_ctx* c) {
 // twin's stream first waits for the work queued on this context's stream
 // before the call, and this stream waits for the twin's at the end, so callers
 // see one ordered stream.  Host results of a batch are scattered once the
-// next batch is queued (or before its context is used again).
-static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_hip_result* out, bool dev_out,
-                      bool timing, bool serial) {
+// next batch is queued (or before its context is used again).  *twin is set
+// once the twin's stream has forked from this one; run_strips joins it on
+// every exit.
+static int run_strips_body(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_hip_result* out, bool dev_out,
+                           bool timing, bool serial, xcorr_hip_ctx** twin) {
   strip_order(w, n, c->order);
   const int cap = std::min(c->max_batch, c->tune_batch > 0 ? c->tune_batch : c->max_batch);
   if (!serial) {
@@ -3331,7 +3333,10 @@ static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_
     if (!serial && !c->one_context && (k & 1)) {
       if (!x2) {
         x2 = ensure_twin(c);
-        if (x2) HIPCK(c, hipStreamWaitEvent(x2->stream, c->ev_fork, 0));
+        if (x2) {
+          HIPCK(c, hipStreamWaitEvent(x2->stream, c->ev_fork, 0));
+          *twin = x2;
+        }
       }
       if (x2) x = x2;
     }
@@ -3370,12 +3375,25 @@ static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_
       pend_perm = c->last_perm;
     }
   }
-  const int rs = scatter();
-  if (rs) return rs;
-  if (x2) {
-    HIPCK(c, hipEventRecord(c->ev_join, x2->stream));
-    HIPCK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  return scatter();
+}
+
+// The stream of this context waits for the twin's work on every exit, so
+// xcorr_hip_sync and the caller's stream order cover every batch that was
+// queued; after a failure the twin's queued kernels (which still write the
+// caller's results and read the images) are also waited for before returning.
+static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_hip_result* out, bool dev_out,
+                      bool timing, bool serial) {
+  xcorr_hip_ctx* x2 = nullptr;
+  const int rc = run_strips_body(c, w, n, out, dev_out, timing, serial, &x2);
+  if (!x2) return rc;
+  if (rc) {
+    (void)hipStreamSynchronize(x2->stream);
+    if (hipEventRecord(c->ev_join, x2->stream) == hipSuccess) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    return rc;
   }
+  HIPCK(c, hipEventRecord(c->ev_join, x2->stream));
+  HIPCK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
   return 0;
 }
 

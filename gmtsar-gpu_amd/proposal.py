@@ -230,7 +230,7 @@ class Proposer:
     the verification -- peak on the border of the shrunken lag window or
     correlation below corr_min -- and redone with the plain search)."""
 
-    def __init__(self, params, tau=0.5, tau_skip=1.01, max_batch=8192, device=0, interp_factor=16, xs_small=32,
+    def __init__(self, params, tau=0.9, tau_skip=1.01, max_batch=8192, device=0, interp_factor=16, xs_small=32,
                  verify=False, corr_min=0.0):
         import torch
         self.torch = torch

@@ -108,10 +108,14 @@ def test_stage_dumps_bit_exact(xg, orc, pair):
             assert got["ip"][slot] == rec["ip"] and got["jp"][slot] == rec["jp"]
 
 
-def test_tolerance_vs_fp64_oracle(xg, orc, pair):
+@pytest.mark.parametrize("xs,nxl,nyl", [(32, 10, 8), (64, 8, 10)], ids=["xs32", "xs64_headline"])
+def test_tolerance_vs_fp64_oracle(xg, orc, pair, xs, nxl, nyl):
+    """GPU (FP32 specification) against the FP64 textbook path; xs 64 is the
+    bench's shape (256 x 256 windows, 16x high-res), where the spectral demean,
+    Hermitian completion and c1 + i c2 packing all take part."""
     m, s = pair
-    p = xg.make_params(32, 32)
-    w, _, _ = _windows(xg, p, 10, 8)
+    p = xg.make_params(xs, xs)
+    w, _, _ = _windows(xg, p, nxl, nyl)
     with xg.Engine(p) as eng:
         eng.set_images(m, s)
         got = eng.correlate(w)

@@ -53,6 +53,73 @@ def test_fft_parseval_and_shift_theorem(orc):
     assert np.abs(y - x).max() < 1e-5 * np.abs(x).max()
 
 
+def _every_plan(n):
+    """Every radix plan the engine's set_tuning accepts for length n (plan_valid:
+    1-4 passes of radix 2/4/8/16 in any order, product n); the autotuner's
+    candidates (non-increasing sequences with at most one radix 2, and their
+    reversals) are a subset."""
+    out = []
+
+    def rec(rem, cur):
+        if rem == 1 and cur:
+            out.append(list(cur))
+            return
+        if len(cur) == 4:
+            return
+        for r in (16, 8, 4, 2):
+            if rem % r == 0:
+                rec(rem // r, cur + [r])
+
+    rec(n, [])
+    return out
+
+
+PLAN_SIZES = [8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
+
+
+@pytest.mark.parametrize("n", PLAN_SIZES)
+def test_fft_every_radix_plan_matches_numpy(orc, n):
+    """xo32_set_fft_plan: the FP32 transform under every accepted radix plan,
+    both directions, against numpy.fft (FP64) with the default plan's FP32 bound.
+    A wrong twiddle index or stride in any pass at any L fails here."""
+    rng = np.random.default_rng(1000 + n)
+    x = _rand(rng, n).astype(np.complex64)
+    ref = np.fft.fft(x.astype(np.complex128))
+    refi = np.fft.ifft(x.astype(np.complex128)) * n
+    scale = np.abs(ref).max()
+    tol = 3e-7 * np.log2(n) + 1e-7
+    plans = _every_plan(n)
+    assert plans and all(int(np.prod(p)) == n for p in plans)
+    for plan in plans:
+        with orc.fft_plans({n: plan}):
+            f, b = orc.fft32(x, -1), orc.fft32(x, +1)
+        assert np.abs(f - ref).max() / scale < tol, plan
+        assert np.abs(b - refi).max() / scale < tol, plan
+    # the plan is undone on exit: the default plan's output again
+    np.testing.assert_array_equal(orc.fft32(x, -1), orc.fft32(x.copy(), -1))
+
+
+@pytest.mark.parametrize("n", [8, 16, 32, 64])
+def test_fft_every_radix_plan_matches_direct_dft(orc, n):
+    """The same plans against a direct O(n^2) DFT (independent of numpy's FFT),
+    on a non-random input where each bin is distinct."""
+    t = np.arange(n)
+    x = (np.cos(2 * np.pi * 3 * t / n) + 0.5j * np.sin(2 * np.pi * 5 * t / n) + 0.01 * t * t).astype(np.complex64)
+    W = np.exp(-2j * np.pi * np.outer(t, t) / n)
+    ref = W @ x.astype(np.complex128)
+    refi = np.conj(W) @ x.astype(np.complex128)
+    for plan in _every_plan(n):
+        with orc.fft_plans({n: plan}):
+            assert np.abs(orc.fft32(x, -1) - ref).max() < 1e-5 * np.abs(x).sum(), plan
+            assert np.abs(orc.fft32(x, +1) - refi).max() < 1e-5 * np.abs(x).sum(), plan
+
+
+def test_invalid_radix_plans_rejected(orc):
+    for n, plan in ((256, [16, 8]), (256, [3, 85]), (4096, [2, 2, 2, 2, 256]), (100, [10, 10])):
+        with pytest.raises(ValueError):
+            orc.set_fft_plan(n, plan)
+
+
 def test_twiddles_exact_on_axes_and_diagonals(orc):
     for n in (8, 64, 256, 4096):
         assert orc.twiddle64(n, 0) == (1.0, 0.0)
@@ -239,6 +306,64 @@ def test_fp32_path_agrees_with_fp64(orc, synth):
         assert abs(r32["yfrac"] - r64["yfrac"]) <= 1 / 16 + 1e-6
 
 
+HEADLINE_SHAPES = [(64, 16), (128, 16), (64, 8)]
+
+
+@pytest.mark.parametrize("xs,ifc", HEADLINE_SHAPES)
+@pytest.mark.parametrize("coh", [0.8, 0.3])
+def test_fp32_path_agrees_with_fp64_headline_shapes(orc, synth, xs, ifc, coh):
+    """The FP32 specification (range interpolation fused with the row FFT,
+    demean in the spectrum, Hermitian completion of real rows, c1 + i c2
+    packing, direct 8-term high-res sums) against the FP64 textbook path
+    (separate zero-pad FFT_N -> IFFT_M, two 2-D FFTs, full inverse, hypot,
+    pow(., 1/4), FP64 zero-pad interpolation) at the bench's window sizes:
+    256^2 and 512^2 windows (do_freq_xcorr.cu:90-200, highres_corr.cu:23-79).
+    Peaks agree outside near-ties (the stated surface bound), corr within 0.05
+    percent points, xfrac/yfrac within one high-res step."""
+    bound = 2.0 ** -16
+    p = orc.make_params(xs, xs, interp_factor=ifc)
+    n = 4 * xs
+    m, s = synth.constant_shift_pair(n + 96, n + 96, 2.6, -4.3, coh, seed=60 + xs + ifc, noise_seed=61)
+    worst = 0.0
+    checked = 0
+    for k, (oy, ox) in enumerate([(8, 8), (40, 24), (80, 88), (24, 72)]):
+        c = lambda a: (a[oy:oy + n, ox:ox + n, 0].astype(np.float32) + 1j * a[oy:oy + n, ox:ox + n, 1].astype(np.float32))
+        r32, d32 = orc.patch(p, c(m), c(s), precision=32, dump=True)
+        r64, d64 = orc.patch(p, c(m), c(s), precision=64, dump=True)
+        s64 = d64["surf"]
+        top = s64.max()
+        worst = max(worst, np.abs(d32["surf"].astype(np.float64) - s64).max() / top)
+        v = np.sort(s64.ravel())
+        if (v[-1] - v[-2]) / top <= 2 * bound:
+            continue                                  # near-tie: a flip is allowed
+        checked += 1
+        assert (r32["ip"], r32["jp"]) == (r64["ip"], r64["jp"]), k
+        assert abs(r32["corr"] - r64["corr"]) < 0.05, k
+        assert abs(r32["xfrac"] - r64["xfrac"]) <= 1 / ifc + 1e-6, k
+        assert abs(r32["yfrac"] - r64["yfrac"]) <= 1 / ifc + 1e-6, k
+        assert abs(r32["cave"] - r64["cave"]) < 1e-5 * abs(r64["cave"]), k
+    assert worst < bound
+    assert checked >= 3
+
+
+@pytest.mark.parametrize("plan", [[4, 4, 4, 4], [2, 8, 16], [16, 4, 4], [8, 2, 16]])
+def test_tuned_plan_patch_agrees_with_fp64(orc, synth, plan):
+    """A whole 256^2 patch with a non-default radix plan on every 256-point
+    transform (a plan the autotuner can choose) against the FP64 path."""
+    xs = 64
+    p = orc.make_params(xs, xs)
+    n = 4 * xs
+    m, s = synth.constant_shift_pair(n + 64, n + 64, -3.2, 5.7, 0.8, seed=70, noise_seed=71)
+    c = lambda a: (a[32:32 + n, 32:32 + n, 0].astype(np.float32) + 1j * a[32:32 + n, 32:32 + n, 1].astype(np.float32))
+    with orc.fft_plans({256: plan}):
+        r32, _ = orc.patch(p, c(m), c(s), precision=32)
+    r64, _ = orc.patch(p, c(m), c(s), precision=64)
+    assert (r32["ip"], r32["jp"]) == (r64["ip"], r64["jp"])
+    assert abs(r32["corr"] - r64["corr"]) < 0.05
+    assert abs(r32["xfrac"] - r64["xfrac"]) <= 1 / 16 + 1e-6
+    assert abs(r32["yfrac"] - r64["yfrac"]) <= 1 / 16 + 1e-6
+
+
 def test_integer_shift_recovered_exactly(orc, synth):
     """Aligned = master shifted by whole pixels (no noise): the integer lag is the
     shift (range lag in 1/ri pixels) and gamma is ~100."""
@@ -281,20 +406,22 @@ def test_zero_windows_are_defined(orc):
         assert rec["corr"] == 0.0 and rec["ip"] == 0 and rec["jp"] == 0
 
 
-def test_integer_peaks_fp32_vs_fp64_outside_near_ties(orc, synth):
+@pytest.mark.parametrize("xs,W,H,step", [(16, 512, 640, (37, 41)), (64, 1024, 1024, (97, 89))])
+def test_integer_peaks_fp32_vs_fp64_outside_near_ties(orc, synth, xs, W, H, step):
     """The FP32 surface (the GPU's arithmetic) stays within a stated error
     bound of the FP64 surface, and wherever the FP64 surface's top two values
     are separated by more than twice that bound the integer peaks agree; a
     flip is only possible on a near-tie.  Low coherence (0.25) makes near-ties
-    common enough to show up; their count is printed."""
-    xs = ys = 16
+    common enough to show up; their count is printed.  xs 64 is the bench's
+    256^2 window with 16x high-res."""
+    ys = xs
     npx, npy = 4 * xs, 4 * ys
-    m, s = synth.constant_shift_pair(512, 640, 1.7, -2.9, 0.25, seed=41, noise_seed=42)
+    m, s = synth.constant_shift_pair(W, H, 1.7, -2.9, 0.25, seed=41, noise_seed=42)
     p = orc.make_params(xs, ys)
     bound = 2.0 ** -16                             # |surf32 - surf64| / max(surf64)
     worst, near, flips = 0.0, 0, 0
-    for my0 in range(0, 640 - npy, 37):
-        for mx0 in range(0, 512 - npx, 41):
+    for my0 in range(0, H - npy, step[0]):
+        for mx0 in range(0, W - npx, step[1]):
             c = lambda a: (a[my0:my0 + npy, mx0:mx0 + npx, 0].astype(np.float32)
                            + 1j * a[my0:my0 + npy, mx0:mx0 + npx, 1].astype(np.float32))
             mw, sw = c(m), c(s)
