@@ -31,7 +31,7 @@ EXPORTED_SYMBOLS = (
     "xcorr_hip_dump", "xcorr_hip_destroy", "xcorr_hip_last_error", "xcorr_hip_correlate_stream",
     "xcorr_hip_get_tuning", "xcorr_hip_set_tuning", "xcorr_hip_autotune", "xcorr_hip_multi_create",
     "xcorr_hip_multi_run", "xcorr_hip_multi_sync", "xcorr_hip_multi_stream", "xcorr_hip_multi_classes",
-    "xcorr_hip_multi_destroy", "xcorr_hip_multi_last_error",
+    "xcorr_hip_multi_destroy", "xcorr_hip_multi_last_error", "xcorr_hip_correlate_noalloc",
 )
 
 # int reader(void *user, int image, int first, int count, void *dst)
@@ -95,6 +95,7 @@ def load_library():
     lib.xcorr_hip_set_images.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_uint]
     lib.xcorr_hip_correlate.argtypes = [P, P, ctypes.c_int, P, ctypes.c_uint]
+    lib.xcorr_hip_correlate_noalloc.argtypes = [P, P, ctypes.c_int, P, ctypes.c_uint]
     lib.xcorr_hip_sync.argtypes = [P]
     lib.xcorr_hip_stream.argtypes = [P]
     lib.xcorr_hip_stream.restype = P
@@ -244,6 +245,15 @@ class Engine:
         rc = self.lib.xcorr_hip_correlate(self.h, _ptr(w), n, _ptr(res), flags)
         self._check(rc, "xcorr_hip_correlate")
         return res
+
+    def correlate_noalloc(self, windows, out, flags=0):
+        """xcorr_hip_correlate_noalloc: windows (n x 4 int32) and out (n result
+        records as bytes) are torch device tensors; nothing is allocated."""
+        n = windows.shape[0]
+        rc = self.lib.xcorr_hip_correlate_noalloc(self.h, ctypes.c_void_p(windows.data_ptr()), n,
+                                                  ctypes.c_void_p(out.data_ptr()), flags)
+        self._check(rc, "xcorr_hip_correlate_noalloc")
+        return out
 
     def sync(self):
         self._check(self.lib.xcorr_hip_sync(self.h), "xcorr_hip_sync")

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <climits>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2610,6 +2611,30 @@ static int fail(xcorr_hip_ctx* c, int code, const std::string& msg) {
   return code;
 }
 
+// Device buffers of one batch of the per-patch kernels (indexed by the
+// patch's slot in the batch; no stride depends on the batch capacity).  A
+// context's own buffers, or ones carved from the static array of
+// xcorr_hip_correlate_noalloc.
+struct PatchBufs {
+  float* amp;                   // [B][2][N] amplitudes
+  unsigned long long* sums;     // [B][2] exact patch sums
+  float2* Z;                    // [B][N] row / 2-D spectra
+  float2* Zu;                   // strips only: unmasked row spectra
+  float2* D;                    // [B][nyc][NX] lag rows
+  float* surf;                  // [B][nyc][nxc] correlation surface (frequency)
+  double* surfd;                // [B][nyc][nxc] correlation surface (time modes)
+  unsigned long long* peak;     // [B] peak keys
+  double* partial;              // [B][max(groups4, nyc/G4)] surface partial sums
+  int* tpeak;                   // [B][4] time-mode peaks
+  double* tcave;                // [B] time-mode surface means
+  double* tcaa;                 // [B][nyc][nxc] -time on 128 x 128 windows: master energies of every lag
+};
+
+static PatchBufs ctx_bufs(const xcorr_hip_ctx* c) {
+  return PatchBufs{c->amp, c->sums, c->Z, c->Zu, c->D, c->surf, c->surfd, c->peak, c->partial, c->tpeak, c->tcave,
+                   c->tcaa};
+}
+
 #define HIPCK(ctx, call)                                                                  \
   do {                                                                                    \
     hipError_t e_ = (call);                                                               \
@@ -3002,7 +3027,8 @@ static int launch_timed(xcorr_hip_ctx* c, bool timing, const char* name, F&& f) 
   return 0;
 }
 
-static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xcorr_hip_result* d_out, bool timing);
+static int run_batch(xcorr_hip_ctx* c, const PatchBufs& b, const xcorr_hip_window* d_win, int nb,
+                     xcorr_hip_result* d_out, bool timing);
 static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_hip_result* out, bool dev_out,
                       bool timing, bool serial);
 
@@ -3037,7 +3063,7 @@ extern "C" int xcorr_hip_correlate(xcorr_hip_ctx* c, const xcorr_hip_window* win
       if (!dev_win)
         HIPCK(c, hipMemcpyAsync(c->d_win, win + off, nb * sizeof(xcorr_hip_window), hipMemcpyHostToDevice,
                                 c->stream));
-      int rc = run_batch(c, dw, nb, dout, timing);
+      int rc = run_batch(c, ctx_bufs(c), dw, nb, dout, timing);
       if (rc) return rc;
       if (!dev_out) {
         HIPCK(c, hipMemcpyAsync(out + off, c->d_out, nb * sizeof(xcorr_hip_result), hipMemcpyDeviceToHost, c->stream));
@@ -3174,21 +3200,22 @@ static size_t strip_plan_bytes(const xcorr_hip_ctx* c, const StripBatch& sb) {
 // k_colpass<N> over nb patches (rows of Z per patch, or strip lines from the
 // row bases rb) on stream st.
 template <int N>
-static void launch_colpass(xcorr_hip_ctx* c, const DevParams& d, hipStream_t st, const int* rb, int nb) {
+static void launch_colpass(xcorr_hip_ctx* c, const PatchBufs& b, const DevParams& d, hipStream_t st, const int* rb,
+                           int nb) {
   const int nwork = nb * (N / (2 * ColCfg<N>::TC));
-  hipLaunchKernelGGL(k_colpass<N>, dim3((nwork + 7) & ~7), dim3(kThreads), 0, st, d, (const float2*)c->Z,
-                     (const float2*)c->Zu, rb, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs, nwork);
+  hipLaunchKernelGGL(k_colpass<N>, dim3((nwork + 7) & ~7), dim3(kThreads), 0, st, d, (const float2*)b.Z,
+                     (const float2*)b.Zu, rb, (const unsigned long long*)b.sums, b.D, (const float2*)c->tabs, nwork);
 }
 
 // The -time search of 128 x 128 windows (xs = ys = 32) on the int8 matrix
 // cores: sliding energies, then the search with its peak scan.  Amplitudes
 // per patch (rb null: A1 = A2 = amp) or on strip lines from row bases rb.
-static void launch_time_i8(xcorr_hip_ctx* c, const DevParams& d, hipStream_t st, const float* A1, const float* A2,
+static void launch_time_i8(const PatchBufs& b, const DevParams& d, hipStream_t st, const float* A1, const float* A2,
                            const int* rb, int nb) {
   hipLaunchKernelGGL(k_time_energy32, dim3((nb + 7) & ~7), dim3(kThreads), 0, st, d, A1, rb,
-                     (const unsigned long long*)c->sums, c->tcaa, nb);
+                     (const unsigned long long*)b.sums, b.tcaa, nb);
   hipLaunchKernelGGL(k_time_search_i8, dim3((nb + 7) & ~7), dim3(kThreads), 0, st, d, A1, A2, rb,
-                     (const unsigned long long*)c->sums, (const double*)c->tcaa, c->surfd, c->tpeak, c->tcave, nb);
+                     (const unsigned long long*)b.sums, (const double*)b.tcaa, b.surfd, b.tpeak, b.tcave, nb);
 }
 
 // Kernels of one strip batch whose plan is in device memory at d_plan, on
@@ -3247,7 +3274,7 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   if (rc) return rc;
   if (!freq) {
     rc = launch_timed(c, timing, "k_time_search", [&] {
-      launch_time_i8(c, d, st, (const float*)c->amp1s, (const float*)c->amp2s, d_rb, nb);
+      launch_time_i8(ctx_bufs(c), d, st, (const float*)c->amp1s, (const float*)c->amp2s, d_rb, nb);
     });
   } else {
   rc = launch_timed(c, timing, d.NX == 256 ? "k_colpass256" : "k_colpass", [&] {
@@ -3256,9 +3283,9 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
                          (const float2*)c->Zu, d_rb, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs,
                          nb * 8);
     else if (d.NX == 128)
-      launch_colpass<128>(c, d, st, d_rb, nb);
+      launch_colpass<128>(c, ctx_bufs(c), d, st, d_rb, nb);
     else
-      launch_colpass<512>(c, d, st, d_rb, nb);
+      launch_colpass<512>(c, ctx_bufs(c), d, st, d_rb, nb);
   });
   if (rc) return rc;
   rc = launch_timed(c, timing, d.NX == 256 ? "k_rowifft_surf256" : "k_rowifft_surf_reg", [&] {
@@ -3397,12 +3424,13 @@ static int run_strips(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_
   return 0;
 }
 
-static int time_search(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, bool timing);
+static int time_search(xcorr_hip_ctx* c, const PatchBufs& b, int nb, bool timing);
 
-static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xcorr_hip_result* d_out, bool timing) {
+static int run_batch(xcorr_hip_ctx* c, const PatchBufs& b, const xcorr_hip_window* d_win, int nb,
+                     xcorr_hip_result* d_out, bool timing) {
   const DevParams& d = c->dp;
   const int rc0 = launch_timed(c, timing, "k_init", [&] {
-    hipLaunchKernelGGL(k_init, dim3((2 * nb + 255) / 256), dim3(256), 0, c->stream, c->sums, c->peak, nb);
+    hipLaunchKernelGGL(k_init, dim3((2 * nb + 255) / 256), dim3(256), 0, c->stream, b.sums, b.peak, nb);
   });
   if (rc0) return rc0;
   const size_t nel1 = (size_t)d.G1 * d.NX;
@@ -3422,34 +3450,34 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
     rc = launch_timed(c, timing, "k_stageA", [&] {
       if (d.NX == 128)
         hipLaunchKernelGGL((k_stageA<128, true>), dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave,
-                           d_win, c->amp, c->sums, c->Z, (const float2*)c->tabs);
+                           d_win, b.amp, b.sums, b.Z, (const float2*)c->tabs);
       else
         hipLaunchKernelGGL((k_stageA<512, true>), dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave,
-                           d_win, c->amp, c->sums, c->Z, (const float2*)c->tabs);
+                           d_win, b.amp, b.sums, b.Z, (const float2*)c->tabs);
     });
   else if (regT)
     rc = launch_timed(c, timing, "k_stageA", [&] {
       if (d.NX == 128)
         hipLaunchKernelGGL((k_stageA<128, false>), dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master,
-                           c->d_slave, d_win, c->amp, c->sums, (float2*)nullptr, (const float2*)c->tabs);
+                           c->d_slave, d_win, b.amp, b.sums, (float2*)nullptr, (const float2*)c->tabs);
       else
         hipLaunchKernelGGL((k_stageA<512, false>), dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master,
-                           c->d_slave, d_win, c->amp, c->sums, (float2*)nullptr, (const float2*)c->tabs);
+                           c->d_slave, d_win, b.amp, b.sums, (float2*)nullptr, (const float2*)c->tabs);
     });
   else if (fusedA)
     rc = launch_timed(c, timing, "k_stageA256", [&] {
       hipLaunchKernelGGL(k_stageA256, dim3(nb), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_win,
-                         c->amp, c->sums, c->Z, (const float2*)c->tabs);
+                         b.amp, b.sums, b.Z, (const float2*)c->tabs);
     });
   else if (row256 && d.ri == 2)
     rc = launch_timed(c, timing, "k_range256", [&] {
       hipLaunchKernelGGL(k_range256, dim3(nb * 2), dim3(kThreads), 0, c->stream, d, c->d_master, c->d_slave, d_win,
-                         c->amp, c->sums, (const float2*)c->tabs);
+                         b.amp, b.sums, (const float2*)c->tabs);
     });
   else
     rc = launch_timed(c, timing, "k_range_ampl", [&] {
       hipLaunchKernelGGL(k_range_ampl, dim3(nb * 2 * (d.NY / d.G1)), dim3(c->nt_rows), sh1, c->stream, d, c->d_master,
-                         c->d_slave, d_win, c->amp, c->sums, (const float2*)c->tabs);
+                         c->d_slave, d_win, b.amp, b.sums, (const float2*)c->tabs);
     });
   if (rc) return rc;
   if (freq) {
@@ -3457,38 +3485,38 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
       rc = 0;
     else if (row256)
       rc = launch_timed(c, timing, "k_rowfft256", [&] {
-        hipLaunchKernelGGL(k_rowfft256, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float*)c->amp,
-                           (const unsigned long long*)c->sums, c->Z, (const float2*)c->tabs);
+        hipLaunchKernelGGL(k_rowfft256, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float*)b.amp,
+                           (const unsigned long long*)b.sums, b.Z, (const float2*)c->tabs);
       });
     else
       rc = launch_timed(c, timing, "k_rowfft", [&] {
         hipLaunchKernelGGL(k_rowfft, dim3(nb * (d.NY / d.G1)), dim3(c->nt_rows), nel1 * sizeof(float2) * 2, c->stream,
-                           d, (const float*)c->amp, (const unsigned long long*)c->sums, c->Z, (const float2*)c->tabs);
+                           d, (const float*)b.amp, (const unsigned long long*)b.sums, b.Z, (const float2*)c->tabs);
       });
     if (rc) return rc;
     const size_t nel3 = (size_t)d.C3 * d.NY;
     if (regA) {
       rc = launch_timed(c, timing, "k_colpass", [&] {
         if (d.NX == 128)
-          launch_colpass<128>(c, d, c->stream, nullptr, nb);
+          launch_colpass<128>(c, b, d, c->stream, nullptr, nb);
         else
-          launch_colpass<512>(c, d, c->stream, nullptr, nb);
+          launch_colpass<512>(c, b, d, c->stream, nullptr, nb);
       });
     } else if (col256 && d.NX >= 32) {
       rc = launch_timed(c, timing, "k_colpass256", [&] {
         hipLaunchKernelGGL(k_colpass256, dim3((nb * (d.NX / 32) + 7) & ~7), dim3(kThreads), 0, c->stream, d,
-                           (const float2*)c->Z, (const float2*)nullptr, (const int*)nullptr,
-                           (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs, nb * (d.NX / 32));
+                           (const float2*)b.Z, (const float2*)nullptr, (const int*)nullptr,
+                           (const unsigned long long*)b.sums, b.D, (const float2*)c->tabs, nb * (d.NX / 32));
       });
     } else {
       rc = launch_timed(c, timing, "k_colfft", [&] {
         hipLaunchKernelGGL(k_colfft, dim3(nb * (d.NX / d.C3)), dim3(c->nt_rows), nel3 * sizeof(float2) * 2,
-                           c->stream, d, c->Z, (const unsigned long long*)c->sums, (const float2*)c->tabs);
+                           c->stream, d, b.Z, (const unsigned long long*)b.sums, (const float2*)c->tabs);
       });
       if (rc) return rc;
       rc = launch_timed(c, timing, "k_product_colifft", [&] {
         hipLaunchKernelGGL(k_product_colifft, dim3(nb * (d.NX / d.C3)), dim3(c->nt_rows), nel3 * sizeof(float2) * 3,
-                           c->stream, d, (const float2*)c->Z, c->D, (const float2*)c->tabs);
+                           c->stream, d, (const float2*)b.Z, b.D, (const float2*)c->tabs);
       });
     }
     if (rc) return rc;
@@ -3496,39 +3524,39 @@ static int run_batch(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int nb, xc
     if (regA)
       rc = launch_timed(c, timing, "k_rowifft_surf_reg", [&] {
         if (d.NX == 128)
-          hipLaunchKernelGGL(k_rowifft_surf_reg<128>, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)c->D,
-                             c->surf, c->peak, c->partial, (const float2*)c->tabs);
+          hipLaunchKernelGGL(k_rowifft_surf_reg<128>, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)b.D,
+                             b.surf, b.peak, b.partial, (const float2*)c->tabs);
         else
-          hipLaunchKernelGGL(k_rowifft_surf_reg<512>, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)c->D,
-                             c->surf, c->peak, c->partial, (const float2*)c->tabs);
+          hipLaunchKernelGGL(k_rowifft_surf_reg<512>, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)b.D,
+                             b.surf, b.peak, b.partial, (const float2*)c->tabs);
       });
     else if (row256)
       rc = launch_timed(c, timing, "k_rowifft_surf256", [&] {
-        hipLaunchKernelGGL(k_rowifft_surf256, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)c->D,
-                           c->surf, c->peak, c->partial, (const float2*)c->tabs);
+        hipLaunchKernelGGL(k_rowifft_surf256, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)b.D,
+                           b.surf, b.peak, b.partial, (const float2*)c->tabs);
       });
     else
       rc = launch_timed(c, timing, "k_rowifft_surf", [&] {
         hipLaunchKernelGGL(k_rowifft_surf, dim3(nb * d.groups4), dim3(kThreads), nel4 * sizeof(float2) * 2,
-                           c->stream, d, (const float2*)c->D, c->surf, c->peak, c->partial, (const float2*)c->tabs);
+                           c->stream, d, (const float2*)b.D, b.surf, b.peak, b.partial, (const float2*)c->tabs);
       });
     if (rc) return rc;
   } else {
-    rc = time_search(c, d_win, nb, timing);
+    rc = time_search(c, b, nb, timing);
     if (rc) return rc;
   }
   const size_t sh5 = d.interp_flag ? (size_t)d.Lx * 16 * sizeof(float2) : 16;
   rc = launch_timed(c, timing, "k_finish", [&] {
-    hipLaunchKernelGGL(k_finish, dim3((nb + 7) & ~7), dim3(kThreads), sh5, c->stream, d, (const float*)c->amp,
-                       (const float*)c->amp + d.N, (const int*)nullptr, (long long)2 * d.N, (const int*)nullptr,
-                       (const unsigned long long*)c->sums, (const float*)c->surf, (const double*)c->surfd,
-                       (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)c->tpeak,
-                       (const double*)c->tcave, d_out, (const float2*)c->tabs, nb);
+    hipLaunchKernelGGL(k_finish, dim3((nb + 7) & ~7), dim3(kThreads), sh5, c->stream, d, (const float*)b.amp,
+                       (const float*)b.amp + d.N, (const int*)nullptr, (long long)2 * d.N, (const int*)nullptr,
+                       (const unsigned long long*)b.sums, (const float*)b.surf, (const double*)b.surfd,
+                       (const unsigned long long*)b.peak, (const double*)b.partial, (const int*)b.tpeak,
+                       (const double*)b.tcave, d_out, (const float2*)c->tabs, nb);
   });
   return rc;
 }
 
-static int time_search(xcorr_hip_ctx* c, const xcorr_hip_window*, int nb, bool timing) {
+static int time_search(xcorr_hip_ctx* c, const PatchBufs& b, int nb, bool timing) {
   const DevParams& d = c->dp;
   const size_t sh = ((size_t)(d.tRC + d.tLR - 1) * d.NX + (size_t)d.tRC * d.nx_corr) * sizeof(int);
   const dim3 grid(nb * (d.nyc / d.tLR));
@@ -3538,31 +3566,31 @@ static int time_search(xcorr_hip_ctx* c, const xcorr_hip_window*, int nb, bool t
   constexpr int TL = 8;                           // lags per thread
   const int LRb = std::max(1, std::min(d.nyc, kThreads * TL / d.nxc));
   const int RCb = std::min(8, (5120 - (LRb - 1) * d.NX) / (d.NX + d.nx_corr));   // <= 40 KB of LDS
-  const bool i8 = d.corr_flag == XCORR_HIP_TIME && d.xs == 32 && d.ys == 32 && c->tcaa;   // peak found in-kernel
+  const bool i8 = d.corr_flag == XCORR_HIP_TIME && d.xs == 32 && d.ys == 32 && b.tcaa;   // peak found in-kernel
   if (i8)
     rc = launch_timed(c, timing, "k_time_search", [&] {
-      launch_time_i8(c, d, c->stream, (const float*)c->amp, (const float*)c->amp, nullptr, nb);
+      launch_time_i8(b, d, c->stream, (const float*)b.amp, (const float*)b.amp, nullptr, nb);
     });
   else if (d.corr_flag == XCORR_HIP_TIME && d.nxc % TL == 0 && RCb >= 1)
     rc = launch_timed(c, timing, "k_time_search", [&] {
       const size_t shb = ((size_t)(RCb + LRb - 1) * d.NX + (size_t)RCb * d.nx_corr) * sizeof(double);
       hipLaunchKernelGGL(k_time_search_blk<TL>, dim3(nb * (d.nyc / LRb)), dim3(kThreads), shb, c->stream, d,
-                         (const float*)c->amp, (const unsigned long long*)c->sums, c->surfd, LRb, RCb);
+                         (const float*)b.amp, (const unsigned long long*)b.sums, b.surfd, LRb, RCb);
     });
   else if (d.corr_flag == XCORR_HIP_TIME)
     rc = launch_timed(c, timing, "k_time_search", [&] {
-      hipLaunchKernelGGL(k_time_search<0>, grid, dim3(kThreads), sh, c->stream, d, (const float*)c->amp,
-                         (const unsigned long long*)c->sums, c->surfd);
+      hipLaunchKernelGGL(k_time_search<0>, grid, dim3(kThreads), sh, c->stream, d, (const float*)b.amp,
+                         (const unsigned long long*)b.sums, b.surfd);
     });
   else
     rc = launch_timed(c, timing, "k_time_search4", [&] {
-      hipLaunchKernelGGL(k_time_search<1>, grid, dim3(kThreads), sh, c->stream, d, (const float*)c->amp,
-                         (const unsigned long long*)c->sums, c->surfd);
+      hipLaunchKernelGGL(k_time_search<1>, grid, dim3(kThreads), sh, c->stream, d, (const float*)b.amp,
+                         (const unsigned long long*)b.sums, b.surfd);
     });
   if (rc || i8) return rc;
   return launch_timed(c, timing, "k_time_peak", [&] {
-    hipLaunchKernelGGL(k_time_peak, dim3(nb), dim3(kThreads), 0, c->stream, d, (const double*)c->surfd, c->tpeak,
-                       c->tcave);
+    hipLaunchKernelGGL(k_time_peak, dim3(nb), dim3(kThreads), 0, c->stream, d, (const double*)b.surfd, b.tpeak,
+                       b.tcave);
   });
 }
 
@@ -3727,6 +3755,99 @@ extern "C" int xcorr_hip_correlate_stream(xcorr_hip_ctx* c, xcorr_hip_line_reade
 // form is exact or keeps its order.  A radix plan other than the default is a
 // different FP32 rounding of the transforms of that length.
 // --------------------------------------------------------------------------
+// --------------------------------------------------------------------------
+// xcorr_hip_correlate_noalloc: the per-patch kernels on buffers carved from
+// one static device array per GPU, for callers that cannot allocate device
+// memory while they run.
+// --------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = XCORR_HIP_NOALLOC_SCRATCH;
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kNoAllocDevices];   // device address of each GPU's copy of `scratch`
+
+// Address of the current device's copy of `scratch`.  In host code the name
+// `scratch` is a host-side stand-in, never a device address: the address is
+// looked up once per device (with that device current) and only this pointer
+// is handed to kernels.
+static int noalloc_scratch(xcorr_hip_ctx* c, unsigned char** out) {
+  if (c->device < 0 || c->device >= kNoAllocDevices)
+    return fail(c, XCORR_HIP_EINVAL, "correlate_noalloc: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[c->device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(c, XCORR_HIP_EDEVICE, std::string("correlate_noalloc: hipGetSymbolAddress(scratch): ") +
+                                            hipGetErrorString(e));
+    g_scratch_ptr[c->device] = ptr;
+  }
+  *out = (unsigned char*)g_scratch_ptr[c->device];
+  return 0;
+}
+
+// The per-patch buffers of nb windows laid out one after another from base,
+// each on a 256-byte boundary (the same per-patch strides as a context's own
+// buffers).  Returns the bytes they take; with base null only the size.
+static size_t carve_patch_bufs(const DevParams& d, bool i8, size_t nb, unsigned char* base, PatchBufs* b) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> void* {
+    void* ptr = base ? (void*)(base + off) : nullptr;
+    off += (size_t)round_up((long long)bytes, 256);
+    return ptr;
+  };
+  *b = PatchBufs{};
+  const size_t N = (size_t)d.N, ns = (size_t)d.nxc * d.nyc;
+  b->amp = (float*)take(nb * 2 * N * sizeof(float));
+  b->sums = (unsigned long long*)take(nb * 2 * sizeof(unsigned long long));
+  b->peak = (unsigned long long*)take(nb * sizeof(unsigned long long));
+  if (d.corr_flag == XCORR_HIP_FREQ) {
+    b->Z = (float2*)take(nb * N * sizeof(float2));
+    b->D = (float2*)take(nb * d.nyc * d.NX * sizeof(float2));
+    b->surf = (float*)take(nb * ns * sizeof(float));
+    b->partial = (double*)take(nb * std::max(d.groups4, d.nyc / d.G4) * sizeof(double));
+  } else {
+    b->surfd = (double*)take(nb * ns * sizeof(double));
+    if (i8) b->tcaa = (double*)take(nb * ns * sizeof(double));
+    b->tpeak = (int*)take(nb * 4 * sizeof(int));
+    b->tcave = (double*)take(nb * sizeof(double));
+  }
+  return off;
+}
+
+extern "C" int xcorr_hip_correlate_noalloc(xcorr_hip_ctx* c, const xcorr_hip_window* d_win, int n,
+                                           xcorr_hip_result* d_out, unsigned flags) {
+  if (!c) return XCORR_HIP_EINVAL;
+  if (n < 0 || (n > 0 && (!d_win || !d_out))) return fail(c, XCORR_HIP_EINVAL, "correlate_noalloc: bad arguments");
+  if (flags & ~XCORR_HIP_NO_SYNC)
+    return fail(c, XCORR_HIP_EINVAL, "correlate_noalloc: the only accepted flag is XCORR_HIP_NO_SYNC");
+  if (!c->images) return fail(c, XCORR_HIP_ESTATE, "correlate_noalloc: no images set");
+  HIPCK(c, hipSetDevice(c->device));
+  if (n == 0) return 0;
+  unsigned char* base = nullptr;
+  const int rs = noalloc_scratch(c, &base);
+  if (rs) return rs;
+  const DevParams& d = c->dp;
+  const bool i8 = d.corr_flag == XCORR_HIP_TIME && d.xs == 32 && d.ys == 32;
+  PatchBufs b;
+  const size_t one = carve_patch_bufs(d, i8, 1, nullptr, &b);
+  if (one > kNoAllocBytes)
+    return fail(c, XCORR_HIP_EINVAL, "correlate_noalloc: the buffers of one " + std::to_string(d.NX) + " x " +
+                                         std::to_string(d.NY) + " window (" + std::to_string(one) +
+                                         " bytes) exceed the static array (" + std::to_string(kNoAllocBytes) +
+                                         " bytes)");
+  size_t cap = kNoAllocBytes / one;              // windows per internal batch
+  while (cap > 1 && carve_patch_bufs(d, i8, cap, nullptr, &b) > kNoAllocBytes) cap--;
+  carve_patch_bufs(d, i8, cap, base, &b);
+  for (int off = 0; off < n; off += (int)cap) {
+    const int nb = (int)std::min<size_t>(cap, (size_t)(n - off));
+    const int rc = run_batch(c, b, d_win + off, nb, d_out + off, false);
+    if (rc) return rc;
+  }
+  if (!(flags & XCORR_HIP_NO_SYNC)) HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 static size_t lds_rows(const DevParams& d, int G) {     // k_range_ampl (the larger of the two row kernels)
   const size_t nel = (size_t)G * d.NX;
   return std::max(nel * sizeof(float2) * (d.ri > 2 ? 3 : 2) + nel * sizeof(float), nel * sizeof(float2) * 2);
@@ -4121,7 +4242,7 @@ static int multi_build(xcorr_hip_multi* m, unsigned flags) {
         rc = launch_strip_batch(c, c->dp, q.master, q.slave, c->stream, (const int*)m->dbuf[k] + b.off, b.sb, q.out,
                                 true, false);
       else
-        rc = run_batch(c, (const xcorr_hip_window*)m->dbuf[k] + b.off, b.nb, q.out + b.first, false);
+        rc = run_batch(c, ctx_bufs(c), (const xcorr_hip_window*)m->dbuf[k] + b.off, b.nb, q.out + b.first, false);
     }
     if (e == hipSuccess && !rc) e = hipEventRecord(m->events[1 + k], c->stream);
     if (e == hipSuccess && !rc) e = hipStreamWaitEvent(m->stream, m->events[1 + k], 0);

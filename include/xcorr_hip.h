@@ -123,6 +123,30 @@ int xcorr_hip_set_images(xcorr_hip_ctx *ctx, const void *master, int m_nx, int m
 int xcorr_hip_correlate(xcorr_hip_ctx *ctx, const xcorr_hip_window *win, int n,
                         xcorr_hip_result *out, unsigned flags);
 
+/* Bytes of the static device array xcorr_hip_correlate_noalloc works in (one
+ * per GPU the library runs on, held until the process ends). */
+#define XCORR_HIP_NOALLOC_SCRATCH (64u << 20)
+
+/* Correlate n locations without allocating anything.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory while they
+ * run -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The results equal xcorr_hip_correlate's bit for bit.
+ * Windows (d_win) and results (d_out) are device memory; the images must be
+ * set.  Every per-batch buffer (amplitudes, spectra, lag rows, surfaces,
+ * sums, peaks) is carved from one static device array of
+ * XCORR_HIP_NOALLOC_SCRATCH bytes per GPU; the call loops over internal
+ * batches of as many windows as fit in it (256 x 256 windows: 48), using the
+ * per-patch kernels (no strip sharing, no second context, no host staging).
+ * A window shape whose buffers exceed the array is rejected with
+ * XCORR_HIP_EINVAL.  No hipMalloc, hipFree, memset node or allocating routine
+ * is called, and with XCORR_HIP_NO_SYNC (the only accepted flag) nothing waits:
+ * the kernels are queued on xcorr_hip_stream(ctx).  All calls on one GPU share
+ * its array, which keeps the last call's contents: calls that overlap in time
+ * on one GPU (other contexts, streams or threads) race and must be serialised
+ * by the caller. */
+int xcorr_hip_correlate_noalloc(xcorr_hip_ctx *ctx, const xcorr_hip_window *d_win, int n,
+                                xcorr_hip_result *d_out, unsigned flags);
+
 /* Streamed images (pairs larger than device memory, or file reads overlapped
  * with the correlation).  reader(user, image, first, count, dst) copies lines
  * [first, first + count) of image 0 (master, m_nx samples per line) or image 1
