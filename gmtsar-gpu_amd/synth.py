@@ -124,13 +124,70 @@ def strip_pair_torch(n_range, n_azimuth, dx_fn, dy_fn, coh_range=(0.3, 0.95), bl
     return q(m), q(s)
 
 
-def dense_pair_torch(n_range, n_azimuth, field, coherence, block=128, margin=64, seed=11, device="cuda"):
-    """Pair with a spatially varying offset field and coherence, generated on
-    the GPU: each block x block tile is Fourier-shifted by field(xc, yc) ->
-    (dx, dy) at its centre (per-tile Fourier resampling with a `margin`-sample
-    apron) and mixed with noise at coherence(xc, yc).  field and coherence take
-    numpy arrays of tile-centre coordinates.  Returns (master, slave) as
-    (lines, samples, 2) int16 tensors on `device`."""
+def sinusoid_field(rng, n=8, amp=15.0, wl=(133.0, 667.0)):
+    """Smooth random offset field: sum of n random 2-D sinusoids per component,
+    total amplitude <= amp px, wavelengths in wl px (2-10 km at 15 m pixels).
+    The returned f(x, y) -> (dx, dy) takes numpy arrays or torch tensors."""
+    comps = []
+    for _ in range(2):
+        a = rng.uniform(0.2, 1.0, n)
+        a *= amp / a.sum()
+        lam = rng.uniform(wl[0], wl[1], n)
+        th = rng.uniform(0, 2 * np.pi, n)
+        ph = rng.uniform(0, 2 * np.pi, n)
+        comps.append((a, 2 * np.pi / lam * np.cos(th), 2 * np.pi / lam * np.sin(th), ph))
+
+    def f(x, y):
+        sin = np.sin if isinstance(x, np.ndarray) or np.isscalar(x) else type(x).sin
+        out = []
+        for a, kx, ky, ph in comps:
+            v = 0.0 * x
+            for i in range(n):
+                v = v + float(a[i]) * sin(float(kx[i]) * x + float(ky[i]) * y + float(ph[i]))
+            out.append(v)
+        return out[0], out[1]
+    return f
+
+
+SINC_TAPS = 32          # windowed-sinc resampling kernel: taps -15 .. 16 around floor(position)
+KAISER_BETA = 6.0       # 3.9 % error power on full-band speckle against an exact Fourier shift
+
+
+def _resample_1d(torch, src, pos, axis):
+    """Samples of the complex image src at fractional positions pos along
+    `axis` (0: lines, 1: samples), the other coordinate being the output's
+    own; Kaiser-windowed sinc of SINC_TAPS taps, zero outside the image."""
+    import math
+    n = src.shape[axis]
+    i0 = torch.floor(pos)
+    fr = pos - i0
+    i0 = i0.to(torch.int64)
+    out = torch.zeros(pos.shape, dtype=src.dtype, device=src.device)
+    half = SINC_TAPS // 2
+    den = float(np.i0(KAISER_BETA))
+    for k in range(-half + 1, half + 1):
+        t = fr - k                                          # distance from tap to position
+        w = torch.sinc(t) * torch.special.i0(KAISER_BETA * torch.sqrt(torch.clamp(1 - (t / half) ** 2, min=0))) / den
+        idx = i0 + k
+        ok = (idx >= 0) & (idx < n)
+        idx = idx.clamp(0, n - 1)
+        if axis == 0:
+            v = torch.gather(src, 0, idx)
+        else:
+            v = torch.gather(src, 1, idx)
+        out += torch.where(ok, w, torch.zeros_like(w)).to(src.dtype) * v
+    return out
+
+
+def field_pair_torch(n_range, n_azimuth, field, coherence, seed=11, device="cuda", rows=256):
+    """Pair with a smooth, spatially varying offset field, resampled per sample
+    on the GPU.  The slave is the master resampled in two separable passes
+    (windowed sinc, SINC_TAPS taps): first along azimuth, s1(x, y) =
+    m(x, y - dy(x, y)), then along range, s(x, y) = s1(x - dx(x, y), y), and
+    mixed with noise at coherence(x, y).  The displacement of output sample
+    (x, y) is therefore exactly (dx(x, y), dy(x - dx(x, y), y))
+    (field_displacement).  field and coherence take torch tensors.  Returns
+    (master, slave) as (lines, samples, 2) int16 tensors on `device`."""
     import math
 
     import torch
@@ -145,34 +202,23 @@ def dense_pair_torch(n_range, n_azimuth, field, coherence, block=128, margin=64,
     m = torch.complex(randn(ny, nx), randn(ny, nx)) / math.sqrt(2.0)
     m = m * torch.exp(SIGMA_LN * randn(ny, nx) - SIGMA_LN ** 2 / 2.0)
     power = float((m.real.square() + m.imag.square()).mean())
-    T = block + 2 * margin
-    nby, nbx = -(-ny // block), -(-nx // block)
-    mp = torch.zeros(nby * block + 2 * margin, nbx * block + 2 * margin, dtype=torch.complex64, device=device)
-    mp[margin:margin + ny, margin:margin + nx] = m
-    s = torch.empty(nby * block, nbx * block, dtype=torch.complex64, device=device)
-    kx = torch.fft.fftfreq(T, device=device)[None, None, :]
-    ky = torch.fft.fftfreq(T, device=device)[None, :, None]
-    xc = np.arange(nbx) * block + block / 2.0
-    rows = 16                                         # tile rows per FFT batch
-    for by0 in range(0, nby, rows):
-        nr = min(rows, nby - by0)
-        strip = mp[by0 * block: (by0 + nr) * block + 2 * margin]
-        tiles = strip.unfold(0, T, block).unfold(1, T, block)          # (nr, nbx, T, T)
-        yc = (by0 + np.arange(nr)) * block + block / 2.0
-        X, Y = np.meshgrid(xc, yc)
-        dx, dy = field(X, Y)
-        gam = np.clip(coherence(X, Y), 0.0, 1.0)
-        dxt = torch.as_tensor(dx, device=device, dtype=torch.float32).reshape(-1, 1, 1)
-        dyt = torch.as_tensor(dy, device=device, dtype=torch.float32).reshape(-1, 1, 1)
-        F = torch.fft.fft2(tiles.reshape(-1, T, T))
-        F = F * torch.exp(-2j * math.pi * (kx * dxt + ky * dyt)).to(torch.complex64)
-        sh = torch.fft.ifft2(F)[:, margin:margin + block, margin:margin + block]
-        gt = torch.as_tensor(gam, device=device, dtype=torch.float32).reshape(-1, 1, 1)
-        w = torch.complex(randn(*sh.shape), randn(*sh.shape)) * math.sqrt(power / 2.0)
-        sh = gt * sh + torch.sqrt(1.0 - gt * gt) * w
-        s[by0 * block:(by0 + nr) * block] = sh.reshape(nr, nbx, block, block).permute(0, 2, 1, 3).reshape(
-            nr * block, nbx * block)
-    s = s[:ny, :nx]
+    xs = torch.arange(nx, device=device, dtype=torch.float32)
+    s1 = torch.empty_like(m)
+    for y0 in range(0, ny, rows):
+        yy = torch.arange(y0, min(ny, y0 + rows), device=device, dtype=torch.float32)
+        Y, X = torch.meshgrid(yy, xs, indexing="ij")
+        _, dy = field(X, Y)
+        lo, hi = max(0, y0 - 64), min(ny, y0 + rows + 64)      # |dy| <= 15 px plus the taps
+        s1[y0:y0 + len(yy)] = _resample_1d(torch, m[lo:hi], Y - dy - lo, 0)
+    s = torch.empty_like(m)
+    for y0 in range(0, ny, rows):
+        yy = torch.arange(y0, min(ny, y0 + rows), device=device, dtype=torch.float32)
+        Y, X = torch.meshgrid(yy, xs, indexing="ij")
+        dx, _ = field(X, Y)
+        gam = torch.clamp(coherence(X, Y), 0.0, 1.0)
+        v = _resample_1d(torch, s1[y0:y0 + len(yy)], X - dx, 1)
+        w = torch.complex(randn(*v.shape), randn(*v.shape)) * math.sqrt(power / 2.0)
+        s[y0:y0 + len(yy)] = gam * v + torch.sqrt(1.0 - gam * gam) * w
     scale = STD_IQ / math.sqrt(power / 2.0)
 
     def q(z):
@@ -184,27 +230,25 @@ def dense_pair_torch(n_range, n_azimuth, field, coherence, block=128, margin=64,
     return q(m), q(s)
 
 
-def sinusoid_field(rng, n=8, amp=15.0, wl=(133.0, 667.0)):
-    """Smooth random offset field: sum of n random 2-D sinusoids per component,
-    total amplitude <= amp px, wavelengths in wl px (2-10 km at 15 m pixels)."""
-    comps = []
-    for _ in range(2):
-        a = rng.uniform(0.2, 1.0, n)
-        a *= amp / a.sum()
-        lam = rng.uniform(wl[0], wl[1], n)
-        th = rng.uniform(0, 2 * np.pi, n)
-        ph = rng.uniform(0, 2 * np.pi, n)
-        comps.append((a, 2 * np.pi / lam * np.cos(th), 2 * np.pi / lam * np.sin(th), ph))
+def field_displacement(field, x, y):
+    """Displacement of slave sample (x, y) in a field_pair_torch pair."""
+    dx, _ = field(x, y)
+    _, dy = field(x - dx, y)
+    return dx, dy
 
-    def f(x, y):
-        out = []
-        for a, kx, ky, ph in comps:
-            v = np.zeros(np.shape(x))
-            for i in range(n):
-                v = v + a[i] * np.sin(kx[i] * x + ky[i] * y + ph[i])
-            out.append(v)
-        return out[0], out[1]
-    return f
+
+def window_truth(field, x, y, half, k=8):
+    """Offset a window centred on (x, y) with a template of 2 half x 2 half
+    samples should report: the mean displacement over the template, on a
+    k x k grid of sample centres.  Also returns the spread of the displacement
+    over the template (max - min per component)."""
+    t = (np.arange(k) + 0.5) / k * (2 * half) - half
+    gx = np.asarray(x, np.float64)[:, None, None] + t[None, None, :]
+    gy = np.asarray(y, np.float64)[:, None, None] + t[None, :, None]
+    gx, gy = np.broadcast_arrays(gx, gy)
+    dx, dy = field_displacement(field, gx, gy)
+    return (dx.mean(axis=(1, 2)), dy.mean(axis=(1, 2)),
+            dx.max(axis=(1, 2)) - dx.min(axis=(1, 2)), dy.max(axis=(1, 2)) - dy.min(axis=(1, 2)))
 
 
 def smooth_coherence(rng, lo=0.15, hi=0.95, n=6, wl=(1000.0, 6000.0)):
@@ -215,8 +259,10 @@ def smooth_coherence(rng, lo=0.15, hi=0.95, n=6, wl=(1000.0, 6000.0)):
     ph = rng.uniform(0, 2 * np.pi, n)
 
     def f(x, y):
-        v = np.zeros(np.shape(x))
+        mod = np if isinstance(x, np.ndarray) or np.isscalar(x) else type(x)
+        v = 0.0 * x
         for i in range(n):
-            v = v + np.sin(2 * np.pi / lam[i] * (np.cos(th[i]) * x + np.sin(th[i]) * y) + ph[i])
-        return lo + (hi - lo) / (1.0 + np.exp(-1.5 * v))
+            v = v + mod.sin(float(2 * np.pi / lam[i]) * (float(np.cos(th[i])) * x + float(np.sin(th[i])) * y)
+                            + float(ph[i]))
+        return lo + (hi - lo) / (1.0 + mod.exp(-1.5 * v))
     return f

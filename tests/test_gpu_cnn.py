@@ -8,7 +8,9 @@ restatement (oracle/cnn_oracle.py).
   backward pass (relative error of each group's gradient < 2 %, cosine >
   0.999), so a wrong tap, stride, transpose or lane layout fails.
 * Box-downsampling and normalised crops agree with their closed forms.
-* Adam steps on the GPU drive the loss of a fixed batch down.
+* Adam steps on the GPU drive the loss of a fixed batch down, and two steps
+  move every parameter as a NumPy Adam (beta 0.9 / 0.999, eps 1e-8, bias
+  correction) computes from the GPU's own gradients.
 """
 import numpy as np
 import pytest
@@ -136,3 +138,42 @@ def test_adam_steps_reduce_loss(env):
     print("loss %.4f -> %.4f" % (first, last))
     assert np.all(np.isfinite(p))
     assert last < 0.5 * first
+
+
+def test_adam_update_matches_numpy(env):
+    torch, prop, co = env
+    n = 32
+    x = _inputs(n, 12)
+    rng = np.random.default_rng(13)
+    tgt = np.stack([rng.uniform(-2, 2, n), rng.uniform(-2, 2, n), rng.integers(0, 2, n), np.ones(n),
+                    np.ones(n)], 1).astype(np.float32)
+    lr = 1e-3
+    with prop.ProposalNet(max_batch=n) as net:
+        p0 = co.init_params(17)
+        net.set_params(p0)
+        x0 = torch.from_numpy(np.ascontiguousarray(_to_bf16_bits(x))).cuda()
+        t = torch.from_numpy(tgt).cuda()
+        _, g1 = net.gradient(x0, t)
+        net.train_step(x0, t, lr)
+        p1 = net.get_params()
+        _, g2 = net.gradient(x0, t)
+        net.train_step(x0, t, lr)
+        p2 = net.get_params()
+    f = np.float64
+    m = v = np.zeros(p0.size)
+    want = [p0.astype(f)]
+    for step, g in ((1, g1.astype(f)), (2, g2.astype(f))):
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * g * g
+        want.append(want[-1] - lr * (m / (1 - 0.9 ** step)) / (np.sqrt(v / (1 - 0.999 ** step)) + 1e-8))
+    # the weight gradients accumulate with FP32 atomics, so the gradient a step
+    # uses may differ from the one fetched before it in the last bits: compare
+    # the updates where the gradient is well away from zero
+    for k, (got, prev) in enumerate(((p1, p0), (p2, p1))):
+        upd, ref = got.astype(f) - prev.astype(f), want[k + 1] - want[k]
+        sel = (np.abs(g1) > 1e-4 * np.abs(g1).max()) & (np.abs(g2) > 1e-4 * np.abs(g2).max())
+        assert sel.sum() > 0.8 * p0.size
+        # plus one float32 ulp of the parameter: the stored parameter is rounded
+        tol = 2e-3 * np.abs(ref) + np.spacing(np.abs(got)).astype(f)
+        bad = np.nonzero(sel & (np.abs(upd - ref) > tol))[0]
+        assert bad.size == 0, "step %d: %s gpu %s numpy %s" % (k + 1, bad[:4], upd[bad[:4]], ref[bad[:4]])

@@ -7,11 +7,22 @@
   With verification on, exactly the shrunken windows whose peak lies on the
   lag-window border or whose correlation is below the threshold are redone,
   and equal the plain engine's results.
-* The trained network (gmtsar-gpu_amd/weights) gains over the method without
-  it: on a pair whose offsets (17..27 px in range, -16..-26 px in azimuth) lie
-  beyond the shrunken search's reach from a zero prior, its predictions are
-  closer to plain xcorr than an untrained network's, and at its prior the
-  shrunken search reproduces plain xcorr on far more windows.
+* The trained network (gmtsar-gpu_amd/weights) against an untrained one on a
+  pair whose offset field spans both signs (-18..+12 px in range, +14..-16 px
+  in azimuth, inside the training range |d| <= 30): its predictions are far
+  closer to plain xcorr (measured: median 4.3 px against 10.8 px).  The
+  shrunken search (xsearch 32: +-16 px in range, +-32 px in azimuth) reaches
+  plain xcorr's peak from either prior on this field (95 % and 96 % of the
+  windows agree within 0.125 px), so there the network's prior gains nothing
+  and the test only requires that it does not lose.
+* Where the offsets lie beyond the shrunken search's reach from a zero prior
+  (+17..+27 px in range), the trained network's prior is what lets the
+  shrunken search find plain xcorr's peak.
+* The proposal form against the method without the network (plain xcorr) on
+  the same pair at the default threshold: offset RMSE against plain xcorr
+  within 0.05 px; the throughput ratio is printed (it is below 1 on these
+  shapes: the network's cost and the displaced windows' lost line sharing
+  outweigh the smaller searches).
 """
 import os
 
@@ -101,12 +112,13 @@ def test_proposal_paths_are_plain_xcorr_runs(xg, synth, env):
         pr.close()
 
 
-def test_trained_network_gains_over_untrained(xg, synth, env):
+@pytest.fixture(scope="module")
+def signed_pair(xg, synth, env):
+    """A 2048 x 4096 pair whose offsets change sign across the image."""
     torch, prop, train = env
     dev = torch.device("cuda:0")
     nx, ny = 2048, 4096
-    # offsets beyond the shrunken search's reach from a zero prior (|dx| > 16 px)
-    m, s = synth.strip_pair_torch(nx, ny, lambda x: 17.0 + 10.0 * x / nx, lambda y: -16.0 - 10.0 * y / ny,
+    m, s = synth.strip_pair_torch(nx, ny, lambda x: -18.0 + 30.0 * x / nx, lambda y: 14.0 - 30.0 * y / ny,
                                   coh_range=(0.5, 0.95), seed=4242, device=dev)
     torch.cuda.synchronize()
     gx, gy = np.arange(0, nx - 255, 96), np.arange(0, ny - 255, 96)
@@ -117,7 +129,14 @@ def test_trained_network_gains_over_untrained(xg, synth, env):
     with xg.Engine(xg.make_params(64, 64), max_batch=1024) as full:
         full.set_images(m, s)
         plain = full.correlate(w)
+    return m, s, w, plain
+
+
+def test_trained_network_gains_over_untrained(xg, env, signed_pair):
+    torch, prop, train = env
+    m, s, w, plain = signed_pair
     pdx, pdy = prop.compose(plain, w, 2)
+    assert pdx.min() < -10 and pdx.max() > 5 and pdy.min() < -10 and pdy.max() > 5
     good = plain["corr"] > 30
     stats = {}
     for name, params in (("trained", np.load(WEIGHTS, allow_pickle=False)), ("untrained", train.init_params(5))):
@@ -135,4 +154,63 @@ def test_trained_network_gains_over_untrained(xg, synth, env):
             pr.close()
     print("median |prediction - plain| px, shrunken-search agreement:", stats)
     assert stats["trained"][0] < 0.6 * stats["untrained"][0]
-    assert stats["trained"][1] > stats["untrained"][1] + 0.3
+    assert stats["trained"][1] >= stats["untrained"][1] - 0.05
+
+
+def test_prior_extends_the_shrunken_search(xg, synth, env):
+    torch, prop, train = env
+    dev = torch.device("cuda:0")
+    nx, ny = 2048, 4096
+    m, s = synth.strip_pair_torch(nx, ny, lambda x: 17.0 + 10.0 * x / nx, lambda y: -16.0 - 10.0 * y / ny,
+                                  coh_range=(0.5, 0.95), seed=4243, device=dev)
+    torch.cuda.synchronize()
+    gx, gy = np.arange(0, nx - 255, 128), np.arange(0, ny - 255, 128)
+    yy, xx = np.meshgrid(gy, gx, indexing="ij")
+    w = np.zeros(xx.size, xg.WINDOW_DTYPE)
+    w["mx0"], w["my0"] = xx.ravel(), yy.ravel()
+    w["sx0"], w["sy0"] = w["mx0"], w["my0"]
+    pr = prop.Proposer(np.load(WEIGHTS, allow_pickle=False), max_batch=1024)
+    try:
+        pr.set_images(m, s)
+        plain = pr.full.correlate(w)
+        pdx, pdy = prop.compose(plain, w, 2)
+        good = plain["corr"] > 30
+        agree = {}
+        qx, qy, _ = pr.predict(w)
+        for name, (rx, ry) in (("network prior", (np.rint(qx), np.rint(qy))), ("zero prior", (0 * qx, 0 * qy))):
+            ww = prop.shrunk_windows(w, rx, ry, pr.xs_small)
+            r = pr.small.correlate(ww)
+            sx, sy = prop.compose(r, ww, 2)
+            agree[name] = float(((np.abs(sx - pdx) <= 0.125) & (np.abs(sy - pdy) <= 0.125))[good].mean())
+    finally:
+        pr.close()
+    print("shrunken search agreeing with plain xcorr:", agree)
+    assert agree["network prior"] > agree["zero prior"] + 0.5
+
+
+def test_proposal_form_against_plain_xcorr(xg, env, signed_pair):
+    """The method with the network against the method without it."""
+    import time
+    torch, prop, train = env
+    m, s, w, plain = signed_pair
+    pdx, pdy = prop.compose(plain, w, 2)
+    pr = prop.Proposer(np.load(WEIGHTS, allow_pickle=False), max_batch=1024)
+    try:
+        pr.set_images(m, s)
+        torch.cuda.synchronize()
+        pr.run(w)
+        t0 = time.perf_counter()
+        dx, dy, res, path = pr.run(w)
+        t_prop = time.perf_counter() - t0
+        pr.full.correlate(w)
+        t0 = time.perf_counter()
+        pr.full.correlate(w)
+        t_plain = time.perf_counter() - t0
+    finally:
+        pr.close()
+    e = np.concatenate([dx - pdx, dy - pdy]).astype(np.float64)
+    rmse = float(np.sqrt(np.mean(e ** 2)))
+    print("tau %.2f: shrunken %.1f %% of %d windows, RMSE vs plain %.4f px, speed vs plain x%.2f"
+          % (pr.tau, 100 * (path > 0).mean(), len(w), rmse, t_plain / t_prop))
+    assert (path > 0).any()
+    assert rmse <= 0.05

@@ -196,7 +196,7 @@ def dense(xg, synth):
     field = synth.sinusoid_field(rng)
     coh = synth.smooth_coherence(rng)
     nx, ny = 2048, 4096
-    m, s = synth.dense_pair_torch(nx, ny, field, coh, device="cuda")
+    m, s = synth.field_pair_torch(nx, ny, field, coh, device="cuda")
     torch.cuda.synchronize()
     # x_inc = (2048 - 192) / (57 + 3) = 30, y_inc = (4096 - 192) / (125 + 1) = 30
     x, y = xg.get_locations(nx, ny, 57, 125, 32, 32)
