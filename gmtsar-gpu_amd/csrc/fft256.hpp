@@ -42,22 +42,19 @@ __device__ __forceinline__ void fft256_group(float2 (&v)[16], int j, float2* til
   Dft<16, DIR>::run(v);                       // pass 2 -> X[j + 16 r] = v[r]
 }
 
-// Hermitian completion of a row spectrum held as v[r] = X[j + 16 r]: when
-// `apply`, X[k] = conj(X[256 - k]) for k > 128; `tile` is the group's 16 x 17
-// scratch.
-__device__ __forceinline__ void hermitian_upper256(float2 (&v)[16], int j, float2* tile, bool apply) {
+// The master-only spectrum of an unmasked row held as v[r] = Z[j + 16 r], the
+// transform of z = amp1 + i amp2 (template columns): when `apply`,
+// v[r] = extract_bin(Z[k], Z[256 - k]) for every k = j + 16 r; `tile` is the
+// group's 16 x 17 scratch.
+__device__ __forceinline__ void extract_real256(float2 (&v)[16], int j, float2* tile, bool apply) {
 #pragma unroll
   for (int r = 0; r < 16; r++) tile[r * kT256Pitch + j] = v[r];
   wave_sync();
   if (apply) {
 #pragma unroll
-    for (int r = 8; r < 16; r++) {
-      const int k = j + 16 * r;
-      if (k > 128) {
-        const int m = 256 - k;
-        const float2 w = tile[(m >> 4) * kT256Pitch + (m & 15)];
-        v[r] = make_float2(w.x, -w.y);
-      }
+    for (int r = 0; r < 16; r++) {
+      const int m = (256 - j - 16 * r) & 255;
+      v[r] = extract_bin(v[r], tile[(m >> 4) * kT256Pitch + (m & 15)]);
     }
   }
   wave_sync();

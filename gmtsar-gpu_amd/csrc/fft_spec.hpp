@@ -54,6 +54,12 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+// The spectrum of the real part of a row transformed as z = re + i im, at bin
+// k, from a = Z[k] and q = Z[(n - k) mod n]: ((a.re + q.re) / 2, (a.im - q.im) / 2).
+// Exactly Hermitian: the value at n - k is the conjugate of the value at k.
+__device__ __forceinline__ float2 extract_bin(float2 a, float2 q) {
+  return make_float2((a.x + q.x) * 0.5f, (a.y - q.y) * 0.5f);
+}
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   float t1 = a.y * b.y;

@@ -112,10 +112,11 @@ struct RegFft<512> {
   }
 };
 
-// Hermitian completion of a row spectrum held as v[r] = X[j + G r]: when
-// `apply`, X[k] = conj(X[N - k]) for k > N / 2.  `tile` is the group's.
+// The master-only spectrum of an unmasked row held as v[r] = Z[j + G r] (the
+// transform of z = amp1 + i amp2 on the template columns): when `apply`,
+// v[r] = extract_bin(Z[k], Z[N - k]) for every k.
 template <int N>
-__device__ __forceinline__ void hermitian_upper(float2 (&v)[RegFft<N>::V], int j, float2* tile, bool apply) {
+__device__ __forceinline__ void extract_real(float2 (&v)[RegFft<N>::V], int j, float2* tile, bool apply) {
   constexpr int G = RegFft<N>::G, V = RegFft<N>::V;
 #pragma unroll
   for (int r = 0; r < V; r++) tile[RegFft<N>::slot(j + G * r)] = v[r];
@@ -124,10 +125,7 @@ __device__ __forceinline__ void hermitian_upper(float2 (&v)[RegFft<N>::V], int j
 #pragma unroll
     for (int r = 0; r < V; r++) {
       const int k = j + G * r;
-      if (k > N / 2) {
-        const float2 w = tile[RegFft<N>::slot(N - k)];
-        v[r] = make_float2(w.x, -w.y);
-      }
+      v[r] = extract_bin(v[r], tile[RegFft<N>::slot((N - k) & (N - 1))]);
     }
   }
   wave_sync();

@@ -390,14 +390,13 @@ __global__ void __launch_bounds__(kThreads, 3) k_stageA256(DevParams P, const vo
     const bool in_y = row >= P.ys && row < NY - P.ys;
     float2 v[16];
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
+    for (int r = 0; r < 16; r++) {              // every row carries the slave's template columns
       const int x = j + 16 * r;
-      const bool in = in_y && x >= P.xs && x < 256 - P.xs;
-      v[r] = make_float2(m1[x], in ? a2[x] : 0.0f);
+      v[r] = make_float2(m1[x], (x >= P.xs && x < 256 - P.xs) ? a2[x] : 0.0f);
     }
     wave_sync();                                   // read before the transform reuses the tile
     fft256_group<-1>(v, j, tile, t);
-    hermitian_upper256(v, j, tile, !in_y);        // unmasked rows: real input
+    extract_real256(v, j, tile, !in_y);           // unmasked rows: the master's spectrum
 #pragma unroll
     for (int r = 0; r < 16; r++) zb[(size_t)row * 256 + j + 16 * r] = v[r];
   }
@@ -526,14 +525,13 @@ __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __
     const bool in_y = row >= P.ys && row < N - P.ys;
     float2 v[V];
 #pragma unroll
-    for (int r = 0; r < V; r++) {
+    for (int r = 0; r < V; r++) {                  // every row carries the slave's template columns
       const int x = j + G * r;
-      const bool in = in_y && x >= P.xs && x < N - P.xs;
-      v[r] = make_float2(m1[x], in ? a2[x] : 0.0f);
+      v[r] = make_float2(m1[x], (x >= P.xs && x < N - P.xs) ? a2[x] : 0.0f);
     }
     wave_sync();
     F::template run<-1>(v, j, tile, t);
-    hermitian_upper<N>(v, j, tile, !in_y);         // unmasked rows: real input
+    extract_real<N>(v, j, tile, !in_y);            // unmasked rows: the master's spectrum
 #pragma unroll
     for (int r = 0; r < V; r++) zb[(size_t)row * N + j + G * r] = v[r];
   }
@@ -719,8 +717,8 @@ struct LineSum {
 
 // One block per kStripBlockLines strip lines; 16 groups of 16 lanes, one line
 // per group and iteration.  Out: amp1 (all columns), amp2 (template columns of
-// masked lines), lsum (per-line fixed-point sums), Zm / Zu (row spectra of
-// masked / unmasked lines, where needed).
+// masked lines), lsum (per-line fixed-point sums), Zm (the row spectrum of
+// z = amp1 + i amp2 on the template columns, where some patch uses the line).
 // Work item b < blk_n (blocks in xcd_item order) covers kStripBlockLines
 // lines from global line blk[2 (b + blk_off) + 1] of strip blk[2 (b + blk_off)].
 // EDGE: the strips' windows may cross the image sides.
@@ -733,7 +731,7 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
                                                         const unsigned char* __restrict__ needU,
                                                         float* __restrict__ amp1, float* __restrict__ amp2,
                                                         long long* __restrict__ lsum, float2* __restrict__ Zm,
-                                                        float2* __restrict__ Zu, const float2* __restrict__ tabs) {
+                                                        const float2* __restrict__ tabs) {
   // one shared array: 16 transpose tiles | 16 master lines (column pairs) |
   // pass-2 twiddles w256^(r j) [16][15] | phase factors phi[0..128]
   __shared__ float2 lds[16 * kT256Tile + 16 * 128 + 240 + 129];
@@ -804,20 +802,14 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
       const int x = j + 16 * r;
       v[r] = make_float2(m1[x], (r >= 4 && r < 12) ? a2[x] : 0.0f);
     }
-    wave_sync();                                // read before the transforms reuse the tile
+    wave_sync();                                // read before the transform reuses the tile
+    // one transform serves both uses of the line: masked rows take it as is,
+    // unmasked rows extract the master's spectrum from it (k_colpass256);
     // wm / wu are uniform over the group's 16 lanes (one line per group)
-    if (wm) {
+    if (wm || wu) {
       fft256_group_lds<-1>(v, j, tile, twl);
 #pragma unroll
       for (int r = 0; r < 16; r++) st_nt(Zm + l * 256 + j + 16 * r, v[r]);
-    }
-    if (wu) {
-#pragma unroll
-      for (int r = 0; r < 16; r++) v[r] = make_float2(m1[j + 16 * r], 0.0f);
-      fft256_group_lds<-1>(v, j, tile, twl);
-#pragma unroll
-      for (int r = 0; r <= 8; r++)              // bins 0..128; the rest are their conjugate mirrors
-        if (r < 8 || j == 0) st_nt(Zu + l * 256 + j + 16 * r, v[r]);
     }
     wave_sync();                                // the next line's loads into mr / tile follow
   }
@@ -884,8 +876,8 @@ __device__ __forceinline__ void interp_line(const unsigned (&u)[RegFft<N>::V], i
 // work item b < blk_n (xcd_item order) covers kStripBlockLines lines from
 // global line blk[2 (b + blk_off) + 1] of strip blk[2 (b + blk_off)], one line
 // per group of G lanes and iteration;
-// the groups never wait for each other.  Zu holds bins 0 .. N/2 of the
-// unmasked rows' spectra (the rest are their conjugate mirrors).
+// the groups never wait for each other.  Zm holds the row spectrum of
+// z = amp1 + i amp2 (template columns) of every line some patch uses.
 template <int N, int FMT, bool EDGE, bool ROWFFT = true>
 __global__ void __launch_bounds__(kThreads) k_stripA(DevParams P, const void* __restrict__ master,
                                                      const void* __restrict__ slave,
@@ -895,7 +887,7 @@ __global__ void __launch_bounds__(kThreads) k_stripA(DevParams P, const void* __
                                                      const unsigned char* __restrict__ needU,
                                                      float* __restrict__ amp1, float* __restrict__ amp2,
                                                      long long* __restrict__ lsum, float2* __restrict__ Zm,
-                                                     float2* __restrict__ Zu, const float2* __restrict__ tabs) {
+                                                     const float2* __restrict__ tabs) {
   using F = RegFft<N>;
   constexpr int G = F::G, V = F::V, GPB = kThreads / G;
   __shared__ float2 tiles[GPB * F::TILE];
@@ -960,7 +952,7 @@ __global__ void __launch_bounds__(kThreads) k_stripA(DevParams P, const void* __
     const float* m1 = (const float*)mr;
     const float* a2 = (const float*)tile;
     float2 v[V];
-    if (wm) {
+    if (wm || wu) {                                // one transform for both uses (k_colpass extracts)
 #pragma unroll
       for (int r = 0; r < V; r++) {
         const int x = j + G * r;
@@ -970,14 +962,6 @@ __global__ void __launch_bounds__(kThreads) k_stripA(DevParams P, const void* __
       F::template run<-1>(v, j, tile, t);
 #pragma unroll
       for (int r = 0; r < V; r++) Zm[l * N + j + G * r] = v[r];
-    }
-    if (wu) {
-#pragma unroll
-      for (int r = 0; r < V; r++) v[r] = make_float2(m1[j + G * r], 0.0f);
-      F::template run<-1>(v, j, tile, t);
-#pragma unroll
-      for (int r = 0; r < V; r++)
-        if (j + G * r <= N / 2) Zu[l * N + j + G * r] = v[r];
     }
     wave_sync();                                   // the next line rewrites mr and the tile
   }
@@ -1019,13 +1003,12 @@ __global__ void __launch_bounds__(kThreads) k_rowfft256(DevParams P, const float
     const bool in_y = row >= P.ys && row < P.NY - P.ys;
     float2 v[16];
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
+    for (int r = 0; r < 16; r++) {              // every row carries the slave's template columns
       const int x = j + 16 * r;
-      const bool in = in_y && x >= P.xs && x < 256 - P.xs;
-      v[r] = make_float2(a1[row * 256 + x], in ? a2[row * 256 + x] : 0.0f);
+      v[r] = make_float2(a1[row * 256 + x], (x >= P.xs && x < 256 - P.xs) ? a2[row * 256 + x] : 0.0f);
     }
     fft256_group<-1>(v, j, tile, t);
-    hermitian_upper256(v, j, tile, !in_y);        // unmasked rows: real input
+    extract_real256(v, j, tile, !in_y);           // unmasked rows: the master's spectrum
 #pragma unroll
     for (int r = 0; r < 16; r++) zb[(size_t)row * 256 + j + 16 * r] = v[r];
   }
@@ -1065,9 +1048,9 @@ __device__ __forceinline__ float2 demean_row_bin(float2 v, int kx, bool ym, floa
 // npx/2 in place of the duplicate column 0 for p = 0), so every column is
 // transformed once and C3 needs no second pass over Z.  Requires npx >= 32.
 // Row spectra come from Z[b] (rb null) or, for strips, from line rb[b] + y of
-// Zm (masked rows) or Zu.
+// Zm (the packed transform; unmasked rows extract the master's spectrum).
 __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const float2* __restrict__ Z,
-                                                         const float2* __restrict__ Zu, const int* __restrict__ rb,
+                                                         const int* __restrict__ rb,
                                                          const unsigned long long* __restrict__ sums,
                                                          float2* __restrict__ D, const float2* __restrict__ tabs,
                                                          int nwork) {
@@ -1086,9 +1069,7 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
   const int p = wi % pairs, b = wi / pairs;
   const size_t row0 = rb ? (size_t)rb[b] : (size_t)b * 256;
   // npy = 256 gives ys = 64: row y0 + 16 k (y0 < 16) is masked iff 4 <= k < 12
-  const float2* zm0 = Z + (row0 + (threadIdx.x >> 4)) * NX;
-  const float2* zu0 = (rb ? Zu : Z) + (row0 + (threadIdx.x >> 4)) * NX;
-  auto zrow = [&](int k) -> const float2* { return (k >= 4 && k < 12 ? zm0 : zu0) + (size_t)(16 * k) * NX; };
+  const float2* z0 = Z + (row0 + (threadIdx.x >> 4)) * NX;
   const int kxT = 16 * p + g;
   const int kxM = (p == 0 && g == 0) ? NX / 2 : NX - 16 * p - g;
   const bool self = (p == 0 && g == 0);                      // columns 0 and npx/2 mirror onto themselves
@@ -1106,17 +1087,27 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
     float2 rT[16], rM[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-      const float2* zr = zrow(k);
+      const float2* zr = z0 + (size_t)(16 * k) * NX;
       rT[k] = zr[kxt];
-      // rows y0 + 16 k outside [64, 192) are unmasked: their spectrum is
-      // Hermitian, so column npx - kx is conj(column kx) (npx/2 is stored)
-      if (k >= 4 && k < 12) {
+      // rows y0 + 16 k outside [64, 192) are unmasked.  Per patch their
+      // spectrum is Hermitian (the producer extracted it), so column npx - kx
+      // is conj(column kx) (npx/2 is stored); strip lines hold the packed
+      // transform, from which the master's spectrum is extracted here
+      if ((k >= 4 && k < 12) || rb) {
         rM[k] = zr[kxm];
       } else {
         const float2 w = zr[self ? kxm : kxt];   // branch-free; a cache hit for kxt
         rM[k] = self ? w : make_float2(w.x, -w.y);
       }
     }
+    if (rb)
+#pragma unroll
+      for (int k = 0; k < 16; k++)
+        if (!(k >= 4 && k < 12)) {
+          const float2 a = rT[k], q = rM[k];      // Z[kx], Z[npx - kx] (self: Z[0], Z[npx/2])
+          rT[k] = extract_bin(a, self ? a : q);
+          rM[k] = extract_bin(q, self ? q : a);
+        }
 #pragma unroll
     for (int k = 0; k < 16; k++) {
       const int y = y0 + 16 * k;
@@ -1265,9 +1256,9 @@ struct ColCfg {
   static constexpr int PITCH = TC + 1;
 };
 
-// Row spectra come from Z[b] (rb null) or, for strips, from line rb[b] + y of
-// Zm (masked rows) or Zu (bins 0 .. N/2 of unmasked rows; the others are the
-// conjugates of their mirrors, as the Hermitian completion makes them).
+// Row spectra come from Z[b] (rb null; unmasked rows already hold the master's
+// spectrum) or, for strips, from line rb[b] + y of Zm, the packed transform,
+// from which unmasked rows extract the master's spectrum.
 // Work items w = (patch w / tiles, column tile w % tiles), w < nwork.
 // Workgroups go to the 8 XCDs round-robin in blockIdx order; block i takes
 // item (i mod 8) (grid / 8) + i / 8 (the grid is a multiple of 8), so each XCD
@@ -1276,7 +1267,7 @@ struct ColCfg {
 // which share lines, meet in one L2.
 template <int N>
 __global__ void __launch_bounds__(kThreads) k_colpass(DevParams P, const float2* __restrict__ Z,
-                                                      const float2* __restrict__ Zu, const int* __restrict__ rb,
+                                                      const int* __restrict__ rb,
                                                       const unsigned long long* __restrict__ sums,
                                                       float2* __restrict__ D, const float2* __restrict__ tabs,
                                                       int nwork) {
@@ -1303,27 +1294,25 @@ __global__ void __launch_bounds__(kThreads) k_colpass(DevParams P, const float2*
   auto kx_of = [&](int i, bool mirror) { return !mirror ? TC * p + i : ((p == 0 && i == 0) ? N / 2 : N - TC * p - i); };
   float2 vT[CPG][V], vM[CPG][V];
   // Thread (y0, i) loads column i of the tile at rows y0 + LS k: all LK loads
-  // are in flight together, then the demeaned values go through LDS.  Rows of
-  // strips come from Zm (masked) or Zu (bins <= N/2, the others conjugated).
+  // are in flight together, then the demeaned values go through LDS.
   static_assert(kThreads % TC == 0 && (N * TC) % kThreads == 0, "column tile");
   constexpr int LS = kThreads / TC, LK = N * TC / kThreads;
   const int i0 = threadIdx.x % TC, y0 = threadIdx.x / TC;
   // square windows: ys = N / 4, so row y0 + LS k (y0 < LS) is masked iff
   // LK / 4 <= k < 3 LK / 4 (known at compile time)
   static_assert(LS * LK == N, "column tile rows");
-  const float2* zm0 = Z + (row0 + y0) * N;
-  const float2* zu0 = (rb ? Zu : Z) + (row0 + y0) * N;
+  const float2* z0 = Z + (row0 + y0) * N;
   for (int half = 0; half < 2; half++) {
     const int kx = kx_of(i0, half == 1);
-    const int ku = rb && kx > N / 2 ? N - kx : kx;      // unmasked strip rows: bins <= N/2 only
-    const bool cj = rb && kx > N / 2;
+    const int kq = (N - kx) & (N - 1);                  // the mirror bin
     const float2 mk = tabs[P.mk_x + kx];
     float2 zr[LK];
 #pragma unroll
     for (int k = 0; k < LK; k++) {
       const bool ym = k >= LK / 4 && k < 3 * LK / 4;
-      zr[k] = ym ? zm0[(size_t)(LS * k) * N + kx] : zu0[(size_t)(LS * k) * N + ku];
-      if (!ym && cj) zr[k].y = -zr[k].y;
+      zr[k] = z0[(size_t)(LS * k) * N + kx];
+      // unmasked strip rows hold the packed transform: extract the master's spectrum
+      if (!ym && rb) zr[k] = extract_bin(zr[k], z0[(size_t)(LS * k) * N + kq]);
     }
     __syncthreads();
 #pragma unroll
@@ -1476,11 +1465,9 @@ __global__ void __launch_bounds__(kMaxThreads) k_rowfft(DevParams P, const float
   float2* A = smem;
   float2* Bf = A + nel;
   // raw amplitudes; the means come off in the row spectrum (demean_row_bin)
-  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
-    const int row = e / NX, x = e - row * NX;
-    const int y = g * G + row;
-    const bool in = y >= P.ys && y < NY - P.ys && x >= P.xs && x < NX - P.xs;
-    A[e] = make_float2(a1[e], in ? a2[e] : 0.0f);
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {    // every row carries the slave's template columns
+    const int x = e % NX;
+    A[e] = make_float2(a1[e], (x >= P.xs && x < NX - P.xs) ? a2[e] : 0.0f);
   }
   __syncthreads();
   float2* R = lds_fft_rows<-1>(A, Bf, NX, G, tabs + P.tw_nx, P.plx);
@@ -1489,10 +1476,8 @@ __global__ void __launch_bounds__(kMaxThreads) k_rowfft(DevParams P, const float
     const int row = e / NX, x = e - row * NX;
     const int y = g * G + row;
     float2 v = R[e];
-    if (!(y >= P.ys && y < NY - P.ys) && x > NX / 2) {   // unmasked rows: Hermitian completion
-      const float2 w = R[row * NX + NX - x];
-      v = make_float2(w.x, -w.y);
-    }
+    if (!(y >= P.ys && y < NY - P.ys))            // unmasked rows: the master's spectrum
+      v = extract_bin(v, R[row * NX + ((NX - x) & (NX - 1))]);
     out[e] = v;
   }
 }
@@ -2561,7 +2546,6 @@ struct xcorr_hip_ctx {
   float* amp1s = nullptr;
   float* amp2s = nullptr;
   long long* lsum = nullptr;
-  float2* Zu = nullptr;                   // unmasked row spectra (masked ones in Z)
   size_t z_elems = 0;                     // capacity of Z (float2)
   unsigned char* need = nullptr;          // needM [max_lines] then needU [max_lines]
   int* d_plan = nullptr;                  // rb | perm | blk_strip | strips, per batch
@@ -2619,7 +2603,6 @@ struct PatchBufs {
   float* amp;                   // [B][2][N] amplitudes
   unsigned long long* sums;     // [B][2] exact patch sums
   float2* Z;                    // [B][N] row / 2-D spectra
-  float2* Zu;                   // strips only: unmasked row spectra
   float2* D;                    // [B][nyc][NX] lag rows
   float* surf;                  // [B][nyc][nxc] correlation surface (frequency)
   double* surfd;                // [B][nyc][nxc] correlation surface (time modes)
@@ -2631,8 +2614,7 @@ struct PatchBufs {
 };
 
 static PatchBufs ctx_bufs(const xcorr_hip_ctx* c) {
-  return PatchBufs{c->amp, c->sums, c->Z, c->Zu, c->D, c->surf, c->surfd, c->peak, c->partial, c->tpeak, c->tcave,
-                   c->tcaa};
+  return PatchBufs{c->amp, c->sums, c->Z, c->D, c->surf, c->surfd, c->peak, c->partial, c->tpeak, c->tcave, c->tcaa};
 }
 
 #define HIPCK(ctx, call)                                                                  \
@@ -2692,7 +2674,7 @@ static int validate(const xcorr_hip_params* p, std::string* why) {
 
 static void free_scratch(xcorr_hip_ctx* c) {
   void* ptrs[] = {c->amp, c->sums, c->Z, c->D, c->surf, c->surfd, c->peak, c->partial, c->tpeak,
-                  c->tcave, c->d_win, c->d_out, c->tabs, c->amp1s, c->amp2s, c->lsum, c->Zu, c->need, c->d_plan,
+                  c->tcave, c->d_win, c->d_out, c->tabs, c->amp1s, c->amp2s, c->lsum, c->need, c->d_plan,
                   c->tcaa};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -2705,7 +2687,7 @@ static void free_scratch(xcorr_hip_ctx* c) {
   c->amp = nullptr; c->sums = nullptr; c->Z = nullptr; c->D = nullptr; c->surf = nullptr;
   c->surfd = nullptr; c->peak = nullptr; c->partial = nullptr; c->tpeak = nullptr; c->tcave = nullptr;
   c->d_win = nullptr; c->d_out = nullptr; c->tabs = nullptr; c->amp1s = nullptr; c->amp2s = nullptr;
-  c->lsum = nullptr; c->Zu = nullptr; c->need = nullptr; c->d_plan = nullptr; c->tcaa = nullptr;
+  c->lsum = nullptr; c->need = nullptr; c->d_plan = nullptr; c->tcaa = nullptr;
   c->z_elems = 0;
 }
 
@@ -2873,7 +2855,6 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
     alloc((void**)&c->amp1s, L * d.NX * sizeof(float));
     alloc((void**)&c->amp2s, L * d.NX * sizeof(float));
     alloc((void**)&c->lsum, L * 2 * sizeof(long long));
-    if (p->corr_flag == XCORR_HIP_FREQ) alloc((void**)&c->Zu, L * d.NX * sizeof(float2));
     alloc((void**)&c->need, L * 2);
     c->plan_bytes = plan_ints(c) * sizeof(int);
     alloc((void**)&c->d_plan, c->plan_bytes);
@@ -3204,7 +3185,7 @@ static void launch_colpass(xcorr_hip_ctx* c, const PatchBufs& b, const DevParams
                            int nb) {
   const int nwork = nb * (N / (2 * ColCfg<N>::TC));
   hipLaunchKernelGGL(k_colpass<N>, dim3((nwork + 7) & ~7), dim3(kThreads), 0, st, d, (const float2*)b.Z,
-                     (const float2*)b.Zu, rb, (const unsigned long long*)b.sums, b.D, (const float2*)c->tabs, nwork);
+                     rb, (const unsigned long long*)b.sums, b.D, (const float2*)c->tabs, nwork);
 }
 
 // The -time search of 128 x 128 windows (xs = ys = 32) on the int8 matrix
@@ -3241,7 +3222,7 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   const bool freq = d.corr_flag == XCORR_HIP_FREQ;    // else -time on 128 x 128 windows
   rc = launch_timed(c, timing, d.NX == 256 ? "k_stripA256" : "k_stripA", [&] {
     using KA = void (*)(DevParams, const void*, const void*, const StripDesc*, const int*, int, int,
-                        const unsigned char*, const unsigned char*, float*, float*, long long*, float2*, float2*,
+                        const unsigned char*, const unsigned char*, float*, float*, long long*, float2*,
                         const float2*);
     KA kin, kedge;
     if (d.NX == 256) {
@@ -3260,11 +3241,11 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
     if (nblk_in > 0)
       hipLaunchKernelGGL(kin, dim3((nblk_in + 7) & ~7), dim3(kThreads), 0, st, d, master, slave, d_st, d_bs, 0,
                          nblk_in, (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s, c->amp2s,
-                         c->lsum, c->Z, c->Zu, (const float2*)c->tabs);
+                         c->lsum, c->Z, (const float2*)c->tabs);
     if (nblk > nblk_in)
       hipLaunchKernelGGL(kedge, dim3((nblk - nblk_in + 7) & ~7), dim3(kThreads), 0, st, d, master, slave, d_st, d_bs,
                          nblk_in, nblk - nblk_in, (const unsigned char*)needM, (const unsigned char*)needU, c->amp1s,
-                         c->amp2s, c->lsum, c->Z, c->Zu, (const float2*)c->tabs);
+                         c->amp2s, c->lsum, c->Z, (const float2*)c->tabs);
   });
   if (rc) return rc;
   rc = launch_timed(c, timing, "k_patch_sums", [&] {
@@ -3279,9 +3260,8 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   } else {
   rc = launch_timed(c, timing, d.NX == 256 ? "k_colpass256" : "k_colpass", [&] {
     if (d.NX == 256)
-      hipLaunchKernelGGL(k_colpass256, dim3((nb * 8 + 7) & ~7), dim3(kThreads), 0, st, d, (const float2*)c->Z,
-                         (const float2*)c->Zu, d_rb, (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs,
-                         nb * 8);
+      hipLaunchKernelGGL(k_colpass256, dim3((nb * 8 + 7) & ~7), dim3(kThreads), 0, st, d, (const float2*)c->Z, d_rb,
+                         (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs, nb * 8);
     else if (d.NX == 128)
       launch_colpass<128>(c, ctx_bufs(c), d, st, d_rb, nb);
     else
@@ -3505,7 +3485,7 @@ static int run_batch(xcorr_hip_ctx* c, const PatchBufs& b, const xcorr_hip_windo
     } else if (col256 && d.NX >= 32) {
       rc = launch_timed(c, timing, "k_colpass256", [&] {
         hipLaunchKernelGGL(k_colpass256, dim3((nb * (d.NX / 32) + 7) & ~7), dim3(kThreads), 0, c->stream, d,
-                           (const float2*)b.Z, (const float2*)nullptr, (const int*)nullptr,
+                           (const float2*)b.Z, (const int*)nullptr,
                            (const unsigned long long*)b.sums, b.D, (const float2*)c->tabs, nb * (d.NX / 32));
       });
     } else {

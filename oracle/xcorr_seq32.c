@@ -424,9 +424,11 @@ int xo32_patch(const xo_params *p, const c32 *mwin, const c32 *swin, xo_result *
       i1[o] = (int32_t)c1;
       i2[o] = (int32_t)c2;
       /* the transform takes the raw amplitudes; the means come off in the
-       * spectrum (mask_spectrum / spectral split below) */
+       * spectrum (mask_spectrum / spectral split below).  Every row carries
+       * the slave on the template columns: rows outside the mask take the
+       * master's spectrum out of the packed transform (below) */
       z[o].re = amp1[o];
-      z[o].im = inmask ? amp2[o] : 0.0f;
+      z[o].im = (x >= p->xsearch && x < NX - p->xsearch) ? amp2[o] : 0.0f;
     }
   }
 
@@ -438,21 +440,25 @@ int xo32_patch(const xo_params *p, const c32 *mwin, const c32 *swin, xo_result *
   if (p->corr_flag == 2) {
     /* (b) frequency correlation: do_freq_xcorr.cu:90-200 with c1 + i*c2 packed */
     c32 *mx = (c32 *)malloc(sizeof(c32) * NX);
-    if (!mx) { rc = -2; goto done; }
+    c32 *zr = (c32 *)malloc(sizeof(c32) * NX);
+    if (!mx || !zr) { free(mx); free(zr); rc = -2; goto done; }
     xo_mask_spectrum(NX, p->xsearch, mx);
     for (int y = 0; y < NY; y++) {
       c32 *row = z + (size_t)y * NX;
       xo32_fft(row, NX, -1);
-      /* rows outside the mask carry a real input (amp1 + 0i): their spectrum is
-       * taken as bins 0..NX/2 with the upper half completed by conjugate
-       * symmetry, X[k] = conj(X[NX - k]) (the transform alone is Hermitian only
-       * to rounding) */
-      if (!((y >= p->ysearch) && (y < NY - p->ysearch)))
-        for (int kx = NX / 2 + 1; kx < NX; kx++) {
-          row[kx].re = row[NX - kx].re;
-          row[kx].im = -row[NX - kx].im;
+      /* rows outside the mask: the spectrum of the real part (the master) of
+       * the packed row, X[k] = ((Z[k].re + Z[-k].re) / 2, (Z[k].im - Z[-k].im) / 2)
+       * (exactly Hermitian) */
+      if (!((y >= p->ysearch) && (y < NY - p->ysearch))) {
+        memcpy(zr, row, sizeof(c32) * NX);
+        for (int kx = 0; kx < NX; kx++) {
+          const c32 a = zr[kx], q = zr[(NX - kx) % NX];
+          row[kx].re = (a.re + q.re) * 0.5f;
+          row[kx].im = (a.im - q.im) * 0.5f;
         }
+      }
     }
+    free(zr);
     /* demean in the row spectrum: z - m1 - i m2 mask has row transform
      * Zr - m1 NX delta(kx) - i m2 [y in mask] Mx[kx] */
     {
