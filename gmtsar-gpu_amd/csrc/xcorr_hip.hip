@@ -1181,7 +1181,10 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
 }
 
 // Stage (b) row IFFT + surface + argmax for npx = 256 (nx_corr/2 = 64, nxc = 128):
-// one block per patch, so the peak key and the sum are written directly.
+// one block per patch, so the peak key and the sum are written directly.  The
+// kept rows are Hermitian (their transforms are real), so rows r and
+// r + nyc/2 go through one inverse transform as ra + i rb; |c3| is |Re| of
+// the result for row r and |Im| for row r + nyc/2.
 __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const float2* __restrict__ D,
                                                               float* __restrict__ surf,
                                                               unsigned long long* __restrict__ peak,
@@ -1199,39 +1202,48 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
   for (int k = threadIdx.x; k < 240; k += kThreads) twl[k] = tabs[P.tw_nx + (k % 15 + 1) * (k / 15)];
   __syncthreads();
   const float sc = 1.0f / (float)P.N;
+  const int hp = P.nyc >> 1;
   unsigned long long best = 0ull;
   double sum = 0.0;
-  const int iters = (P.nyc + 15) >> 4;
-  // one row of D per group and iteration (mirrored bins from 0 .. 128); a
-  // prefetch of the next row costs the fourth wave per SIMD and was slower
-  auto load_row = [&](int row, float2 (&u)[16]) {
+  const int iters = (hp + 15) >> 4;
+  for (int it = 0; it < iters; it++) {
+    const int row = it * 16 + g;                 // rows row and row + hp (mirrored bins from 0 .. 128)
+    const bool active = row < hp;
+    // branch-free loads (an inactive group reads a valid row and discards
+    // it), so all 32 are in flight together
+    const float2* ra = db + (size_t)min(row, hp - 1) * 256;
+    const float2* rc = ra + (size_t)hp * 256;
+    float2 a[16], c[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
       const int x = j + 16 * r;
-      u[r] = make_float2(0.0f, 0.0f);
-      if (row < P.nyc) u[r] = db[(size_t)row * 256 + (x <= 128 ? x : 256 - x)];
+      const int xm = x <= 128 ? x : 256 - x;
+      a[r] = ra[xm];
+      c[r] = rc[xm];
     }
-  };
-  for (int it = 0; it < iters; it++) {
-    const int row = it * 16 + g;
-    const bool active = row < P.nyc;
     float2 v[16];
-    load_row(row, v);
 #pragma unroll
-    for (int r = 0; r < 16; r++)
-      if (j + 16 * r > 128) v[r].y = -v[r].y;      // upper half: conjugate mirror
+    for (int r = 0; r < 16; r++) {
+      float2 ar = a[r], cr = c[r];
+      if (j + 16 * r > 128) {                    // upper half: conjugate mirrors
+        ar.y = -ar.y;
+        cr.y = -cr.y;
+      }
+      v[r] = active ? make_float2(ar.x - cr.y, ar.y + cr.x) : make_float2(0.0f, 0.0f);
+    }
     fft256_group_lds<+1>(v, j, tile, twl);
     if (!active) continue;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      const float re = v[q + 4].x * sc, im = v[q + 4].y * sc;
-      const double dr = re, di = im;
-      const float cv = (float)sqrt(dr * dr + di * di);
-      const unsigned idx = (unsigned)(row * 128 + j + 16 * q);
-      sb[idx] = cv;
-      const unsigned long long k = peak_key(cv, idx);
-      best = k > best ? k : best;
-      sum += (double)cv;
+      const float ca = fabsf(v[q + 4].x * sc), cb = fabsf(v[q + 4].y * sc);
+      const unsigned ia = (unsigned)(row * 128 + j + 16 * q), ib = ia + (unsigned)hp * 128;
+      sb[ia] = ca;
+      sb[ib] = cb;
+      const unsigned long long ka = peak_key(ca, ia), kb = peak_key(cb, ib);
+      best = ka > best ? ka : best;
+      best = kb > best ? kb : best;
+      sum += (double)ca;
+      sum += (double)cb;
     }
   }
   best = block_max_u64(best, redk);
@@ -1396,8 +1408,8 @@ __global__ void __launch_bounds__(kThreads) k_colpass(DevParams P, const float2*
 }
 
 // Stage (b) row IFFT + surface + argmax for npx = npy = N in {128, 512}: one
-// block per patch (the arithmetic of k_rowifft_surf; the surface sum is one
-// partial per patch).
+// block per patch (the arithmetic of k_rowifft_surf, rows paired as in
+// k_rowifft_surf256; the surface sum is one partial per patch).
 template <int N>
 __global__ void __launch_bounds__(kThreads) k_rowifft_surf_reg(DevParams P, const float2* __restrict__ D,
                                                                float* __restrict__ surf,
@@ -1419,26 +1431,33 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf_reg(DevParams P, cons
   const float sc = 1.0f / (float)P.N;
   unsigned long long best = 0ull;
   double sum = 0.0;
-  for (int row0 = 0; row0 < NYC; row0 += GPB) {   // uniform trip count (NYC is a multiple of GPB)
+  constexpr int HP = NYC / 2;                      // rows row and row + HP share one transform
+  for (int row0 = 0; row0 < HP; row0 += GPB) {    // uniform trip count (HP is a multiple of GPB)
     const int row = row0 + g;
     float2 v[V];
 #pragma unroll
     for (int r = 0; r < V; r++) {
       const int x = j + G * r;
-      v[r] = db[(size_t)row * N + (x <= N / 2 ? x : N - x)];
-      if (x > N / 2) v[r].y = -v[r].y;              // upper half: conjugate mirror
+      const int xm = x <= N / 2 ? x : N - x;
+      float2 a = db[(size_t)row * N + xm], c = db[(size_t)(row + HP) * N + xm];
+      if (x > N / 2) {                              // upper half: conjugate mirrors
+        a.y = -a.y;
+        c.y = -c.y;
+      }
+      v[r] = make_float2(a.x - c.y, a.y + c.x);
     }
     F::template run<+1>(v, j, tile, t);
 #pragma unroll
     for (int q = 0; q < V / 2; q++) {
-      const float re = v[q + V / 4].x * sc, im = v[q + V / 4].y * sc;
-      const double dr = re, di = im;
-      const float cv = (float)sqrt(dr * dr + di * di);
-      const unsigned idx = (unsigned)(row * NYC + j + G * q);
-      sb[idx] = cv;
-      const unsigned long long k = peak_key(cv, idx);
-      best = k > best ? k : best;
-      sum += (double)cv;
+      const float ca = fabsf(v[q + V / 4].x * sc), cb = fabsf(v[q + V / 4].y * sc);
+      const unsigned ia = (unsigned)(row * NYC + j + G * q), ib = ia + (unsigned)(HP * NYC);
+      sb[ia] = ca;
+      sb[ib] = cb;
+      const unsigned long long ka = peak_key(ca, ia), kb = peak_key(cb, ib);
+      best = ka > best ? ka : best;
+      best = kb > best ? kb : best;
+      sum += (double)ca;
+      sum += (double)cb;
     }
   }
   best = block_max_u64(best, redk);
@@ -1556,42 +1575,45 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf(DevParams P, const fl
   extern __shared__ float2 smem[];
   __shared__ unsigned long long redk[kThreads / 64];
   __shared__ double redd[kThreads / 64];
-  const int NX = P.NX, G = P.G4, nel = G * NX, nxc = P.nxc;
+  const int NX = P.NX, G = P.G4, nxc = P.nxc;
   const int groups = P.groups4;
   const int g = blockIdx.x % groups, b = blockIdx.x / groups;
-  const float2* db = D + ((size_t)b * P.nyc + (size_t)g * G) * NX;
+  // G surface rows per block as G / 2 pairs (r, r + nyc/2), r from a0: the
+  // kept rows are Hermitian, so a pair goes through one transform as ra + i rb
+  const int hp = P.nyc >> 1, G2 = G >> 1, nel2 = G2 * NX, a0 = g * G2;
+  const float2* db = D + (size_t)b * P.nyc * NX;
   float2* A = smem;
-  float2* Bf = A + nel;
+  float2* Bf = A + nel2;
   // columns above npx/2 are the conjugate mirrors of the lower half (C3 is Hermitian)
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
+  for (int e = threadIdx.x; e < nel2; e += kThreads) {
     const int row = e / NX, x = e - row * NX;
-    float2 v;
-    if (x <= NX / 2) {
-      v = db[e];
-    } else {
-      v = db[row * NX + NX - x];
-      v.y = -v.y;
+    const int xm = x <= NX / 2 ? x : NX - x;
+    float2 a = db[(size_t)(a0 + row) * NX + xm], c = db[(size_t)(a0 + row + hp) * NX + xm];
+    if (x > NX / 2) {
+      a.y = -a.y;
+      c.y = -c.y;
     }
-    A[e] = v;
+    A[e] = make_float2(a.x - c.y, a.y + c.x);
   }
   __syncthreads();
-  float2* R = lds_fft_rows<+1>(A, Bf, NX, G, tabs + P.tw_nx, P.plx);
+  float2* R = lds_fft_rows<+1>(A, Bf, NX, G2, tabs + P.tw_nx, P.plx);
   const int q0 = P.nx_corr / 2;
   const float sc = 1.0f / (float)P.N;
   float* sb = surf + (size_t)b * P.nyc * nxc;
   unsigned long long best = 0ull;
   double sum = 0.0;
-  for (int e = threadIdx.x; e < G * nxc; e += kThreads) {
+  for (int e = threadIdx.x; e < G2 * nxc; e += kThreads) {
     const int row = e / nxc, ix = e - row * nxc;
     const float2 v = R[row * NX + q0 + ix];
-    const float re = v.x * sc, im = v.y * sc;
-    const double dr = re, di = im;
-    const float cv = (float)sqrt(dr * dr + di * di);
-    const unsigned idx = (unsigned)((g * G + row) * nxc + ix);
-    sb[idx] = cv;
-    const unsigned long long k = peak_key(cv, idx);
-    best = k > best ? k : best;
-    sum += (double)cv;
+    const float ca = fabsf(v.x * sc), cb = fabsf(v.y * sc);
+    const unsigned ia = (unsigned)((a0 + row) * nxc + ix), ib = ia + (unsigned)(hp * nxc);
+    sb[ia] = ca;
+    sb[ib] = cb;
+    const unsigned long long ka = peak_key(ca, ia), kb = peak_key(cb, ib);
+    best = ka > best ? ka : best;
+    best = kb > best ? kb : best;
+    sum += (double)ca;
+    sum += (double)cb;
   }
   best = block_max_u64(best, redk);
   sum = block_sum_f64(sum, redd);
@@ -2769,7 +2791,7 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   d.plx = plan_for(d.NX);
   d.ply = plan_for(d.NY);
   d.G1 = std::min(d.NY, std::max(1, kTile / d.NX));
-  d.G4 = std::min(d.nyc, std::max(1, kTile / d.NX));
+  d.G4 = std::min(d.nyc, std::max(2, kTile / d.NX));   // k_rowifft_surf pairs its rows
   d.groups4 = groups4_for(c);                      // needs the plans and G4
   d.C3 = std::min(d.NX, std::max(1, kTile / d.NY));
   // time search: LR surface rows x nxc lags per block (<= 2 lags per thread),

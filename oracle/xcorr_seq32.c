@@ -512,17 +512,28 @@ int xo32_patch(const xo_params *p, const c32 *mwin, const c32 *swin, xo_result *
         v.im = -v.im;
         c3[(size_t)y * NX + x] = v;
       }
+    /* the kept rows are Hermitian, so their inverse transforms are real: rows
+     * iy and iy + nyc/2 go through one transform as ra + i rb, and
+     * |c3| is |Re| of the result for row iy and |Im| for row iy + nyc/2
+     * (d_fillcorrmat's hypot of a real value, do_freq_xcorr.cu:26-29, 67-84) */
     const float sc = 1.0f / (float)N;
-    for (int iy = 0; iy < nyc; iy++) {
-      c32 *row = c3 + (size_t)(iy + r0) * NX;
-      xo32_fft(row, NX, +1);
+    const int hp = nyc / 2;
+    c32 *u = (c32 *)malloc(sizeof(c32) * NX);
+    if (!u) { rc = -2; goto done; }
+    for (int iy = 0; iy < hp; iy++) {
+      const c32 *ra = c3 + (size_t)(iy + r0) * NX, *rb = c3 + (size_t)(iy + hp + r0) * NX;
+      for (int x = 0; x < NX; x++) {
+        u[x].re = ra[x].re - rb[x].im;
+        u[x].im = ra[x].im + rb[x].re;
+      }
+      xo32_fft(u, NX, +1);
       for (int ix = 0; ix < nxc; ix++) {
-        c32 v = row[ix + q0];
-        float re = v.re * sc, im = v.im * sc;
-        double dr = re, di = im;
-        surf[(size_t)iy * nxc + ix] = (float)sqrt(dr * dr + di * di);
+        c32 v = u[ix + q0];
+        surf[(size_t)iy * nxc + ix] = fabsf(v.re * sc);
+        surf[(size_t)(iy + hp) * nxc + ix] = fabsf(v.im * sc);
       }
     }
+    free(u);
     /* argmax: strict '>' from 0 in row-major order (conv_ampl_demean.cu:434-554, 642-675) */
     int idx = 0;
     float best = 0.0f;
