@@ -22,7 +22,7 @@ HOST_LIB_PATH = os.path.join(HERE, "libxcorr_host.so")
 
 TIME, TIME4, FREQ = 0, 1, 2
 CINT16, FLOAT32 = 0, 1
-DEVICE_PTRS, NO_SYNC, TIMING, HOST_WINDOWS, SERIAL = 1, 2, 4, 8, 64
+DEVICE_PTRS, NO_SYNC, TIMING, HOST_WINDOWS, SERIAL, KEEP_SURFACE = 1, 2, 4, 8, 64, 128
 TUNE, TUNE_RADIX = 16, 32
 
 EXPORTED_SYMBOLS = (

@@ -52,6 +52,9 @@ struct DevParams {
   // lines held in device memory (all of each image, or a streamed band):
   // addresses are only ever formed for lines in [lo, hi)
   int m_lo, m_hi, s_lo, s_hi;
+  // 1: the correlation surface is written whole (stage dumps); 0: the row
+  // kernels that find the peak in one block write only its neighbourhood
+  int keep_surf;
 };
 
 // --------------------------------------------------------------------------
@@ -1180,16 +1183,52 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
     for (int e = threadIdx.x; e < 128; e += kThreads) db[(size_t)e * NX + NX / 2] = sx[e];
 }
 
+// The 8 x 8 neighbourhood of the surface peak that k_finish reads (the flat
+// index k = (ip - 4 + iy) nxc + jp - 4 + ix of d_cppow_strip,
+// conv_ampl_demean.cu:75-95; 0 outside the surface), stored as 64 values at
+// the start of the patch's surface slot when the surface is not kept whole:
+// value (iy, ix) at iy * 8 + ix.  Surface row y holds the entries of rows
+// y_lo <= y <= y_hi.
+struct NbrWindow {
+  int k0, ktot, lg, y_lo, y_hi;
+  __device__ void zero_outside(float* sb) const {   // thread e < 64: entry e when it lies outside the surface
+    const int e = threadIdx.x;
+    const int k = k0 + ((e >> 3) << lg) + (e & 7);
+    if (k < 0 || k >= ktot) sb[e] = 0.0f;
+  }
+  __device__ void put(float* sb, int y, int col, float v) const {   // surface value (y, col)
+    const int d = (y << lg) + col - k0;
+    const int iy = d >> lg, ix = d & ((1 << lg) - 1);
+    if (d >= 0 && iy < 8 && ix < 8) sb[iy * 8 + ix] = v;
+  }
+};
+__device__ inline NbrWindow nbr_window(unsigned long long best, int nxc, int nyc) {
+  float vb;
+  unsigned ib;
+  key_decode(best, &vb, &ib);
+  NbrWindow w;
+  w.lg = __builtin_ctz(nxc);                   // nxc is a power of two >= 8
+  const int ip = (int)(ib >> w.lg), jp = (int)(ib & (nxc - 1));
+  w.k0 = ((ip - 4) << w.lg) + jp - 4;
+  w.ktot = nyc << w.lg;
+  // floor divisions of k0 >= -4 nxc - 4 by nxc, kept non-negative
+  w.y_lo = max(0, ((w.k0 + (8 << w.lg)) >> w.lg) - 8);
+  w.y_hi = min(nyc - 1, ((w.k0 + (7 << w.lg) + 7 + (8 << w.lg)) >> w.lg) - 8);
+  return w;
+}
+
 // Stage (b) row IFFT + surface + argmax for npx = 256 (nx_corr/2 = 64, nxc = 128):
 // one block per patch, so the peak key and the sum are written directly.  The
 // kept rows are Hermitian (their transforms are real), so rows r and
 // r + nyc/2 go through one inverse transform as ra + i rb; |c3| is |Re| of
-// the result for row r and |Im| for row r + nyc/2.
+// the result for row r and |Im| for row r + nyc/2.  With keep the surface is
+// written whole; otherwise only the 8 x 8 neighbourhood of the peak that
+// k_finish reads goes out (surface_nbr).
 __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const float2* __restrict__ D,
                                                               float* __restrict__ surf,
                                                               unsigned long long* __restrict__ peak,
                                                               double* __restrict__ partial,
-                                                              const float2* __restrict__ tabs) {
+                                                              const float2* __restrict__ tabs, int keep) {
   __shared__ float2 tiles[16 * kT256Tile];
   __shared__ float2 twl[240];                   // pass-2 twiddles w256^(r j), twl[15 j + r - 1]
   __shared__ unsigned long long redk[kThreads / 64];
@@ -1237,8 +1276,10 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
     for (int q = 0; q < 8; q++) {
       const float ca = fabsf(v[q + 4].x * sc), cb = fabsf(v[q + 4].y * sc);
       const unsigned ia = (unsigned)(row * 128 + j + 16 * q), ib = ia + (unsigned)hp * 128;
-      sb[ia] = ca;
-      sb[ib] = cb;
+      if (keep) {
+        sb[ia] = ca;
+        sb[ib] = cb;
+      }
       const unsigned long long ka = peak_key(ca, ia), kb = peak_key(cb, ib);
       best = ka > best ? ka : best;
       best = kb > best ? kb : best;
@@ -1252,6 +1293,32 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
     peak[b] = best;
     partial[b] = sum;
   }
+  if (keep) return;
+  // The neighbourhood: surface rows y_lo .. y_hi (at most 10), one per group,
+  // transformed again with the same arithmetic; a wave with no row left stops.
+  const NbrWindow nw = nbr_window(best, 128, P.nyc);
+  if (threadIdx.x < 64) nw.zero_outside(sb);
+  if (nw.y_lo + (g & ~3) > nw.y_hi) return;
+  const int y = min(nw.y_lo + g, nw.y_hi);       // spare groups of a wave redo the last row
+  const int pr = y < hp ? y : y - hp;
+  const float2* ra = db + (size_t)pr * 256;
+  const float2* rc = ra + (size_t)hp * 256;
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int x = j + 16 * r;
+    const int xm = x <= 128 ? x : 256 - x;
+    float2 ar = ra[xm], cr = rc[xm];
+    if (x > 128) {
+      ar.y = -ar.y;
+      cr.y = -cr.y;
+    }
+    v[r] = make_float2(ar.x - cr.y, ar.y + cr.x);
+  }
+  fft256_group_lds<+1>(v, j, tile, twl);
+  if (nw.y_lo + g <= nw.y_hi)
+#pragma unroll
+    for (int q = 0; q < 8; q++) nw.put(sb, y, j + 16 * q, fabsf((y < hp ? v[q + 4].x : v[q + 4].y) * sc));
 }
 
 // Stage (b) columns for npx = npy = N in {128, 512}: FFT along y, C3 = C1
@@ -1415,7 +1482,7 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf_reg(DevParams P, cons
                                                                float* __restrict__ surf,
                                                                unsigned long long* __restrict__ peak,
                                                                double* __restrict__ partial,
-                                                               const float2* __restrict__ tabs) {
+                                                               const float2* __restrict__ tabs, int keep) {
   using F = RegFft<N>;
   constexpr int G = F::G, V = F::V, GPB = kThreads / G, NYC = N / 2;
   __shared__ float2 tiles[GPB * F::TILE];
@@ -1451,8 +1518,10 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf_reg(DevParams P, cons
     for (int q = 0; q < V / 2; q++) {
       const float ca = fabsf(v[q + V / 4].x * sc), cb = fabsf(v[q + V / 4].y * sc);
       const unsigned ia = (unsigned)(row * NYC + j + G * q), ib = ia + (unsigned)(HP * NYC);
-      sb[ia] = ca;
-      sb[ib] = cb;
+      if (keep) {
+        sb[ia] = ca;
+        sb[ib] = cb;
+      }
       const unsigned long long ka = peak_key(ca, ia), kb = peak_key(cb, ib);
       best = ka > best ? ka : best;
       best = kb > best ? kb : best;
@@ -1465,6 +1534,31 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf_reg(DevParams P, cons
   if (threadIdx.x == 0) {
     peak[b] = best;
     partial[b] = sum;
+  }
+  if (keep) return;
+  // the peak's neighbourhood (NbrWindow): its surface rows, transformed again
+  const NbrWindow nw = nbr_window(best, NYC, NYC);
+  if (threadIdx.x < 64) nw.zero_outside(sb);
+  for (int yb = nw.y_lo; yb <= nw.y_hi; yb += GPB) {   // uniform over the block
+    const int y = min(yb + g, nw.y_hi);
+    const int pr = y < HP ? y : y - HP;
+    float2 v[V];
+#pragma unroll
+    for (int r = 0; r < V; r++) {
+      const int x = j + G * r;
+      const int xm = x <= N / 2 ? x : N - x;
+      float2 a = db[(size_t)pr * N + xm], c = db[(size_t)(pr + HP) * N + xm];
+      if (x > N / 2) {
+        a.y = -a.y;
+        c.y = -c.y;
+      }
+      v[r] = make_float2(a.x - c.y, a.y + c.x);
+    }
+    F::template run<+1>(v, j, tile, t);
+    if (yb + g <= nw.y_hi)
+#pragma unroll
+      for (int q = 0; q < V / 2; q++)
+        nw.put(sb, y, j + G * q, fabsf((y < HP ? v[q + V / 4].x : v[q + V / 4].y) * sc));
   }
 }
 
@@ -1643,7 +1737,7 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
                                                      const int* __restrict__ tpeak,
                                                      const double* __restrict__ tcave,
                                                      xcorr_hip_result* __restrict__ out,
-                                                     const float2* __restrict__ tabs, int nb) {
+                                                     const float2* __restrict__ tabs, int nb, int nbr) {
   extern __shared__ float2 smem[];
   __shared__ double redd[kThreads / 64];
   __shared__ unsigned long long redk[kThreads / 64];
@@ -1772,7 +1866,8 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
       const long k = (long)(ip - 4 + iy) * nxc + (jp - 4 + ix);
       float q = 0.0f;
       if (k >= 0 && k < (long)nxc * nyc) {
-        const float v = freq ? (float)((double)surf[(size_t)b * nxc * nyc + k] / s_msc)
+        // nbr: the row kernel left only this neighbourhood, at the start of the slot (NbrWindow)
+        const float v = freq ? (float)((double)surf[(size_t)b * nxc * nyc + (nbr ? threadIdx.x : k)] / s_msc)
                              : (float)surfd[(size_t)b * nxc * nyc + k];
         q = sqrtf(sqrtf(v));
       }
@@ -3041,6 +3136,7 @@ extern "C" int xcorr_hip_correlate(xcorr_hip_ctx* c, const xcorr_hip_window* win
   if (n < 0 || (n > 0 && (!win || !out))) return fail(c, XCORR_HIP_EINVAL, "correlate: bad arguments");
   if (!c->images) return fail(c, XCORR_HIP_ESTATE, "correlate: no images set");
   HIPCK(c, hipSetDevice(c->device));
+  c->dp.keep_surf = (flags & XCORR_HIP_KEEP_SURFACE) ? 1 : 0;
   const bool dev_out = (flags & XCORR_HIP_DEVICE_PTRS) != 0;
   const bool dev_win = dev_out && !(flags & XCORR_HIP_HOST_WINDOWS);
   const bool timing = (flags & XCORR_HIP_TIMING) != 0;
@@ -3293,13 +3389,13 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   rc = launch_timed(c, timing, d.NX == 256 ? "k_rowifft_surf256" : "k_rowifft_surf_reg", [&] {
     if (d.NX == 256)
       hipLaunchKernelGGL(k_rowifft_surf256, dim3(nb), dim3(kThreads), 0, st, d, (const float2*)c->D, c->surf,
-                         c->peak, c->partial, (const float2*)c->tabs);
+                         c->peak, c->partial, (const float2*)c->tabs, d.keep_surf);
     else if (d.NX == 128)
       hipLaunchKernelGGL(k_rowifft_surf_reg<128>, dim3(nb), dim3(kThreads), 0, st, d, (const float2*)c->D, c->surf,
-                         c->peak, c->partial, (const float2*)c->tabs);
+                         c->peak, c->partial, (const float2*)c->tabs, d.keep_surf);
     else
       hipLaunchKernelGGL(k_rowifft_surf_reg<512>, dim3(nb), dim3(kThreads), 0, st, d, (const float2*)c->D, c->surf,
-                         c->peak, c->partial, (const float2*)c->tabs);
+                         c->peak, c->partial, (const float2*)c->tabs, d.keep_surf);
   });
   }
   if (rc) return rc;
@@ -3309,7 +3405,8 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
                        (const float*)c->amp2s, d_rb, 0LL, dev_out ? d_perm : (const int*)nullptr,
                        (const unsigned long long*)c->sums, (const float*)c->surf, (const double*)c->surfd,
                        (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)c->tpeak,
-                       (const double*)c->tcave, dev_out ? out : c->d_out, (const float2*)c->tabs, nb);
+                       (const double*)c->tcave, dev_out ? out : c->d_out, (const float2*)c->tabs, nb,
+                       (int)(freq && !d.keep_surf));
   });
 }
 
@@ -3527,15 +3624,15 @@ static int run_batch(xcorr_hip_ctx* c, const PatchBufs& b, const xcorr_hip_windo
       rc = launch_timed(c, timing, "k_rowifft_surf_reg", [&] {
         if (d.NX == 128)
           hipLaunchKernelGGL(k_rowifft_surf_reg<128>, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)b.D,
-                             b.surf, b.peak, b.partial, (const float2*)c->tabs);
+                             b.surf, b.peak, b.partial, (const float2*)c->tabs, d.keep_surf);
         else
           hipLaunchKernelGGL(k_rowifft_surf_reg<512>, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)b.D,
-                             b.surf, b.peak, b.partial, (const float2*)c->tabs);
+                             b.surf, b.peak, b.partial, (const float2*)c->tabs, d.keep_surf);
       });
     else if (row256)
       rc = launch_timed(c, timing, "k_rowifft_surf256", [&] {
         hipLaunchKernelGGL(k_rowifft_surf256, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)b.D,
-                           b.surf, b.peak, b.partial, (const float2*)c->tabs);
+                           b.surf, b.peak, b.partial, (const float2*)c->tabs, d.keep_surf);
       });
     else
       rc = launch_timed(c, timing, "k_rowifft_surf", [&] {
@@ -3553,7 +3650,8 @@ static int run_batch(xcorr_hip_ctx* c, const PatchBufs& b, const xcorr_hip_windo
                        (const float*)b.amp + d.N, (const int*)nullptr, (long long)2 * d.N, (const int*)nullptr,
                        (const unsigned long long*)b.sums, (const float*)b.surf, (const double*)b.surfd,
                        (const unsigned long long*)b.peak, (const double*)b.partial, (const int*)b.tpeak,
-                       (const double*)b.tcave, d_out, (const float2*)c->tabs, nb);
+                       (const double*)b.tcave, d_out, (const float2*)c->tabs, nb,
+                       (int)(freq && (regA || row256) && !d.keep_surf));
   });
   return rc;
 }
@@ -3825,6 +3923,7 @@ extern "C" int xcorr_hip_correlate_noalloc(xcorr_hip_ctx* c, const xcorr_hip_win
     return fail(c, XCORR_HIP_EINVAL, "correlate_noalloc: the only accepted flag is XCORR_HIP_NO_SYNC");
   if (!c->images) return fail(c, XCORR_HIP_ESTATE, "correlate_noalloc: no images set");
   HIPCK(c, hipSetDevice(c->device));
+  c->dp.keep_surf = 0;
   if (n == 0) return 0;
   unsigned char* base = nullptr;
   const int rs = noalloc_scratch(c, &base);
@@ -4369,6 +4468,9 @@ extern "C" long xcorr_hip_dump(xcorr_hip_ctx* c, int what, int slot, void* host,
       break;
     case 2:
       if (d.corr_flag == XCORR_HIP_FREQ) {
+        if (!d.keep_surf)
+          return fail(c, XCORR_HIP_ESTATE, "dump: the correlation surface is kept only by correlate calls with "
+                                           "XCORR_HIP_KEEP_SURFACE");
         src = bf->surf + (size_t)slot * d.nxc * d.nyc;
         nbytes = (size_t)d.nxc * d.nyc * sizeof(float);
       } else {

@@ -53,6 +53,8 @@ extern "C" {
 #define XCORR_HIP_HOST_WINDOWS 8u  /* correlate with DEVICE_PTRS: windows are host memory, results device memory */
 #define XCORR_HIP_SERIAL 64u       /* correlate: strip batches one after another on the context's stream (no second
                                       context; kernel durations then are those of a kernel alone on the device) */
+#define XCORR_HIP_KEEP_SURFACE 128u  /* correlate: write every correlation surface whole, for xcorr_hip_dump (by
+                                        default the kernels keep only each peak's 8 x 8 neighbourhood) */
 
 /* Run parameters, named after struct xcorr (xcorr_d.h:23-77).  Sizes follow
  * parse_xcorr_input.c:139-158: nx_corr = nxc = 2*xsearch, npx = 4*xsearch. */
@@ -252,7 +254,7 @@ int xcorr_hip_kernel_times(xcorr_hip_ctx *ctx, const char **names, float *ms, in
  * correlate call; it must belong to that call's last batch) to host memory,
  * for stage-by-stage parity tests.  what: 0 amp master (npy*npx f32),
  * 1 amp slave (valid on the template window; elsewhere only in some modes),
- * 2 correlation surface (nyc*nxc f32; f64 in time modes),
+ * 2 correlation surface (nyc*nxc f32, only after a call with XCORR_HIP_KEEP_SURFACE; f64 in time modes),
  * 3 patch means (2 f64).  Returns bytes copied or a negative code. */
 long xcorr_hip_dump(xcorr_hip_ctx *ctx, int what, int slot, void *host, size_t bytes);
 

@@ -89,7 +89,7 @@ def test_stage_dumps_bit_exact(xg, orc, pair):
     w, _, _ = _windows(xg, p, 3, 2)
     with xg.Engine(p) as eng:
         eng.set_images(m, s)
-        got = eng.correlate(w)
+        got = eng.correlate(w, flags=xg.KEEP_SURFACE)
         for slot in range(len(w)):
             amp1, amp2 = eng.dump(0, slot), eng.dump(1, slot)
             means, surf = eng.dump(3, slot), eng.dump(2, slot)
@@ -196,3 +196,39 @@ def test_device_pointer_path(xg, orc, pair):
         got = dout.cpu().numpy().view(xg.RESULT_DTYPE)
         host = eng.correlate(w)
     _assert_bit_exact(got, host, "device pointers vs host pointers")
+
+
+@pytest.mark.parametrize("xs", [32, 64, 128])
+def test_peak_on_surface_border_bit_exact(xg, orc, synth, xs):
+    """Offsets that put the integer peak on or next to the border of the lag
+    window, so the 8 x 8 high-res neighbourhood wraps across surface rows or
+    leaves the surface (d_cppow_strip's flat index, conv_ampl_demean.cu:75-95).
+    The default kernels keep only that neighbourhood (recomputing its rows);
+    with XCORR_HIP_KEEP_SURFACE they write the whole surface: both equal the
+    restatement bit for bit, and a surface dump needs the flag."""
+    k = xs / 64.0
+    p = xg.make_params(xs, xs)
+    n = 4 * xs
+    # lags cover D in (-31.5, 32] px in range (ri 2) and (-63, 64] px in azimuth (scaled by xs / 64)
+    shifts = [(31.7, 0.3), (-31.3, 1.2), (0.4, 63.6), (-0.6, -62.7), (31.6, 63.4), (-31.2, -62.6), (31.9, -62.8)]
+    for dx, dy in shifts:
+        m, s = synth.constant_shift_pair(2 * n + 64, 2 * n + 64, dx * k, dy * k, 0.9, seed=31, noise_seed=32)
+        w = np.zeros(6, xg.WINDOW_DTYPE)
+        w["mx0"] = [0, 32, 64, 16, 48, 60]
+        w["my0"] = [0, 64, 32, 48, 16, 60]
+        w["sx0"], w["sy0"] = w["mx0"], w["my0"]
+        with xg.Engine(p, max_batch=8) as eng:
+            eng.set_images(m, s)
+            nbr = eng.correlate(w)
+            full = eng.correlate(w, flags=xg.KEEP_SURFACE)
+            eng.correlate(w[:1], flags=xg.KEEP_SURFACE)        # dumps reach the last call's last batch
+            assert eng.dump(2, 0).max() == full["cmax"][0]
+            eng.correlate(w[:1])
+            with pytest.raises(xg.XcorrError, match="KEEP_SURFACE"):
+                eng.dump(2, 0)
+        ref = orc.run(_oracle_params(orc, p), m, s, _locs(w), precision=32)
+        label = "xs %d shift (%.1f, %.1f)" % (xs, dx * k, dy * k)
+        _assert_bit_exact(nbr, full, label + " neighbourhood vs whole surface")
+        _assert_bit_exact(nbr, ref, label)
+        edge = (np.minimum(nbr["ip"], 2 * xs - 1 - nbr["ip"]) < 4) | (np.minimum(nbr["jp"], 2 * xs - 1 - nbr["jp"]) < 4)
+        assert edge.sum() >= 3, label

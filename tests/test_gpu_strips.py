@@ -117,7 +117,7 @@ def test_strip_dumps_in_shared_strip(xg, orc, pairs, xs):
         xg.WINDOW_DTYPE).reshape(-1)
     with xg.Engine(p) as eng:
         eng.set_images(m, s)
-        eng.correlate(w)
+        eng.correlate(w, flags=xg.KEEP_SURFACE)
         for slot in (1, 2):
             amp1, amp2 = eng.dump(0, slot), eng.dump(1, slot)
             means, surf = eng.dump(3, slot), eng.dump(2, slot)
@@ -168,7 +168,7 @@ def test_strip_batches_on_two_contexts(xg, orc, pairs, corr):
             _assert_bit_exact(got, ref, "%s max_batch %d, host results" % (corr, mb))
             _assert_bit_exact(eng.correlate(w, flags=xg.SERIAL), ref, "%s max_batch %d, serial" % (corr, mb))
             d = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
-            eng.correlate(w, out=d, flags=xg.NO_SYNC)
+            eng.correlate(w, out=d, flags=xg.NO_SYNC | xg.KEEP_SURFACE)
             eng.sync()
             _assert_bit_exact(d.cpu().numpy().view(xg.RESULT_DTYPE), ref, "%s max_batch %d, device" % (corr, mb))
             dumps = {}
@@ -180,7 +180,7 @@ def test_strip_batches_on_two_contexts(xg, orc, pairs, corr):
             assert dumps, "no location of the last batch found"
             t = (slice(xs, 3 * xs), slice(xs, 3 * xs))   # the slave amplitudes are kept on the template only
             for loc, got_d in dumps.items():
-                eng.correlate(w[loc:loc + 1])       # one batch, on the engine's own context
+                eng.correlate(w[loc:loc + 1], flags=xg.KEEP_SURFACE)   # one batch, on the engine's own context
                 for k in range(4):
                     one = eng.dump(k, 0)
                     a, b = (got_d[k][t], one[t]) if k == 1 else (got_d[k], one)
