@@ -1052,6 +1052,8 @@ __device__ __forceinline__ float2 demean_row_bin(float2 v, int kx, bool ym, floa
 // transformed once and C3 needs no second pass over Z.  Requires npx >= 32.
 // Row spectra come from Z[b] (rb null) or, for strips, from line rb[b] + y of
 // Zm (the packed transform; unmasked rows extract the master's spectrum).
+// STRIPS: rows from strip lines (rb), else from Z[b] (rb unused).
+template <bool STRIPS>
 __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const float2* __restrict__ Z,
                                                          const int* __restrict__ rb,
                                                          const unsigned long long* __restrict__ sums,
@@ -1070,7 +1072,7 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
   const int wi = xcd_item();
   if (wi >= nwork) return;
   const int p = wi % pairs, b = wi / pairs;
-  const size_t row0 = rb ? (size_t)rb[b] : (size_t)b * 256;
+  const size_t row0 = STRIPS ? (size_t)rb[b] : (size_t)b * 256;
   // npy = 256 gives ys = 64: row y0 + 16 k (y0 < 16) is masked iff 4 <= k < 12
   const float2* z0 = Z + (row0 + (threadIdx.x >> 4)) * NX;
   const int kxT = 16 * p + g;
@@ -1096,14 +1098,14 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
       // spectrum is Hermitian (the producer extracted it), so column npx - kx
       // is conj(column kx) (npx/2 is stored); strip lines hold the packed
       // transform, from which the master's spectrum is extracted here
-      if ((k >= 4 && k < 12) || rb) {
+      if ((k >= 4 && k < 12) || STRIPS) {
         rM[k] = zr[kxm];
       } else {
         const float2 w = zr[self ? kxm : kxt];   // branch-free; a cache hit for kxt
         rM[k] = self ? w : make_float2(w.x, -w.y);
       }
     }
-    if (rb)
+    if (STRIPS)
 #pragma unroll
       for (int k = 0; k < 16; k++)
         if (!(k >= 4 && k < 12)) {
@@ -3378,7 +3380,7 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   } else {
   rc = launch_timed(c, timing, d.NX == 256 ? "k_colpass256" : "k_colpass", [&] {
     if (d.NX == 256)
-      hipLaunchKernelGGL(k_colpass256, dim3((nb * 8 + 7) & ~7), dim3(kThreads), 0, st, d, (const float2*)c->Z, d_rb,
+      hipLaunchKernelGGL(k_colpass256<true>, dim3((nb * 8 + 7) & ~7), dim3(kThreads), 0, st, d, (const float2*)c->Z, d_rb,
                          (const unsigned long long*)c->sums, c->D, (const float2*)c->tabs, nb * 8);
     else if (d.NX == 128)
       launch_colpass<128>(c, ctx_bufs(c), d, st, d_rb, nb);
@@ -3603,7 +3605,7 @@ static int run_batch(xcorr_hip_ctx* c, const PatchBufs& b, const xcorr_hip_windo
       });
     } else if (col256 && d.NX >= 32) {
       rc = launch_timed(c, timing, "k_colpass256", [&] {
-        hipLaunchKernelGGL(k_colpass256, dim3((nb * (d.NX / 32) + 7) & ~7), dim3(kThreads), 0, c->stream, d,
+        hipLaunchKernelGGL(k_colpass256<false>, dim3((nb * (d.NX / 32) + 7) & ~7), dim3(kThreads), 0, c->stream, d,
                            (const float2*)b.Z, (const int*)nullptr,
                            (const unsigned long long*)b.sums, b.D, (const float2*)c->tabs, nb * (d.NX / 32));
       });
