@@ -31,7 +31,7 @@ struct RegFft<128> {
       t[7 + r - 1] = tw[r * (j + 8)];
     }
   }
-  // natural-order slot of element i in the tile (Hermitian completion, line buffers)
+  // natural-order slot of element i in the tile (spectrum extraction, line buffers)
   __device__ __forceinline__ static int slot(int i) { return i + (i >> 4); }
   template <int DIR>
   __device__ __forceinline__ static void run(float2 (&v)[V], int j, float2* tile, const float2 (&t)[TW]) {

@@ -467,7 +467,7 @@ __device__ __forceinline__ void range_row(const void* __restrict__ src, int form
 // images' range interpolation and amplitudes (kept for the NCC: the master's
 // everywhere, the slave's on the template), exact fixed-point sums, then
 // z = amp1 + i amp2 mask and its FFT along x (the means come off in the
-// spectrum, demean_row_bin), Hermitian completion of unmasked rows.
+// spectrum, demean_row_bin); unmasked rows keep the master's spectrum extracted from the packed transform.
 template <int N, bool ROWFFT>
 __global__ void __launch_bounds__(kThreads) k_stageA(DevParams P, const void* __restrict__ master,
                                                      const void* __restrict__ slave,
