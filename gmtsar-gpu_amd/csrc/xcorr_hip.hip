@@ -1862,7 +1862,7 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
   if (P.interp_flag) {
     const int Lx = P.Lx, Ly = P.Ly;
     float2* rows = smem;                 // 8 x Lx
-    float2* Xc = rows + 8 * Lx;          // Lx x 8
+    float2* Xc = rows + 8 * Lx;          // 8 x Lx: column t's spectrum at Xc[k Lx + t] (consecutive t, no bank conflicts)
     if (threadIdx.x < 64) {
       const int iy = threadIdx.x >> 3, ix = threadIdx.x & 7;
       const long k = (long)(ip - 4 + iy) * nxc + (jp - 4 + ix);
@@ -1910,7 +1910,7 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
       for (int k = 0; k < 8; k++) v[k] = rows[k * Lx + t];
       Dft<8, -1>::run(v);
 #pragma unroll
-      for (int k = 0; k < 8; k++) Xc[t * 8 + k] = v[k];
+      for (int k = 0; k < 8; k++) Xc[k * Lx + t] = v[k];
     }
     __syncthreads();
     // E8[s][k] = e^{2 pi i kt s / Ly}, the row factors of output row s, come
@@ -1926,7 +1926,7 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
     auto scan = [&](int s0, int t) {
       float2 X[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++) X[k] = Xc[t * 8 + k];
+      for (int k = 0; k < 8; k++) X[k] = Xc[k * Lx + t];
       for (int s = s0; s < Ly; s += per) {
         const float2* es = E8 + s * 8;
         float acc = 0.0f;
