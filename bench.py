@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--batch", type=int, default=8192, help="locations per engine batch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-others", action="store_true",
+                    help="strip workload only: skip the compact batched / proposal / dense records")
     ap.add_argument("--serial", action="store_true",
                     help="strip batches one after another on one stream (default: consecutive batches overlap on "
                          "two contexts); rocprof kernel durations are then those of each kernel alone")
@@ -107,18 +109,28 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="batched: skip the autotuner")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
+    ap.add_argument("--copy-probe-mode", default="plain", choices=("plain", "transpose"),
+                    help="transpose: the copy reads a (MIB*4096 x 32) complex64 matrix column by column "
+                         "(strided 8-byte reads, each byte read once) into its transpose")
     ap.add_argument("--traffic", default="", metavar="JSON",
                     help="per-launch HBM bytes of the dominant kernel from rocprofv3 FETCH_SIZE/WRITE_SIZE runs")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
         nbytes = args.copy_probe << 20
-        src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
+        if args.copy_probe_mode == "plain":
+            src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+            dst = torch.empty_like(src)
+            for _ in range(3):
+                dst.copy_(src)
+        else:
+            rows = nbytes // (32 * 8)
+            src = torch.ones(rows, 32, dtype=torch.complex64, device="cuda")
+            dst = torch.empty(32, rows, dtype=torch.complex64, device="cuda")
+            for _ in range(3):
+                dst.copy_(src.t())
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
+        print(json.dumps({"copy_probe_bytes": nbytes, "mode": args.copy_probe_mode}), flush=True)
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the xcorr path has independent patches and this build "
@@ -134,12 +146,10 @@ def main():
         print("no GPU available", file=sys.stderr)
         return 1
     dev = torch.device("cuda:0")
-    if args.workload == "batched":
-        return bench_batched(args, xg, synth, torch, dev)
-    if args.workload == "proposal":
-        return bench_proposal(args, xg, synth, torch, dev)
-    if args.workload == "dense":
-        return bench_dense(args, xg, synth, torch, dev)
+    others = {"batched": bench_batched, "proposal": bench_proposal, "dense": bench_dense}
+    if args.workload in others:
+        print(json.dumps(others[args.workload](args, xg, synth, torch, dev)), flush=True)
+        return 0
 
     t0 = time.time()
     if args.small or args.workload == "small":
@@ -300,9 +310,42 @@ def main():
                       "batches": "serial" if args.serial else "overlapped on two contexts"},
            "roofline": roofline, "cpu_baseline": cpu, "path_roofline": path, "kernels": kernels,
            "step_stats": step_stats, "h2d": h2d, "parity": parity}
-    print(json.dumps(out), flush=True)
     eng.close()
+    del m, s, dout
+    torch.cuda.empty_cache()
+    if not (args.small or args.workload == "small" or args.no_others):
+        out["workloads"] = other_workloads(args, xg, synth, torch, dev)
+    print(json.dumps(out), flush=True)
     return 0
+
+
+def other_workloads(args, xg, synth, torch, dev):
+    """Compact records of the batched, proposal and dense workloads, run after
+    the headline one with few steps and a short CPU sample each (their full
+    records: --workload batched / proposal / dense)."""
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup = min(args.steps, 3), 1
+    sub.cpu_seconds = max(1.0, args.cpu_seconds / 4)
+    recs = {}
+    for name, fn in (("batched", bench_batched), ("proposal", bench_proposal), ("dense", bench_dense)):
+        t0 = time.time()
+        r = fn(sub, xg, synth, torch, dev)
+        torch.cuda.empty_cache()
+        rec = {"workload": r["config"]["workload"], "patches": r["config"]["patches"], "value": r["value"],
+               "unit": r["unit"], "ms_per_step": r["ms_per_step"], "steps": r["steps"],
+               "cpu_baseline": None if not r.get("cpu_baseline") else
+               {k: r["cpu_baseline"][k] for k in ("value", "cores", "kind", "sample")},
+               "parity": r.get("parity"), "wall_s": round(time.time() - t0, 1)}
+        if "accuracy" in r:
+            rec["accuracy"] = r["accuracy"]
+        if "proposal" in r:
+            pp = r["proposal"]
+            rec["proposal"] = {k: pp[k] for k in ("plain_patches_per_s", "proposal_patches_per_s", "speedup",
+                                                  "tau", "shrunken_fraction", "rmse_dx_px", "rmse_dy_px",
+                                                  "cnn_ms")}
+        recs[name] = rec
+        log("%s: %.0f patches/s (%.1f s)" % (name, r["value"], rec["wall_s"]))
+    return recs
 
 
 def path_flops(npx, npy, ri, ifc):
@@ -318,7 +361,7 @@ def path_flops(npx, npy, ri, ifc):
 
 
 def batched_workload(xg, synth, torch, dev, npairs, seed=3):
-    """Config 3: npairs complex-int16 pairs, sizes uniform in [2048, 12000] range x
+    """The batched uneven workload: npairs complex-int16 pairs, sizes uniform in [2048, 12000] range x
     [4096, 28000] azimuth, each with its own constant-plus-linear offset field
     (|d| <= 20 px; the constant part at the image centre goes into rshift /
     ashift as a GMTSAR PRM would carry it), coherence in [0.3, 0.95], patch
@@ -409,9 +452,8 @@ def bench_batched(args, xg, synth, torch, dev):
                                   "interp 8/16, one HIP graph launch per step" % len(work),
                       "patches": n, "classes": per_class},
            "roofline": roofline, "cpu_baseline": cpu, "path_roofline": path, "accuracy": accuracy, "parity": parity}
-    print(json.dumps(out), flush=True)
     mu.close()
-    return 0
+    return out
 
 
 def bench_proposal(args, xg, synth, torch, dev):
@@ -492,9 +534,8 @@ def bench_proposal(args, xg, synth, torch, dev):
                                   "-xsearch 64 -ysearch 64, interp 16; shrunken search xsearch %d" % args.xs_small,
                       "patches": n, "weights": os.path.relpath(args.weights, ROOT)},
            "roofline": None, "cpu_baseline": None, "proposal": proposal}
-    print(json.dumps(out), flush=True)
     pr.close()
-    return 0
+    return out
 
 
 def bench_dense(args, xg, synth, torch, dev):
@@ -572,9 +613,8 @@ def bench_dense(args, xg, synth, torch, dev):
                         "unit": "GB/s", "frac": round(nbytes / step_s / 1e9 / HBM_PEAK_GBS, 5), "traffic": None,
                         "kernel": "whole graph (both classes)", "launch_ms": round(step_s * 1e3, 4)},
            "cpu_baseline": cpu, "accuracy": accuracy, "parity": parity}
-    print(json.dumps(out), flush=True)
     mu.close()
-    return 0
+    return out
 
 
 def dense_accuracy(xg, synth, field, work):

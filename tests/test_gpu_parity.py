@@ -130,7 +130,7 @@ def test_tolerance_vs_fp64_oracle(xg, orc, pair, xs, nxl, nyl):
     np.testing.assert_allclose(got["cmax"], ref["cmax"], rtol=1e-5)
 
 
-def test_full_config1_recovers_shift(xg, orc, pair):
+def test_small_workload_recovers_shift(xg, orc, pair):
     """The small workload: 1024 x 2048, -nx 20 -ny 50 -xsearch 64 -ysearch 64 (1000 patches)."""
     m, s = pair
     p = xg.make_params(64, 64)
