@@ -726,7 +726,7 @@ struct LineSum {
 // lines from global line blk[2 (b + blk_off) + 1] of strip blk[2 (b + blk_off)].
 // EDGE: the strips' windows may cross the image sides.
 template <int FMT, bool EDGE>
-__global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams P, const void* __restrict__ master,
+__global__ void __launch_bounds__(kThreads, EDGE ? 2 : 4) k_stripA256(DevParams P, const void* __restrict__ master,
                                                         const void* __restrict__ slave,
                                                         const StripDesc* __restrict__ strips,
                                                         const int* __restrict__ blk, int blk_off, int blk_n,
@@ -735,17 +735,16 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
                                                         float* __restrict__ amp1, float* __restrict__ amp2,
                                                         long long* __restrict__ lsum, float2* __restrict__ Zm,
                                                         const float2* __restrict__ tabs) {
-  // one shared array: 16 transpose tiles | 16 master lines (column pairs) |
-  // pass-2 twiddles w256^(r j) [16][15] | phase factors phi[0..128]
-  __shared__ float2 lds[16 * kT256Tile + 16 * 128 + 240 + 129];
+  // one shared array: 16 transpose tiles | pass-2 twiddles w256^(r j)
+  // [16][15] | phase factors phi[0..128] (37.8 KB: 4 blocks per CU)
+  __shared__ float2 lds[16 * kT256Tile + 240 + 129];
   const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
   const int wi = xcd_item();
   if (wi >= blk_n) return;
   const int bb = wi + blk_off;
   const StripDesc S = strips[blk[2 * bb]];
   float2* tile = lds + g * kT256Tile;
-  float2* mr = lds + 16 * kT256Tile + g * 128;
-  float2* twl = lds + 16 * kT256Tile + 16 * 128;
+  float2* twl = lds + 16 * kT256Tile;
   float2* sph = twl + 240;
   for (int k = threadIdx.x; k < 240; k += kThreads) twl[k] = tabs[P.tw_nx + (k % 15 + 1) * (k / 15)];
   for (int k = threadIdx.x; k <= 128; k += kThreads) sph[k] = tabs[P.phi + k];
@@ -754,7 +753,7 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
   const int first = blk[2 * bb + 1] - S.line0;                   // strip-relative line of this block
   const int nit = min(kStripBlockLines, S.nlines - first + 15) >> 4;
   // Both lines of an iteration are loaded together, before the master
-  // transforms (3 waves per SIMD hide the rest of the latency).
+  // transforms (4 waves per SIMD hide the rest of the latency).
   __syncthreads();
   for (int it = 0; it < nit; it++) {
     const int i = first + 16 * it + g;                            // strip-relative line of this group
@@ -773,13 +772,21 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
       for (int q = 0; q < 8; q++) {
         const float2 a = make_float2(ea[q], oa[q]);
         if (act) st_nt(o + j + 16 * q, a);
-        mr[j + 16 * q] = a;
+        tile[j + 16 * q] = a;                      // through the tile into column order
         s.add(ea[q]);
         s.add(oa[q]);
       }
       const long long st = s.group_total();
       if (act && j == 15) lsum[2 * l] = st;
     }
+    float m1v[16];                              // master amplitude of column j + 16 r
+    wave_sync();
+    {
+      const float* m1 = (const float*)tile;
+#pragma unroll
+      for (int r = 0; r < 16; r++) m1v[r] = m1[j + 16 * r];
+    }
+    wave_sync();                                // read before the slave's transforms reuse the tile
     {
       float ea[8], oa[8];
       interp_line256<FMT>(us, j, tile, twl, sph, ea, oa);
@@ -796,14 +803,13 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
       const long long st = s.group_total();
       if (act && j == 15) lsum[2 * l + 1] = st;
     }
-    wave_sync();                                // the group's two lines are in LDS
-    const float* m1 = (const float*)mr;
+    wave_sync();                                // the slave line is in LDS
     const float* a2 = (const float*)tile;
     float2 v[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {              // npx = 256, xs = 64: x in [xs, npx - xs) iff r in [4, 12)
       const int x = j + 16 * r;
-      v[r] = make_float2(m1[x], (r >= 4 && r < 12) ? a2[x] : 0.0f);
+      v[r] = make_float2(m1v[r], (r >= 4 && r < 12) ? a2[x] : 0.0f);
     }
     wave_sync();                                // read before the transform reuses the tile
     // one transform serves both uses of the line: masked rows take it as is,
@@ -814,7 +820,7 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 3) k_stripA256(DevParams 
 #pragma unroll
       for (int r = 0; r < 16; r++) st_nt(Zm + l * 256 + j + 16 * r, v[r]);
     }
-    wave_sync();                                // the next line's loads into mr / tile follow
+    wave_sync();                                // the next line's writes into the tile follow
   }
 }
 
