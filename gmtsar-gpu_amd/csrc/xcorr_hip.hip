@@ -1380,43 +1380,54 @@ __global__ void __launch_bounds__(kThreads) k_colpass(DevParams P, const float2*
   const size_t row0 = rb ? (size_t)rb[b] : (size_t)b * N;
   auto kx_of = [&](int i, bool mirror) { return !mirror ? TC * p + i : ((p == 0 && i == 0) ? N / 2 : N - TC * p - i); };
   float2 vT[CPG][V], vM[CPG][V];
-  // Thread (y0, i) loads column i of the tile at rows y0 + LS k: all LK loads
-  // are in flight together, then the demeaned values go through LDS.
+  // Thread (y0, i) loads columns i of the T and M tiles at rows y0 + LS k:
+  // all 2 LK loads are in flight together (the unmasked strip rows extract the
+  // master's spectrum from the pair in registers), then the demeaned values go
+  // through LDS.
   static_assert(kThreads % TC == 0 && (N * TC) % kThreads == 0, "column tile");
   constexpr int LS = kThreads / TC, LK = N * TC / kThreads;
   const int i0 = threadIdx.x % TC, y0 = threadIdx.x / TC;
   // square windows: ys = N / 4, so row y0 + LS k (y0 < LS) is masked iff
   // LK / 4 <= k < 3 LK / 4 (known at compile time)
   static_assert(LS * LK == N, "column tile rows");
-  const float2* z0 = Z + (row0 + y0) * N;
-  for (int half = 0; half < 2; half++) {
-    const int kx = kx_of(i0, half == 1);
-    const int kq = (N - kx) & (N - 1);                  // the mirror bin
-    const float2 mk = tabs[P.mk_x + kx];
-    float2 zr[LK];
+  {
+    const float2* z0 = Z + (row0 + y0) * N;
+    const int kxt = kx_of(i0, false), kxm = kx_of(i0, true);   // mirror bins of each other
+    const bool self = p == 0 && i0 == 0;                        // columns 0 and N/2 mirror onto themselves
+    const float2 mkt = tabs[P.mk_x + kxt], mkm = tabs[P.mk_x + kxm];
+    float2 rT[LK], rM[LK];
 #pragma unroll
     for (int k = 0; k < LK; k++) {
-      const bool ym = k >= LK / 4 && k < 3 * LK / 4;
-      zr[k] = z0[(size_t)(LS * k) * N + kx];
-      // unmasked strip rows hold the packed transform: extract the master's spectrum
-      if (!ym && rb) zr[k] = extract_bin(zr[k], z0[(size_t)(LS * k) * N + kq]);
+      rT[k] = z0[(size_t)(LS * k) * N + kxt];
+      rM[k] = z0[(size_t)(LS * k) * N + kxm];
     }
-    __syncthreads();
+    // unmasked strip rows hold the packed transform: extract the master's spectrum
+    if (rb)
 #pragma unroll
-    for (int k = 0; k < LK; k++) {
-      const int y = y0 + LS * k;
-      T[y * PITCH + i0] = demean_row_bin(zr[k], kx, k >= LK / 4 && k < 3 * LK / 4, m1NX, m2f, mk);
-    }
+      for (int k = 0; k < LK; k++)
+        if (!(k >= LK / 4 && k < 3 * LK / 4)) {
+          const float2 a = rT[k], q = rM[k];                    // Z[kx], Z[N - kx] (self: Z[0], Z[N/2])
+          rT[k] = extract_bin(a, self ? a : q);
+          rM[k] = extract_bin(q, self ? q : a);
+        }
+#pragma unroll
+    for (int k = 0; k < LK; k++)
+      T[(y0 + LS * k) * PITCH + i0] = demean_row_bin(rT[k], kxt, k >= LK / 4 && k < 3 * LK / 4, m1NX, m2f, mkt);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < CPG; c++)
 #pragma unroll
-      for (int r = 0; r < V; r++) {
-        const float2 a = T[(j + G * r) * PITCH + g * CPG + c];
-        if (half == 0) vT[c][r] = a;
-        else vM[c][r] = a;
-      }
+      for (int r = 0; r < V; r++) vT[c][r] = T[(j + G * r) * PITCH + g * CPG + c];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < LK; k++)
+      T[(y0 + LS * k) * PITCH + i0] = demean_row_bin(rM[k], kxm, k >= LK / 4 && k < 3 * LK / 4, m1NX, m2f, mkm);
   }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < CPG; c++)
+#pragma unroll
+    for (int r = 0; r < V; r++) vM[c][r] = T[(j + G * r) * PITCH + g * CPG + c];
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < CPG; c++) {

@@ -22,6 +22,7 @@ CASES = [
     # xsearch, ysearch, corr_flag (0 time, 1 time4, 2 freq), interp factor, windows of call 1, of call 2
     (64, 64, 2, 16, 100, 20),       # 48 windows per internal batch: 3 batches, then 1
     (128, 128, 2, 8, 30, 7),        # 12 per batch
+    (32, 32, 2, 8, 400, 41),        # 128 x 128 windows through the register-resident per-patch kernels
     (16, 32, 2, 16, 500, 50),       # 388 per batch
     (32, 32, 0, 8, 400, 37),        # 339 per batch
     (32, 32, 1, 16, 450, 9),        # 407 per batch
