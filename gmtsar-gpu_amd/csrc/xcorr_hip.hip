@@ -1518,6 +1518,10 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf_reg(DevParams P, cons
   unsigned long long best = 0ull;
   double sum = 0.0;
   constexpr int HP = NYC / 2;                      // rows row and row + HP share one transform
+  // one pass (N = 128: a row pair per group): the lanes keep their surface
+  // values for the neighbourhood instead of transforming its rows again
+  constexpr bool ONE = HP == GPB;
+  float ka_v[ONE ? V / 2 : 1], kb_v[ONE ? V / 2 : 1];
   for (int row0 = 0; row0 < HP; row0 += GPB) {    // uniform trip count (HP is a multiple of GPB)
     const int row = row0 + g;
     float2 v[V];
@@ -1541,6 +1545,10 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf_reg(DevParams P, cons
         sb[ia] = ca;
         sb[ib] = cb;
       }
+      if (ONE) {
+        ka_v[ONE ? q : 0] = ca;
+        kb_v[ONE ? q : 0] = cb;
+      }
       const unsigned long long ka = peak_key(ca, ia), kb = peak_key(cb, ib);
       best = ka > best ? ka : best;
       best = kb > best ? kb : best;
@@ -1555,9 +1563,18 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf_reg(DevParams P, cons
     partial[b] = sum;
   }
   if (keep) return;
-  // the peak's neighbourhood (NbrWindow): its surface rows, transformed again
+  // the peak's neighbourhood (NbrWindow): from the values the lanes hold, or
+  // its surface rows transformed again
   const NbrWindow nw = nbr_window(best, NYC, NYC);
   if (threadIdx.x < 64) nw.zero_outside(sb);
+  if (ONE) {
+#pragma unroll
+    for (int q = 0; q < (ONE ? V / 2 : 0); q++) {
+      nw.put(sb, g, j + G * q, ka_v[q]);
+      nw.put(sb, g + HP, j + G * q, kb_v[q]);
+    }
+    return;
+  }
   for (int yb = nw.y_lo; yb <= nw.y_hi; yb += GPB) {   // uniform over the block
     const int y = min(yb + g, nw.y_hi);
     const int pr = y < HP ? y : y - HP;

@@ -203,7 +203,8 @@ def test_peak_on_surface_border_bit_exact(xg, orc, synth, xs):
     """Offsets that put the integer peak on or next to the border of the lag
     window, so the 8 x 8 high-res neighbourhood wraps across surface rows or
     leaves the surface (d_cppow_strip's flat index, conv_ampl_demean.cu:75-95).
-    The default kernels keep only that neighbourhood (recomputing its rows);
+    The default kernels keep only that neighbourhood (recomputing its rows,
+    or, for 128 x 128 windows, from the values the lanes still hold);
     with XCORR_HIP_KEEP_SURFACE they write the whole surface: both equal the
     restatement bit for bit, and a surface dump needs the flag."""
     k = xs / 64.0
