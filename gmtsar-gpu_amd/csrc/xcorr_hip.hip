@@ -46,7 +46,7 @@ struct DevParams {
   int groups4;                    // k_rowifft_surf blocks per patch
   int tLR, tRC;                   // k_time_search: surface rows per block, template rows per LDS chunk
   int tw_nx, tw_ny, tw_lx, tw_ly, phi;   // offsets into the table buffer
-  int e8;                         // high-res row factors e^{2 pi i k~ s / Ly}, [Ly][8]
+  int e4;                         // high-res row factors e^{2 pi i k s / Ly}, [Ly/2 + 1][4] (k = 1..4)
   int mk_x, mk_y;                 // spectra of the 1-D mask boxes (npx, npy entries)
   int m_nx, m_ny, s_nx, s_ny;
   // lines held in device memory (all of each image, or a streamed band):
@@ -1947,32 +1947,48 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
       for (int k = 0; k < 8; k++) Xc[k * Lx + t] = v[k];
     }
     __syncthreads();
-    // E8[s][k] = e^{2 pi i kt s / Ly}, the row factors of output row s, come
-    // from the table in global memory.  Each thread owns one column t (its 8
-    // spectrum values in registers) and walks the rows s in increasing order;
-    // when Lx is a multiple of 64 the lanes of a wave share s and the factors
-    // are scalar loads.  The running maximum keeps the first index (strict >)
-    // and skips values <= 0 and NaN, as the peak key does.
-    const float2* E8 = tabs + P.e8;
-    float bestv = 0.0f;
-    unsigned besti = 0u;
+    // The column pass forms the real part only, with the bins k and 8 - k
+    // paired (P_k = X[k].re + X[8-k].re, Q_k = X[k].im - X[8-k].im) and rows
+    // s and Ly - s from the same two sums: a = X[0].re + sum_k cos P_k +
+    // cos_4 X[4].re, c = sum_k sin Q_k - sin_4 X[4].im (k = 1, 2, 3 in order),
+    // v(s) = (a - c) / 8, v(Ly - s) = (a + c) / 8.  E4[s][k - 1] =
+    // e^{2 pi i k s / Ly} (k = 1..4, s = 0..Ly/2) comes from the table in
+    // global memory.  Each thread owns one column t (its paired spectrum in
+    // registers) and walks s = 0..Ly/2; when Lx is a multiple of 64 the lanes
+    // of a wave share s and the factors are scalar loads.  The maximum is over
+    // peak keys (the largest value, then the smallest index; values <= 0 and
+    // NaN never win), so the order of the rows does not matter.
+    const float2* E4 = tabs + P.e4;
+    unsigned long long bestk = 0ull;
     const int per = kThreads >= Lx ? kThreads / Lx : 1;
+    const int hL = Ly / 2;
     auto scan = [&](int s0, int t) {
       float2 X[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) X[k] = Xc[k * Lx + t];
-      for (int s = s0; s < Ly; s += per) {
-        const float2* es = E8 + s * 8;
-        float acc = 0.0f;
+      float pk[3], qk[3];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+      for (int k = 1; k < 4; k++) {
+        pk[k - 1] = X[k].x + X[8 - k].x;
+        qk[k - 1] = X[k].y - X[8 - k].y;
+      }
+      for (int s = s0; s <= hL; s += per) {
+        const float2* es = E4 + s * 4;
+        float a = X[0].x, c = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
           const float2 ey = es[k];
-          acc = __builtin_fmaf(X[k].x, ey.x, __builtin_fmaf(-X[k].y, ey.y, acc));
+          a = __builtin_fmaf(ey.x, pk[k], a);
+          c = __builtin_fmaf(ey.y, qk[k], c);
         }
-        const float v = acc * sy;
-        if (v > bestv) {
-          bestv = v;
-          besti = (unsigned)(s * Lx + t);
+        const float2 e4 = es[3];
+        a = __builtin_fmaf(e4.x, X[4].x, a);
+        c = __builtin_fmaf(-e4.y, X[4].y, c);
+        const unsigned long long k1 = peak_key((a - c) * sy, (unsigned)(s * Lx + t));
+        bestk = k1 > bestk ? k1 : bestk;
+        if (s > 0 && s < hL) {
+          const unsigned long long k2 = peak_key((a + c) * sy, (unsigned)((Ly - s) * Lx + t));
+          bestk = k2 > bestk ? k2 : bestk;
         }
       }
     };
@@ -1983,7 +1999,7 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
       if (per == 1 || Lx % 64 == 0) scan(__builtin_amdgcn_readfirstlane(s0), t);   // s uniform over the wave
       else scan(s0, t);
     }
-    unsigned long long best = block_max_u64(peak_key(bestv, besti), redk);
+    unsigned long long best = block_max_u64(bestk, redk);
     float bv;
     unsigned bidx;
     key_decode(best, &bv, &bidx);
@@ -2960,9 +2976,9 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   };
   d.mk_x = add_mask(d.NX, d.xs);
   d.mk_y = add_mask(d.NY, d.ys);
-  d.e8 = (int)tab.size();                 // E8[s][k] = e^{2 pi i k~ s / Ly}, k~ = k (k < 4) or k - 8
-  for (int s = 0; s < d.Ly; s++)
-    for (int k = 0; k < 8; k++) tab.push_back(twf(d.Ly, (long)(((k < 4 ? k : k - 8) * s) & (d.Ly - 1))));
+  d.e4 = (int)tab.size();                 // E4[s][k - 1] = e^{2 pi i k s / Ly}, k = 1..4, s = 0..Ly/2
+  for (int s = 0; s <= d.Ly / 2; s++)
+    for (int k = 1; k <= 4; k++) tab.push_back(twf(d.Ly, (long)((k * s) & (d.Ly - 1))));
   d.phi = (int)tab.size();
   const int M = d.ri * d.NX;
   // the strip kernel stores phi[0..npx/2] only and rebuilds phi[k] = conj(phi[npx - k])

@@ -642,6 +642,14 @@ done:
 /* (fft_interpolate_routines.cu:76-147: rows, then columns, Nyquist bin at  */
 /* -n/2, each pass * 1/n); returns the real part.  Each 1-D pass is the    */
 /* n-term sum sum_k X[k] e^{2 pi i ktilde t / L} / n, evaluated directly.  */
+/* The column pass forms only the real part, with the bins k and n - k     */
+/* paired: Re(X[k] e^{i k phi} + X[n-k] e^{-i k phi}) = cos(k phi) P_k -     */
+/* sin(k phi) Q_k (P_k = X[k].re + X[n-k].re, Q_k = X[k].im - X[n-k].im),   */
+/* and rows s and L - s (phi -> -phi) from the same two sums:              */
+/* a = X[0].re + sum_k cos P_k + cos_{n/2} X[n/2].re,                      */
+/* c = sum_k sin Q_k - sin_{n/2} X[n/2].im (k = 1 .. n/2 - 1, in order),   */
+/* v(s) = (a - c) / n and v(L - s) = (a + c) / n, with the factors of row   */
+/* s (0 <= s <= L/2).                                                        */
 /* ------------------------------------------------------------------------ */
 int xo32_interp2d(const float *md, int n2x, int n2y, int ifc, float *hres) {
   const int Lx = n2x * ifc, Ly = n2y * ifc;
@@ -667,19 +675,22 @@ int xo32_interp2d(const float *md, int n2x, int n2y, int ifc, float *hres) {
       rows[(size_t)iy * Lx + t] = acc;
     }
   }
+  const int h = n2y / 2;
   for (int t = 0; t < Lx; t++) {
     for (int k = 0; k < n2y; k++) X[k] = rows[(size_t)k * Lx + t];
     xo32_fft(X, n2y, -1);
-    for (int s = 0; s < Ly; s++) {
-      float acc = 0.0f;
-      for (int k = 0; k < n2y; k++) {
-        long kt = (k < n2y / 2) ? k : k - n2y;
-        long m = (kt * s) % Ly;
-        if (m < 0) m += Ly;
-        c32 e = twf(Ly, m, +1);
-        acc = fmaf(X[k].re, e.re, fmaf(-X[k].im, e.im, acc));
+    for (int s = 0; s <= Ly / 2; s++) {
+      float a = X[0].re, c = 0.0f;
+      for (int k = 1; k < h; k++) {
+        c32 e = twf(Ly, ((long)k * s) % Ly, +1);
+        a = fmaf(e.re, X[k].re + X[n2y - k].re, a);
+        c = fmaf(e.im, X[k].im - X[n2y - k].im, c);
       }
-      hres[(size_t)s * Lx + t] = acc * sy;
+      c32 e = twf(Ly, ((long)h * s) % Ly, +1);
+      a = fmaf(e.re, X[h].re, a);
+      c = fmaf(-e.im, X[h].im, c);
+      hres[(size_t)s * Lx + t] = (a - c) * sy;
+      if (s > 0 && s < Ly / 2) hres[(size_t)(Ly - s) * Lx + t] = (a + c) * sy;
     }
   }
   free(rows);
