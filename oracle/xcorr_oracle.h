@@ -20,7 +20,7 @@
  *           independent radix-2 FFT.  It is the tolerance reference for
  *           sub-pixel offsets and correlation values.
  *
- * Parity pins (tests/test_oracle_*.py): FFTs against numpy.fft and a direct
+ * Parity pins (tests/test_oracle.py): FFTs against numpy.fft and a direct
  * DFT; range interpolation against its closed forms (original samples at
  * multiples of ri, complex exponentials reproduced exactly); the frequency
  * correlation surface against a direct (brute-force) cross-correlation;

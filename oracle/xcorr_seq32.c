@@ -9,11 +9,16 @@
  *                      table (xo_twiddle64 rounded to float)
  *   amplitude          sqrtf(fma(re, re, im*im))
  *   patch mean         exact fixed-point sum of amplitudes (2^-20 units)
- *   |c3|               (float)sqrt((double)re*re + (double)im*im)
- *   row spectra        z = amp1 + i amp2 mask transformed along x; rows
- *                      outside the mask (real input) keep bins 0..NX/2 and
- *                      complete the rest as X[k] = conj(X[NX - k]); the means
+ *   row spectra        every row transforms z = amp1 + i amp2 (amp2 on the
+ *                      template columns only); rows outside the mask take the
+ *                      master's spectrum out of it, X[k] = ((Z[k].re +
+ *                      Z[-k].re) / 2, (Z[k].im - Z[-k].im) / 2); the means
  *                      come off in this spectrum
+ *   |c3|               the kept lag rows iy and iy + nyc/2 share one inverse
+ *                      row transform as ra + i rb: |Re| * 1/N and |Im| * 1/N
+ *   high-res           direct 8-term sums per 1-D pass; the column pass pairs
+ *                      the bins k and 8 - k and the rows s and L - s
+ *                      (xo32_interp2d)
  *
  * Compile with -ffp-contract=off so that only the explicit fmaf calls fuse.
  */
