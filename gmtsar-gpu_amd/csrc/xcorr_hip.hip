@@ -1066,7 +1066,8 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
                                                          float2* __restrict__ D, const float2* __restrict__ tabs,
                                                          int nwork) {
   __shared__ float2 T[256 * kT256Pitch];
-  __shared__ float2 sx[256];                                 // spectrum of a self-mirrored column
+  __shared__ float2 sx[256];                                 // spectrum of column 0 (self-mirrored)
+  __shared__ float2 sxm[256];                                // spectrum of column npx/2 (p == 0)
   __shared__ float2 twl[240];                                // pass-2 twiddles w256^(r j), twl[15 j + r - 1]
   const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
   const int NX = P.NX;
@@ -1146,7 +1147,10 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
   for (int r = 0; r < 16; r++) T[(j + 16 * r) * kT256Pitch + g] = vM[r];
   if (self)
 #pragma unroll
-    for (int r = 0; r < 16; r++) sx[j + 16 * r] = vT[r];
+    for (int r = 0; r < 16; r++) {
+      sx[j + 16 * r] = vT[r];
+      sxm[j + 16 * r] = vM[r];
+    }
   __syncthreads();
   float2 pT[16];
 #pragma unroll
@@ -1156,39 +1160,34 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
     const float2 zm = self ? sx[my] : T[my * kT256Pitch + g];
     pT[r] = c3_bin(vT[r], zm, kxT, ky);
   }
-  // Only columns 0..npx/2 are needed downstream (the rest are conjugate
-  // mirrors); of the M columns that is npx/2 alone, held by group 0 of block 0.
-  if (p == 0) {
-    __syncthreads();
-    if (self)
-#pragma unroll
-      for (int r = 0; r < 16; r++) sx[j + 16 * r] = vM[r];
-    __syncthreads();
-    if (self)
-#pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const int ky = j + 16 * r;
-        vM[r] = c3_bin(vM[r], sx[(256 - ky) & 255], kxM, ky);
-      }
-  }
   __syncthreads();
   fft256_group_lds<+1>(pT, j, T + g * kT256Tile, twl);
-  // the M column npx/2 alone (group 0 of block 0): the other lanes of its wave
-  // stay masked and waves 1..3 skip the transform
-  if (self) fft256_group_lds<+1>(vM, j, T + g * kT256Tile, twl);
   __syncthreads();                                           // every group is done with its tile
   // lag rows 64..191 (r = 4..11)
 #pragma unroll
   for (int q = 0; q < 8; q++) T[(j + 16 * q) * kT256Pitch + g] = pT[q + 4];
-  if (self)
-#pragma unroll
-    for (int q = 0; q < 8; q++) sx[j + 16 * q] = vM[q + 4];
   __syncthreads();
   float2* db = D + (size_t)b * 128 * NX;
   for (int e = threadIdx.x; e < 128 * 16; e += kThreads)
     db[(size_t)(e >> 4) * NX + 16 * p + (e & 15)] = T[(e >> 4) * kT256Pitch + (e & 15)];
-  if (p == 0)
-    for (int e = threadIdx.x; e < 128; e += kThreads) db[(size_t)e * NX + NX / 2] = sx[e];
+  // Only columns 0..npx/2 are needed downstream (the rest are conjugate
+  // mirrors); of the M columns that is npx/2 alone, held by group 0 of the
+  // patch's first tile.  Its product and inverse transform come last, once
+  // every wave has stored its rows, so waves 1..3 finish instead of waiting.
+  if (p == 0) {
+    __syncthreads();                                         // T is free for the group's tile
+    if (self) {
+      float2 u[16];
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int ky = j + 16 * r;
+        u[r] = c3_bin(sxm[ky], sxm[(256 - ky) & 255], kxM, ky);
+      }
+      fft256_group_lds<+1>(u, j, T + g * kT256Tile, twl);
+#pragma unroll
+      for (int q = 0; q < 8; q++) db[(size_t)(j + 16 * q) * NX + NX / 2] = u[q + 4];
+    }
+  }
 }
 
 // The 8 x 8 neighbourhood of the surface peak that k_finish reads (the flat
