@@ -4,8 +4,9 @@
   pointer mode.
 * One location per batch (max_batch 1) on the strip and per-patch paths.
 * The largest window the parameters allow (xsearch = ysearch = 256: 1024 x
-  1024 windows, 512 x 512 templates) and the largest range-interpolation
-  factors (ri 8 and 16).
+  1024 windows, 512 x 512 templates), the largest range-interpolation
+  factors (ri 8 and 16) and the smallest high-res factor (1: an 8 x 8 grid,
+  the paired column sums with rows 0..4).
 * All-zero images: every window is zero, the surface has no positive value,
   and the result is the defined one (peak 0, corr 0), as in the restatement.
 """
@@ -107,6 +108,17 @@ def test_largest_range_interpolation(xg, orc, pair, ri):
         eng.set_images(m, s)
         got = eng.correlate(w)
     _same(got, orc.run(_oracle_params(orc, p), m, s, _locs(w), precision=32), "ri %d" % ri)
+
+
+@pytest.mark.parametrize("xs,corr", [(64, 2), (16, 2), (32, 0)])
+def test_high_res_factor_one(xg, orc, pair, xs, corr):
+    m, s = pair
+    p = xg.make_params(xs, xs, interp_factor=1, corr_flag=corr)
+    w = _windows(xg, 12, p.npx, p.npy, seed=70 + xs)
+    with xg.Engine(p, max_batch=8) as eng:
+        eng.set_images(m, s)
+        got = eng.correlate(w)
+    _same(got, orc.run(_oracle_params(orc, p), m, s, _locs(w), precision=32), "interp 1, xs %d" % xs)
 
 
 @pytest.mark.parametrize("xs,corr", [(64, 2), (32, 2), (16, 2), (32, 0)])
