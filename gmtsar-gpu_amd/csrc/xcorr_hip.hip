@@ -1039,6 +1039,20 @@ __device__ __forceinline__ float2 c3_bin(float2 a, float2 zm, int kx, int ky) {
   return make_float2(p.x * f, p.y * f);
 }
 
+// c3_bin without its final factor (-1)^(kx+ky) / 4 (the RAW spectra of the
+// strip column pass).  Both move into the inverse transforms exactly: a sign
+// (-1)^k on the inputs of the 16 x 16 transforms is a shift of their outputs by
+// N/2 (every butterfly sees the negated values of the shifted ones, and IEEE
+// negation and round-to-nearest commute), so the kept lag rows and columns are
+// read N/2 further on, and 1/4 joins the surface scale 1/N (a power of two,
+// which commutes with every rounding away from the subnormal range).
+__device__ __forceinline__ float2 c3_raw(float2 a, float2 zm) {
+  const float2 bb = make_float2(zm.x, -zm.y);
+  const float2 c1 = make_float2(a.x + bb.x, a.y + bb.y);
+  const float2 d = csub(a, bb);
+  return cmul(c1, make_float2(d.y, d.x));
+}
+
 // Demean in the row spectrum: the row transform of z = amp1 + i amp2 mask minus
 // that of m1 + i m2 mask, i.e. Zr - m1 npx delta(kx) - i m2 [y in mask] Mx[kx].
 __device__ __forceinline__ float2 demean_row_bin(float2 v, int kx, bool ym, float m1NX, float m2f, float2 mx) {
@@ -1158,14 +1172,15 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
     const int ky = j + 16 * r;
     const int my = (256 - ky) & 255;
     const float2 zm = self ? sx[my] : T[my * kT256Pitch + g];
-    pT[r] = c3_bin(vT[r], zm, kxT, ky);
+    pT[r] = STRIPS ? c3_raw(vT[r], zm) : c3_bin(vT[r], zm, kxT, ky);
   }
   __syncthreads();
   fft256_group_lds<+1>(pT, j, T + g * kT256Tile, twl);
   __syncthreads();                                           // every group is done with its tile
-  // lag rows 64..191 (r = 4..11)
+  // lag rows 64..191: r = 4..11, or r = 12..15, 0..3 for the RAW spectra
+  constexpr int R0 = STRIPS ? 12 : 4;
 #pragma unroll
-  for (int q = 0; q < 8; q++) T[(j + 16 * q) * kT256Pitch + g] = pT[q + 4];
+  for (int q = 0; q < 8; q++) T[(j + 16 * q) * kT256Pitch + g] = pT[(q + R0) & 15];
   __syncthreads();
   float2* db = D + (size_t)b * 128 * NX;
   for (int e = threadIdx.x; e < 128 * 16; e += kThreads)
@@ -1181,11 +1196,11 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
 #pragma unroll
       for (int r = 0; r < 16; r++) {
         const int ky = j + 16 * r;
-        u[r] = c3_bin(sxm[ky], sxm[(256 - ky) & 255], kxM, ky);
+        u[r] = STRIPS ? c3_raw(sxm[ky], sxm[(256 - ky) & 255]) : c3_bin(sxm[ky], sxm[(256 - ky) & 255], kxM, ky);
       }
       fft256_group_lds<+1>(u, j, T + g * kT256Tile, twl);
 #pragma unroll
-      for (int q = 0; q < 8; q++) db[(size_t)(j + 16 * q) * NX + NX / 2] = u[q + 4];
+      for (int q = 0; q < 8; q++) db[(size_t)(j + 16 * q) * NX + NX / 2] = u[(q + R0) & 15];
     }
   }
 }
@@ -1231,6 +1246,7 @@ __device__ inline NbrWindow nbr_window(unsigned long long best, int nxc, int nyc
 // the result for row r and |Im| for row r + nyc/2.  With keep the surface is
 // written whole; otherwise only the 8 x 8 neighbourhood of the peak that
 // k_finish reads goes out (surface_nbr).
+template <bool RAW>
 __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const float2* __restrict__ D,
                                                               float* __restrict__ surf,
                                                               unsigned long long* __restrict__ peak,
@@ -1247,7 +1263,10 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
   float2* tile = tiles + g * kT256Tile;
   for (int k = threadIdx.x; k < 240; k += kThreads) twl[k] = tabs[P.tw_nx + (k % 15 + 1) * (k / 15)];
   __syncthreads();
-  const float sc = 1.0f / (float)P.N;
+  // RAW lag rows (k_colpass256 on strips): the factor 1/4 joins the scale and
+  // the kept columns 64..191 are outputs r = 12..15, 0..3 instead of 4..11
+  const float sc = (RAW ? 0.25f : 1.0f) / (float)P.N;
+  constexpr int R0 = RAW ? 12 : 4;
   const int hp = P.nyc >> 1;
   unsigned long long best = 0ull;
   double sum = 0.0;
@@ -1281,7 +1300,7 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
     if (!active) continue;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      const float ca = fabsf(v[q + 4].x * sc), cb = fabsf(v[q + 4].y * sc);
+      const float ca = fabsf(v[(q + R0) & 15].x * sc), cb = fabsf(v[(q + R0) & 15].y * sc);
       const unsigned ia = (unsigned)(row * 128 + j + 16 * q), ib = ia + (unsigned)hp * 128;
       if (keep) {
         sb[ia] = ca;
@@ -1325,7 +1344,8 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
   fft256_group_lds<+1>(v, j, tile, twl);
   if (nw.y_lo + g <= nw.y_hi)
 #pragma unroll
-    for (int q = 0; q < 8; q++) nw.put(sb, y, j + 16 * q, fabsf((y < hp ? v[q + 4].x : v[q + 4].y) * sc));
+    for (int q = 0; q < 8; q++)
+      nw.put(sb, y, j + 16 * q, fabsf((y < hp ? v[(q + R0) & 15].x : v[(q + R0) & 15].y) * sc));
 }
 
 // Stage (b) columns for npx = npy = N in {128, 512}: FFT along y, C3 = C1
@@ -3439,7 +3459,7 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   if (rc) return rc;
   rc = launch_timed(c, timing, d.NX == 256 ? "k_rowifft_surf256" : "k_rowifft_surf_reg", [&] {
     if (d.NX == 256)
-      hipLaunchKernelGGL(k_rowifft_surf256, dim3(nb), dim3(kThreads), 0, st, d, (const float2*)c->D, c->surf,
+      hipLaunchKernelGGL(k_rowifft_surf256<true>, dim3(nb), dim3(kThreads), 0, st, d, (const float2*)c->D, c->surf,
                          c->peak, c->partial, (const float2*)c->tabs, d.keep_surf);
     else if (d.NX == 128)
       hipLaunchKernelGGL(k_rowifft_surf_reg<128>, dim3(nb), dim3(kThreads), 0, st, d, (const float2*)c->D, c->surf,
@@ -3682,7 +3702,7 @@ static int run_batch(xcorr_hip_ctx* c, const PatchBufs& b, const xcorr_hip_windo
       });
     else if (row256)
       rc = launch_timed(c, timing, "k_rowifft_surf256", [&] {
-        hipLaunchKernelGGL(k_rowifft_surf256, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)b.D,
+        hipLaunchKernelGGL(k_rowifft_surf256<false>, dim3(nb), dim3(kThreads), 0, c->stream, d, (const float2*)b.D,
                            b.surf, b.peak, b.partial, (const float2*)c->tabs, d.keep_surf);
       });
     else
