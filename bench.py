@@ -79,6 +79,82 @@ def committed_traffic():
     return {}
 
 
+def trace_marker(torch):
+    """A one-cycle marker kernel (torch.cuda._sleep, `spin_kernel` in a trace)
+    launched right before and right after each timed loop, with the device
+    idle on both sides, so a kernel trace of the run shows where the timed
+    steps are."""
+    torch.cuda.synchronize()
+    torch.cuda._sleep(1)
+    torch.cuda.synchronize()
+
+
+def graph_occupancy(path, slots_per_cu=4, cus=256):
+    """Device occupancy over the timed steps in a rocprofv3 kernel trace of a
+    bench.py run: the engine's kernels (`xc::` names) dispatched between the
+    last two trace markers (see trace_marker).  Reported: the span from the
+    first start to the last end, the union of the kernels' busy intervals,
+    the time at each number of concurrently running kernels, and a fill
+    estimate that weights time by min(1, running workgroups / (slots_per_cu x
+    cus)), i.e. how much of it could have held a full device of 256-thread
+    workgroups.  Times are those of the profiled run (the tracer slows each
+    launch and may serialise a graph's branches)."""
+    import csv
+    import collections
+    rows = []
+    marks = []
+    with open(path) as f:
+        for d in csv.DictReader(f):
+            did = int(d["Dispatch_Id"])
+            if "spin_kernel" in d["Kernel_Name"]:
+                marks.append(did)
+                continue
+            if "xc::" not in d["Kernel_Name"]:
+                continue
+            wg = (int(d["Grid_Size_X"]) * int(d["Grid_Size_Y"]) * int(d["Grid_Size_Z"]) //
+                  max(1, int(d["Workgroup_Size_X"]) * int(d["Workgroup_Size_Y"]) * int(d["Workgroup_Size_Z"])))
+            name = d["Kernel_Name"].replace("void ", "").split("(")[0]
+            rows.append((did, int(d["Start_Timestamp"]), int(d["End_Timestamp"]), name, wg))
+    marks.sort()
+    if len(marks) < 2:
+        return {"error": "fewer than two trace markers in %s" % path}
+    lo, hi = marks[-2], marks[-1]
+    rep = sorted((r[1], r[2], r[3], r[4]) for r in rows if lo < r[0] < hi)
+    if not rep:
+        return {"error": "no engine kernels between the last two markers in %s" % path}
+    t0, t1 = rep[0][0], max(r[1] for r in rep)
+    busy, cs, ce = 0, rep[0][0], rep[0][1]
+    for s, e, _, _ in rep[1:]:
+        if s > ce:
+            busy += ce - cs
+            cs, ce = s, e
+        else:
+            ce = max(ce, e)
+    busy += ce - cs
+    events = sorted([(r[0], 1, r[3]) for r in rep] + [(r[1], -1, -r[3]) for r in rep])
+    conc = collections.Counter()
+    running = wg = 0
+    last = t0
+    fill = 0.0
+    for t, dk, dw in events:
+        conc[running] += t - last
+        fill += min(1.0, wg / float(slots_per_cu * cus)) * (t - last)
+        last = t
+        running += dk
+        wg += dw
+    per = collections.defaultdict(lambda: [0, 0])
+    for s, e, name, _ in rep:
+        per[name][0] += 1
+        per[name][1] += e - s
+    span = t1 - t0
+    return {"trace": os.path.basename(path), "kernels": len(rep), "span_ms": round(span / 1e6, 3),
+            "busy_ms": round(busy / 1e6, 3), "busy_frac": round(busy / span, 4),
+            "fill_frac": round(fill / span, 4),
+            "ms_at_concurrency": {str(k): round(v / 1e6, 3) for k, v in sorted(conc.items())},
+            "kernel_ms": {k: {"launches": v[0], "ms": round(v[1] / 1e6, 3)}
+                          for k, v in sorted(per.items(), key=lambda kv: -kv[1][1])}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,7 +190,13 @@ def main():
                          "(strided 8-byte reads, each byte read once) into its transpose")
     ap.add_argument("--traffic", default="", metavar="JSON",
                     help="per-launch HBM bytes of the dominant kernel from rocprofv3 FETCH_SIZE/WRITE_SIZE runs")
+    ap.add_argument("--graph-occupancy", default="", metavar="CSV",
+                    help="only summarise a rocprofv3 kernel trace (kernel_trace.csv) of --workload batched or "
+                         "dense: how busy the device is during the last graph replay")
     args = ap.parse_args()
+    if args.graph_occupancy:
+        print(json.dumps(graph_occupancy(args.graph_occupancy)), flush=True)
+        return 0
     if args.copy_probe:
         import torch
         nbytes = args.copy_probe << 20
@@ -183,6 +265,7 @@ def main():
     # HIP events on the engine's stream between the steps give the per-step spread
     est = torch.cuda.ExternalStream(eng.stream())
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    trace_marker(torch)
     t1 = time.perf_counter()
     evs[0].record(est)
     for k in range(args.steps):
@@ -191,6 +274,7 @@ def main():
     eng.sync()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    trace_marker(torch)
     step_s = (t2 - t1) / args.steps
     value = n / step_s
     per_step = sorted(evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps))
@@ -403,13 +487,14 @@ def bench_batched(args, xg, synth, torch, dev):
     for _ in range(args.warmup):
         mu.run(flags=xg.NO_SYNC)
     mu.sync()
-    torch.cuda.synchronize()
+    trace_marker(torch)
     t1 = time.perf_counter()
     for _ in range(args.steps):
         mu.run(flags=xg.NO_SYNC)
     mu.sync()
     torch.cuda.synchronize()
     step_s = (time.perf_counter() - t1) / args.steps
+    trace_marker(torch)
     value = n / step_s
     log("timed %d steps: %.3f ms/step, %.0f patches/s" % (args.steps, step_s * 1e3, value))
     nbytes = sum(len(q["windows"]) * (2 * q["params"].npx * q["params"].npy * 4 + 20) for q in work)
@@ -573,13 +658,14 @@ def bench_dense(args, xg, synth, torch, dev):
     for _ in range(args.warmup):
         mu.run(flags=xg.NO_SYNC)
     mu.sync()
-    torch.cuda.synchronize()
+    trace_marker(torch)
     t1 = time.perf_counter()
     for _ in range(args.steps):
         mu.run(flags=xg.NO_SYNC)
     mu.sync()
     torch.cuda.synchronize()
     step_s = (time.perf_counter() - t1) / args.steps
+    trace_marker(torch)
     value = n / step_s
     log("timed %d steps: %.3f ms/step, %.0f patches/s" % (args.steps, step_s * 1e3, value))
     # per-class time from one single-class graph each
