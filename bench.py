@@ -277,9 +277,11 @@ def main():
     trace_marker(torch)
     step_s = (t2 - t1) / args.steps
     value = n / step_s
-    per_step = sorted(evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps))
+    in_order = [evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps)]
+    per_step = sorted(in_order)
     step_stats = {"median_ms": round(per_step[len(per_step) // 2], 4), "min_ms": round(per_step[0], 4),
-                  "max_ms": round(per_step[-1], 4), "source": "HIP events on the engine stream"}
+                  "max_ms": round(per_step[-1], 4), "ms": [round(v, 3) for v in in_order],
+                  "source": "HIP events on the engine stream"}
     log("timed %d steps: %.3f ms/step, %.0f patches/s (median step %.3f ms)"
         % (args.steps, step_s * 1e3, value, step_stats["median_ms"]))
 
