@@ -1199,8 +1199,14 @@ __global__ void __launch_bounds__(kThreads) k_colpass256(DevParams P, const floa
         u[r] = STRIPS ? c3_raw(sxm[ky], sxm[(256 - ky) & 255]) : c3_bin(sxm[ky], sxm[(256 - ky) & 255], kxM, ky);
       }
       fft256_group_lds<+1>(u, j, T + g * kT256Tile, twl);
+      // strips: lag row y's value of column npx/2 is kept in row 0 at column
+      // npx/2 + y (unused there), so the row pass reads every lag row as its
+      // first npx/2 columns, whole cache lines, and these stores fill lines
 #pragma unroll
-      for (int q = 0; q < 8; q++) db[(size_t)(j + 16 * q) * NX + NX / 2] = u[(q + R0) & 15];
+      for (int q = 0; q < 8; q++) {
+        const int y = j + 16 * q;
+        db[STRIPS ? (size_t)(NX / 2 + y) : (size_t)y * NX + NX / 2] = u[(q + R0) & 15];
+      }
     }
   }
 }
@@ -1276,15 +1282,18 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
     const bool active = row < hp;
     // branch-free loads (an inactive group reads a valid row and discards
     // it), so all 32 are in flight together
-    const float2* ra = db + (size_t)min(row, hp - 1) * 256;
+    const int rowa = min(row, hp - 1);
+    const float2* ra = db + (size_t)rowa * 256;
     const float2* rc = ra + (size_t)hp * 256;
     float2 a[16], c[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
       const int x = j + 16 * r;
       const int xm = x <= 128 ? x : 256 - x;
-      a[r] = ra[xm];
-      c[r] = rc[xm];
+      // RAW (strip) lag rows keep column 128 in row 0 at column 128 + row
+      const bool nyq = RAW && r == 8 && j == 0;
+      a[r] = *(nyq ? db + 128 + rowa : ra + xm);
+      c[r] = *(nyq ? db + 128 + rowa + hp : rc + xm);
     }
     float2 v[16];
 #pragma unroll
@@ -1334,7 +1343,8 @@ __global__ void __launch_bounds__(kThreads) k_rowifft_surf256(DevParams P, const
   for (int r = 0; r < 16; r++) {
     const int x = j + 16 * r;
     const int xm = x <= 128 ? x : 256 - x;
-    float2 ar = ra[xm], cr = rc[xm];
+    const bool nyq = RAW && r == 8 && j == 0;
+    float2 ar = *(nyq ? db + 128 + pr : ra + xm), cr = *(nyq ? db + 128 + pr + hp : rc + xm);
     if (x > 128) {
       ar.y = -ar.y;
       cr.y = -cr.y;
