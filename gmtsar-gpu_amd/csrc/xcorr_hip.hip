@@ -2421,6 +2421,82 @@ __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed
   if (r < r_end) mma(r, A0, B0);
 }
 
+// The window integers (int)(a - m1) of a 128 x 128 block (rows pa floats
+// apart from a) and the template integers (int)(t - m2) of a 64 x 64 block
+// (rows pb floats apart from t), as three 7-bit limbs each, into the LDS
+// layouts i8_lag_tiles reads: Al, 3 x 128 limb rows of kLimbPitch bytes (the
+// last 16 zero); Bl, 3 x kBRows rows of 64 bytes with kBPad zero rows on either
+// side of the 64 template rows.  big: a value outside the limb range (|v| >=
+// 2^20); hia / hib: limb 2 nonzero somewhere in the window / the template;
+// b2 += the template's sum of squares.  Four columns per thread and step: the
+// float4 loads of the window (16 per thread, in NH groups) and of the template
+// (4, with the first group) are issued before their first use; each limb row
+// takes one dword per 4 columns.  NH = 2 halves the registers the loads hold,
+// for callers that keep accumulators live across blocks.
+template <int NH = 1>
+__device__ __forceinline__ void pack_i8_limbs(const float* __restrict__ a1, int pa, const float* __restrict__ t,
+                                              int pb, float m1f, float m2f, signed char* Al, signed char* Bl,
+                                              int& big, int& hia, int& hib, long long& b2) {
+  static_assert(kThreads == 256, "16 window and 4 template float4 per thread");
+  auto pack4 = [&](const float4 f, float mf, unsigned (&w)[3], int& hi, long long* e2) {
+    const int v[4] = {(int)(f.x - mf), (int)(f.y - mf), (int)(f.z - mf), (int)(f.w - mf)};
+    w[0] = w[1] = w[2] = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      big |= (v[q] >= (1 << 20) || v[q] <= -(1 << 20));
+      if (e2) *e2 += (long long)v[q] * v[q];
+      signed char l[3];
+      limbs7(v[q], l);
+      hi |= l[2];
+#pragma unroll
+      for (int k = 0; k < 3; k++) w[k] |= (unsigned)(unsigned char)l[k] << (8 * q);
+    }
+  };
+  static_assert(NH == 1 || NH == 2, "window loads in one or two groups");
+  constexpr int KW = 16 / NH;
+  float4 fb[4];
+#pragma unroll
+  for (int hh = 0; hh < NH; hh++) {
+    float4 fa[KW];
+#pragma unroll
+    for (int k = 0; k < KW; k++) {
+      const int idx = threadIdx.x + kThreads * (hh * KW + k);   // row idx / 32, columns 4 (idx % 32) ..
+      fa[k] = ((const float4*)(a1 + (size_t)(idx >> 5) * pa))[idx & 31];
+    }
+    if (hh == 0)
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int idx = threadIdx.x + kThreads * k;   // template row idx / 16, columns 4 (idx % 16) ..
+        fb[k] = ((const float4*)(t + (size_t)(idx >> 4) * pb))[idx & 15];
+      }
+#pragma unroll
+    for (int k = 0; k < KW; k++) {
+      const int idx = threadIdx.x + kThreads * (hh * KW + k), r = idx >> 5, x = 4 * (idx & 31);
+      unsigned w[3];
+      pack4(fa[k], m1f, w, hia, nullptr);
+#pragma unroll
+      for (int l = 0; l < 3; l++) *(unsigned*)(Al + (l * 128 + r) * kLimbPitch + x) = w[l];
+    }
+  }
+  for (int e = threadIdx.x; e < 3 * 128 * 4; e += kThreads)      // columns 128 .. 143 of every limb row
+    *(unsigned*)(Al + (e >> 2) * kLimbPitch + 128 + 4 * (e & 3)) = 0u;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int idx = threadIdx.x + kThreads * k, y = idx >> 4, x = 4 * (idx & 15);
+    unsigned w[3];
+    pack4(fb[k], m2f, w, hib, &b2);
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      const int row = l * kBRows + kBPad + y;
+      *(unsigned*)(Bl + row * 64 + bl_chunk(row, x >> 4) + (x & 15)) = w[l];
+    }
+  }
+  for (int e = threadIdx.x; e < 3 * 64 * 4; e += kThreads) {       // the zero rows, 16 bytes a store
+    const int l = e / 256, q = e % 256, row = q < 128 ? q >> 2 : kBPad + 64 + ((q - 128) >> 2);
+    *(uint4*)(Bl + (l * kBRows + row) * 64 + 16 * (q & 3)) = make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+
 __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, const float* __restrict__ A1,
                                                              const float* __restrict__ A2,
                                                              const int* __restrict__ rb,
@@ -2446,61 +2522,8 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
   }
   __syncthreads();
   int big = 0, hia = 0, hib = 0;
-  // window integers and their limbs, four columns per thread and step: all
-  // float4 loads of the window (16 per thread) and of the template (4) are
-  // issued before the first use; each limb row takes one dword per 4 columns
-  static_assert(kThreads == 256, "16 window and 4 template float4 per thread");
-  auto pack4 = [&](const float4 f, float mf, unsigned (&w)[3], int& hi, long long* e2) {
-    const int v[4] = {(int)(f.x - mf), (int)(f.y - mf), (int)(f.z - mf), (int)(f.w - mf)};
-    w[0] = w[1] = w[2] = 0u;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      big |= (v[q] >= (1 << 20) || v[q] <= -(1 << 20));
-      if (e2) *e2 += (long long)v[q] * v[q];
-      signed char l[3];
-      limbs7(v[q], l);
-      hi |= l[2];
-#pragma unroll
-      for (int k = 0; k < 3; k++) w[k] |= (unsigned)(unsigned char)l[k] << (8 * q);
-    }
-  };
-  float4 fa[16], fb[4];
-#pragma unroll
-  for (int k = 0; k < 16; k++) {
-    const int idx = threadIdx.x + kThreads * k;   // row idx / 32, columns 4 (idx % 32) ..
-    fa[k] = ((const float4*)(a1 + (idx >> 5) * 128))[idx & 31];
-  }
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const int idx = threadIdx.x + kThreads * k;   // template row idx / 16, columns 4 (idx % 16) ..
-    fb[k] = ((const float4*)(a2 + (size_t)(P.ys + (idx >> 4)) * 128 + P.xs))[idx & 15];
-  }
-#pragma unroll
-  for (int k = 0; k < 16; k++) {
-    const int idx = threadIdx.x + kThreads * k, r = idx >> 5, x = 4 * (idx & 31);
-    unsigned w[3];
-    pack4(fa[k], m1f, w, hia, nullptr);
-#pragma unroll
-    for (int l = 0; l < 3; l++) *(unsigned*)(Al + (l * 128 + r) * kLimbPitch + x) = w[l];
-  }
-  for (int e = threadIdx.x; e < 3 * 128 * 4; e += kThreads)      // columns 128 .. 143 of every limb row
-    *(unsigned*)(Al + (e >> 2) * kLimbPitch + 128 + 4 * (e & 3)) = 0u;
   long long b2 = 0;
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const int idx = threadIdx.x + kThreads * k, y = idx >> 4, x = 4 * (idx & 15);
-    unsigned w[3];
-    pack4(fb[k], m2f, w, hib, &b2);
-#pragma unroll
-    for (int l = 0; l < 3; l++) {
-      const int row = l * kBRows + kBPad + y;
-      *(unsigned*)(Bl + row * 64 + bl_chunk(row, x >> 4) + (x & 15)) = w[l];
-    }
-  }
-  for (int e = threadIdx.x; e < 3 * 64 * 4; e += kThreads) {       // the zero rows, 16 bytes a store
-    const int l = e / 256, q = e % 256, row = q < 128 ? q >> 2 : kBPad + 64 + ((q - 128) >> 2);
-    *(uint4*)(Bl + (l * kBRows + row) * 64 + 16 * (q & 3)) = make_uint4(0u, 0u, 0u, 0u);
-  }
+  pack_i8_limbs(a1, 128, a2 + (size_t)P.ys * 128 + P.xs, 128, m1f, m2f, Al, Bl, big, hia, hib, b2);
   if (big) atomicOr(&sbig, 1);
   if (hia) atomicOr(&shi[0], 1);
   if (hib) atomicOr(&shi[1], 1);
@@ -2607,6 +2630,161 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
     tpeak[4 * b + 3] = 0;
     tcave[b] = ssum / 4096.0;
   }
+}
+
+// Exact master energies sum_{y, x < 128} A[y + dy][x + dx]^2 of the 128 x 128
+// lags of a 256 x 256 window (xsearch = ysearch = 64), as doubles (the value
+// 1.0 * aa takes in the NCC).  One block per patch.  Row energies
+// E[r][dx] = sum_{x < 128} A[r][x + dx]^2 of 32 rows at a time go to LDS (lane
+// l of a wave holds A[r][64 s + l], s < 4: E[r][l] = suffix of segment 0 from l
+// + segment 1 + prefix of segment 2 below l, E[r][64 + l] likewise from segment
+// 1), then threads 0..127 run down one lag column each: with S(r) the sum of
+// the first r row energies, S(r) for r < 128 is parked in the output and
+// aa[dy] = S(dy + 128) - S(dy) overwrites it (int64 throughout, exact).
+__global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const float* __restrict__ amp,
+                                                             const unsigned long long* __restrict__ sums,
+                                                             double* __restrict__ aa, int nb) {
+  __shared__ long long E[32 * 128];
+  const int b = xcd_item();
+  if (b >= nb) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float m1f = (float)patch_mean(sums[2 * b], P.N);
+  const float* a1 = amp + (size_t)(2 * b) * P.N;
+  double* ob = aa + (size_t)b * 128 * 128;
+  long long* sb = (long long*)ob;
+  const int col = threadIdx.x & 127;
+  long long S = 0;
+  for (int ch = 0; ch < 8; ch++) {
+    float f[8][4];                                // rows 32 ch + wave + 4 k: all loads in flight together
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+#pragma unroll
+      for (int sg = 0; sg < 4; sg++) f[k][sg] = a1[(32 * ch + wave + 4 * k) * 256 + 64 * sg + lane];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      long long q[4], pf[4], tt[4];
+#pragma unroll
+      for (int sg = 0; sg < 4; sg++) {
+        const long long v = (int)(f[k][sg] - m1f);
+        q[sg] = v * v;
+        pf[sg] = wave_incl_scan_i64(q[sg]);
+        tt[sg] = wave_lane63_i64(pf[sg]);
+      }
+      const int rr = wave + 4 * k;
+      E[rr * 128 + lane] = (tt[0] - pf[0] + q[0]) + tt[1] + (pf[2] - q[2]);
+      E[rr * 128 + 64 + lane] = (tt[1] - pf[1] + q[1]) + tt[2] + (pf[3] - q[3]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 128)
+      for (int rr = 0; rr < 32; rr++) {
+        const int r = 32 * ch + rr;               // S holds S(r)
+        if (r < 128) {
+          sb[r * 128 + col] = S;
+        } else {
+          const long long s0 = sb[(r - 128) * 128 + col];
+          ob[(r - 128) * 128 + col] = (double)(S - s0);
+        }
+        S += E[rr * 128 + col];
+      }
+    __syncthreads();
+  }
+}
+
+// -time search on the int8 matrix cores for xsearch = ysearch = 64 (256 x 256
+// windows, 128 x 128 template, 128 x 128 lags).  Block (patch b, quadrant q)
+// computes the 64 x 64 lags dy = dyq + 0..63, dx = dxq + 0..63, and
+// num(dy, dx) = sum over the template blocks (h, k) < 2 of a sub-search of
+// k_time_search_i8's shape: the 128 x 128 window block at rows dyq + 64 h,
+// columns dxq + 64 k against the 64 x 64 template block at (64 h, 64 k).  The
+// four sub-searches add into the same int32 accumulators, each of which sums
+// at most 3 x 4 x 4096 limb products of magnitude <= 2^12: still exact.  The
+// surface goes to surfd for k_time_peak; a patch with a value outside the limb
+// range takes the exact int64 loop.
+__global__ void __launch_bounds__(kThreads, 2) k_time_search_i8q(DevParams P, const float* __restrict__ amp,
+                                                              const unsigned long long* __restrict__ sums,
+                                                              const double* __restrict__ aa,
+                                                              double* __restrict__ surfd, int nb) {
+  __shared__ __attribute__((aligned(16))) signed char Al[3 * 128 * kLimbPitch];
+  __shared__ __attribute__((aligned(16))) signed char Bl[3 * kBRows * 64];
+  __shared__ long long redl[kThreads / 64];
+  __shared__ int sbig[4], shi[4][2];              // per template block: values beyond the limbs; limb 2 nonzero
+  const int item = xcd_item();                    // the four quadrants of a patch are consecutive items
+  if (item >= 4 * nb) return;
+  const int b = item >> 2, q = item & 3;
+  const int dyq = 64 * (q >> 1), dxq = 64 * (q & 1);
+  const float m1f = (float)patch_mean(sums[2 * b], P.N);
+  const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
+  const float* a1 = amp + (size_t)(2 * b) * P.N;
+  const float* a2 = a1 + P.N;
+  if (threadIdx.x < 4) {
+    sbig[threadIdx.x] = 0;
+    shi[threadIdx.x][0] = 0;
+    shi[threadIdx.x][1] = 0;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+  const int dx0 = __builtin_amdgcn_readfirstlane(32 * (wave & 1));
+  const int dy0 = __builtin_amdgcn_readfirstlane(32 * (wave >> 1));
+  v4i_t acc[2][2][5];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int w = 0; w < 5; w++) acc[i][j][w] = v4i_t{0, 0, 0, 0};
+  long long b2 = 0;
+  bool anybig = false;
+  for (int ph = 0; ph < 4; ph++) {
+    const int h = ph >> 1, k = ph & 1;
+    __syncthreads();                              // the flags are cleared; the last block's limbs are read
+    int big = 0, hia = 0, hib = 0;
+    pack_i8_limbs<2>(a1 + (size_t)(dyq + 64 * h) * 256 + dxq + 64 * k, 256,
+                  a2 + (size_t)(P.ys + 64 * h) * 256 + P.xs + 64 * k, 256, m1f, m2f, Al, Bl, big, hia, hib, b2);
+    if (big) atomicOr(&sbig[ph], 1);
+    if (hia) atomicOr(&shi[ph][0], 1);
+    if (hib) atomicOr(&shi[ph][1], 1);
+    __syncthreads();                              // the limbs and flags are published
+    anybig = anybig || sbig[ph];
+    if (anybig) continue;
+    // limb 2 is zero throughout a block whose values all lie in [-8256, 8127]
+    if (shi[ph][0] && shi[ph][1]) i8_lag_tiles<3, 3>(Al, Bl, dx0, dy0, m, g, acc);
+    else if (shi[ph][0]) i8_lag_tiles<3, 2>(Al, Bl, dx0, dy0, m, g, acc);
+    else if (shi[ph][1]) i8_lag_tiles<2, 3>(Al, Bl, dx0, dy0, m, g, acc);
+    else i8_lag_tiles<2, 2>(Al, Bl, dx0, dy0, m, g, acc);
+  }
+  const double d2d = (double)block_sum_i64(b2, redl);
+  double* sd = surfd + (size_t)b * 128 * 128;
+  const double* ab_aa = aa + (size_t)b * 128 * 128;
+  if (anybig) {                                   // exact int64 loop (values beyond the limb range)
+    for (int l = threadIdx.x; l < 64 * 64; l += kThreads) {
+      const int dy = dyq + (l >> 6), dx = dxq + (l & 63);
+      long long num = 0;
+      for (int y = 0; y < 128; y++)
+        for (int x = 0; x < 128; x++) {
+          const long long av = (int)(a1[(size_t)(y + dy) * 256 + x + dx] - m1f);
+          const long long bv = (int)(a2[(size_t)(P.ys + y) * 256 + P.xs + x] - m2f);
+          num += av * bv;
+        }
+      const int lag = dy * 128 + dx;
+      const double den = sqrt(1.0 * ab_aa[lag] * d2d);
+      sd[lag] = (den == 0.0) ? 0.0 : 100.0 * fabs((double)num / den);
+    }
+    return;
+  }
+  // lane holds lags dx = dxq + dx0 + 16 i + 4 g + q, dy = dyq + dy0 + 16 j + m
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int qq = 0; qq < 4; qq++) {
+        long long num = 0;
+#pragma unroll
+        for (int w = 0; w < 5; w++) num += (long long)acc[i][j][w][qq] << (7 * w);
+        const int dx = dxq + dx0 + 16 * i + 4 * g + qq, dy = dyq + dy0 + 16 * j + m;
+        const int lag = dy * 128 + dx;
+        const double den = sqrt(1.0 * ab_aa[lag] * d2d);
+        sd[lag] = (den == 0.0) ? 0.0 : 100.0 * fabs((double)num / den);
+      }
 }
 
 // Peak of the time-domain surface with do_time_corr's scan semantics
@@ -2780,7 +2958,7 @@ struct xcorr_hip_ctx {
   int tune_batch = 0;                     // locations per launch (0: max_batch)
   int nt_rows = kThreads;                 // threads of the generic row / column kernels
   bool reg_paths = true;                  // register-resident kernels where the shape has them
-  double* tcaa = nullptr;                 // -time, xsearch = ysearch = 32: master energies of every lag
+  double* tcaa = nullptr;                 // -time, xsearch = ysearch = 32 or 64: master energies of every lag
   // strip batches alternate between this context and a second one (own
   // buffers and stream, created on the first call with more than one batch),
   // so the kernels of one batch overlap those of the next
@@ -2815,7 +2993,7 @@ struct PatchBufs {
   double* partial;              // [B][max(groups4, nyc/G4)] surface partial sums
   int* tpeak;                   // [B][4] time-mode peaks
   double* tcave;                // [B] time-mode surface means
-  double* tcaa;                 // [B][nyc][nxc] -time on 128 x 128 windows: master energies of every lag
+  double* tcaa;                 // [B][nyc][nxc] -time on 128 x 128 and 256 x 256 windows: master energies of every lag
 };
 
 static PatchBufs ctx_bufs(const xcorr_hip_ctx* c) {
@@ -3080,7 +3258,7 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
     alloc((void**)&c->partial, B * std::max(d.groups4, d.nyc / d.G4) * sizeof(double));
   } else {
     alloc((void**)&c->surfd, B * ns * sizeof(double));
-    if (p->corr_flag == XCORR_HIP_TIME && p->xsearch == 32 && p->ysearch == 32)
+    if (p->corr_flag == XCORR_HIP_TIME && (p->xsearch == 32 || p->xsearch == 64) && p->ysearch == p->xsearch)
       alloc((void**)&c->tcaa, B * ns * sizeof(double));
     alloc((void**)&c->tpeak, B * 4 * sizeof(int));
     alloc((void**)&c->tcave, B * sizeof(double));
@@ -3789,9 +3967,17 @@ static int time_search(xcorr_hip_ctx* c, const PatchBufs& b, int nb, bool timing
   const int LRb = std::max(1, std::min(d.nyc, kThreads * TL / d.nxc));
   const int RCb = std::min(8, (5120 - (LRb - 1) * d.NX) / (d.NX + d.nx_corr));   // <= 40 KB of LDS
   const bool i8 = d.corr_flag == XCORR_HIP_TIME && d.xs == 32 && d.ys == 32 && b.tcaa;   // peak found in-kernel
+  const bool i8q = d.corr_flag == XCORR_HIP_TIME && d.xs == 64 && d.ys == 64 && b.tcaa;  // 256 x 256 windows
   if (i8)
     rc = launch_timed(c, timing, "k_time_search", [&] {
       launch_time_i8(b, d, c->stream, (const float*)b.amp, (const float*)b.amp, nullptr, nb);
+    });
+  else if (i8q)
+    rc = launch_timed(c, timing, "k_time_search", [&] {
+      hipLaunchKernelGGL(k_time_energy256, dim3((nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
+                         (const float*)b.amp, (const unsigned long long*)b.sums, b.tcaa, nb);
+      hipLaunchKernelGGL(k_time_search_i8q, dim3((4 * nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
+                         (const float*)b.amp, (const unsigned long long*)b.sums, (const double*)b.tcaa, b.surfd, nb);
     });
   else if (d.corr_flag == XCORR_HIP_TIME && d.nxc % TL == 0 && RCb >= 1)
     rc = launch_timed(c, timing, "k_time_search", [&] {
@@ -4051,7 +4237,7 @@ extern "C" int xcorr_hip_correlate_noalloc(xcorr_hip_ctx* c, const xcorr_hip_win
   const int rs = noalloc_scratch(c, &base);
   if (rs) return rs;
   const DevParams& d = c->dp;
-  const bool i8 = d.corr_flag == XCORR_HIP_TIME && d.xs == 32 && d.ys == 32;
+  const bool i8 = d.corr_flag == XCORR_HIP_TIME && (d.xs == 32 || d.xs == 64) && d.ys == d.xs;   // energies too
   PatchBufs b;
   const size_t one = carve_patch_bufs(d, i8, 1, nullptr, &b);
   if (one > kNoAllocBytes)

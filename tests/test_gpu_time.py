@@ -130,3 +130,64 @@ def test_time_xs32_limb_counts(xg, orc, pair, fm, fs, left):
     den = np.sqrt(d1.astype(np.float64) * float(d2))
     brute = np.where(den == 0, 0.0, 100.0 * np.abs(num / np.where(den == 0, 1.0, den)))
     assert np.array_equal(surf, brute)
+
+
+def _brute_surface(i1, i2, xs):
+    """The -time surface of one window from int64 sums, one lag row at a time
+    (the sums are exact; the quotient in float64 as the NCC forms it)."""
+    from numpy.lib.stride_tricks import sliding_window_view
+    n = 2 * xs
+    b = i2[xs:xs + n, xs:xs + n]
+    d2 = int((b * b).sum())
+    out = np.zeros((n, n))
+    for dy in range(n):
+        win = sliding_window_view(i1[dy:dy + n, :2 * n - 1], (n, n))[0]      # (dx, y, x)
+        num = np.einsum("jkl,kl->j", win, b)
+        d1 = np.einsum("jkl,jkl->j", win, win)
+        den = np.sqrt(d1.astype(np.float64) * float(d2))
+        out[dy] = np.where(den == 0, 0.0, 100.0 * np.abs(num / np.where(den == 0, 1.0, den)))
+    return out
+
+
+@pytest.mark.parametrize("fm,fs,left", [(1, 1, False), (5, 1, False), (1, 5, False), (5, 5, True)],
+                         ids=["small", "master_big", "template_big", "mixed"])
+def test_time_xs64_matches_int64_bruteforce(xg, orc, pair, fm, fs, left):
+    """-xsearch 64 -ysearch 64 -time (the reference's default search size, 256 x
+    256 windows) runs on the int8 matrix cores as four lag quadrants of four
+    sub-searches each: bit-exact against the restatement whatever limbs the
+    values need, and its surface equal to the int64 correlation."""
+    sp = _scaled(pair, fm, fs, left)
+    p, got, ref, surf, (a1, a2, means) = _run(xg, orc, sp, 64, 64, 0, nxl=2, nyl=2)
+    _exact(got, ref)
+    m1f, m2f = np.float32(means[0]), np.float32(means[1])
+    i1 = (a1 - m1f).astype(np.int64)
+    i2 = (a2 - m2f).astype(np.int64)
+    print("max |v| master %d template %d" % (np.abs(i1).max(), np.abs(i2[64:192, 64:192]).max()))
+    assert np.array_equal(surf, _brute_surface(i1, i2, 64))
+
+
+def test_time4_xs64_bit_exact(xg, orc, pair):
+    """-time4 keeps its 128-bit search at the default search size."""
+    _, got, ref, _, _ = _run(xg, orc, pair, 64, 64, 1, nxl=2, nyl=1)
+    _exact(got, ref)
+
+
+@pytest.mark.parametrize("xs", [32, 64])
+def test_time_values_beyond_limbs(xg, orc, pair, xs):
+    """Float (-real) input whose window integers reach past 2^20, beyond three
+    7-bit limbs: the matrix-core searches hand such a patch to their exact
+    int64 loop, and the result stays bit-exact."""
+    m, s = pair
+    mf = np.ascontiguousarray(m[..., 0].astype(np.float32) * np.float32(1500.0))
+    sf = np.ascontiguousarray(s[..., 0].astype(np.float32) * np.float32(1500.0))
+    assert np.abs(mf).max() > 2 * (1 << 20) and np.abs(sf).max() > 2 * (1 << 20)
+    p = xg.make_params(xs, xs, format=xg.FLOAT32, corr_flag=xg.TIME)
+    x, y = xg.get_locations(512, 768, 2, 2, xs, xs)
+    w = xg.make_windows(x, y, p.npx, p.npy)
+    with xg.Engine(p) as eng:
+        eng.set_images(mf, sf)
+        got = eng.correlate(w)
+    op = orc.make_params(xs, xs, corr_flag=0)
+    locs = np.stack([w["mx0"], w["my0"], w["sx0"], w["sy0"]], 1).astype(np.int32)
+    ref = orc.run(op, mf, sf, locs, precision=32, fmt=1)
+    _exact(got, ref)
