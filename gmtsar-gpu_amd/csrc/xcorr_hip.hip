@@ -2835,6 +2835,69 @@ __global__ void __launch_bounds__(kThreads) k_time_peak(DevParams P, const doubl
   }
 }
 
+// k_time_peak for surfaces of a multiple of 4096 lags (-time / -time4 with
+// nxc nyc >= 4096): the surface passes through LDS 4096 lags at a time with
+// coalesced loads (padded 1 per 16), and thread t scans lags 16 t .. 16 t + 15
+// of each chunk knowing the maximum of every earlier lag -- a wave prefix
+// maximum by lane shifts, the earlier waves' maxima and the earlier chunks'
+// carry -- as k_time_search_i8's fused scan.  Same peak, same (float) maximum;
+// the mean sums in another order.
+__global__ void __launch_bounds__(kThreads) k_time_peak4k(DevParams P, const double* __restrict__ surfd,
+                                                          int* __restrict__ tpeak, double* __restrict__ tcave) {
+  __shared__ double gs[4096 + 256];
+  __shared__ double wmax[kThreads / 64];
+  __shared__ unsigned long long redk[kThreads / 64];
+  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63;
+  const int total = P.nxc * P.nyc;
+  const double* g = surfd + (size_t)b * total;
+  const double* gc = gs + 17 * t;
+  double carry = -1.0, sm = 0.0;                  // carry: the maximum of every earlier chunk
+  long long last = -1;
+  for (int c0 = 0; c0 < total; c0 += 4096) {
+    __syncthreads();                              // the last chunk's values and maxima are read
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int e = t + kThreads * k;
+      gs[e + (e >> 4)] = g[c0 + e];
+    }
+    __syncthreads();
+    double mx = -1.0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      mx = fabs(gc[k]) > mx ? fabs(gc[k]) : mx;
+      sm += gc[k];
+    }
+    double inc = mx;                              // inclusive prefix maximum within the wave
+    for (int o = 1; o < 64; o <<= 1) {
+      const double v = __shfl_up(inc, o);
+      if (lane >= o) inc = v > inc ? v : inc;
+    }
+    if (lane == 63) wmax[t >> 6] = inc;
+    __syncthreads();
+    double pm = __shfl_up(inc, 1);
+    if (lane == 0) pm = -1.0;
+    for (int u = 0; u < (t >> 6); u++) pm = wmax[u] > pm ? wmax[u] : pm;
+    pm = carry > pm ? carry : pm;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const double v = fabs(gc[k]);
+      if (v > (double)(float)pm) last = c0 + 16 * t + k;
+      pm = v > pm ? v : pm;
+    }
+    for (int u = 0; u < kThreads / 64; u++) carry = wmax[u] > carry ? wmax[u] : carry;
+  }
+  const unsigned long long key = block_max_u64((unsigned long long)(last + 1), redk);
+  const double ssum = block_sum_f64(sm, (double*)redk);
+  if (t == 0) {
+    const int idx = (int)key - 1;
+    tpeak[4 * b + 0] = idx / P.nxc;
+    tpeak[4 * b + 1] = idx % P.nxc;
+    tpeak[4 * b + 2] = __float_as_int((float)carry);
+    tpeak[4 * b + 3] = 0;
+    tcave[b] = ssum / (double)total;
+  }
+}
+
 __global__ void k_init(unsigned long long* __restrict__ sums, unsigned long long* __restrict__ peak, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < 2 * n) sums[i] = 0ull;
@@ -3997,8 +4060,12 @@ static int time_search(xcorr_hip_ctx* c, const PatchBufs& b, int nb, bool timing
     });
   if (rc || i8) return rc;
   return launch_timed(c, timing, "k_time_peak", [&] {
-    hipLaunchKernelGGL(k_time_peak, dim3(nb), dim3(kThreads), 0, c->stream, d, (const double*)b.surfd, b.tpeak,
-                       b.tcave);
+    if ((d.nxc * d.nyc) % 4096 == 0)
+      hipLaunchKernelGGL(k_time_peak4k, dim3(nb), dim3(kThreads), 0, c->stream, d, (const double*)b.surfd, b.tpeak,
+                         b.tcave);
+    else
+      hipLaunchKernelGGL(k_time_peak, dim3(nb), dim3(kThreads), 0, c->stream, d, (const double*)b.surfd, b.tpeak,
+                         b.tcave);
   });
 }
 
