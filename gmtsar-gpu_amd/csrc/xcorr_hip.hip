@@ -2698,8 +2698,8 @@ __global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const 
 // columns dxq + 64 k against the 64 x 64 template block at (64 h, 64 k).  The
 // four sub-searches add into the same int32 accumulators, each of which sums
 // at most 3 x 4 x 4096 limb products of magnitude <= 2^12: still exact.  The
-// surface goes to surfd for k_time_peak; a patch with a value outside the limb
-// range takes the exact int64 loop.
+// surface goes to surfd for the peak scan (k_time_peak4k); a patch with a
+// value outside the limb range takes the exact int64 loop.
 __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8q(DevParams P, const float* __restrict__ amp,
                                                               const unsigned long long* __restrict__ sums,
                                                               const double* __restrict__ aa,
