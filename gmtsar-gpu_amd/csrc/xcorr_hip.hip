@@ -23,6 +23,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <climits>
 #include <functional>
@@ -2268,8 +2270,6 @@ __global__ void __launch_bounds__(kThreads) k_time_search_blk(DevParams P, const
 // --------------------------------------------------------------------------
 typedef int v4i_t __attribute__((ext_vector_type(4)));
 
-// Exact master energies sum_{y, x < 64} A[y + dy][x + dx]^2 of every lag
-// (int64 sliding sums), as doubles (exact: < 2^53).  One block per patch.
 // Inclusive prefix sum of an int64 over the 64 lanes of a wave with DPP lane
 // moves (row shifts by 1, 2, 4, 8 within rows of 16, then the row broadcasts
 // of lanes 15 and 31); exact in any order.
@@ -2288,6 +2288,24 @@ __device__ __forceinline__ long long wave_lane63_i64(long long x) {
   return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
+// The power POW (2 or 4) of a window integer for the master energies; for
+// -time4 the value is clamped to the range the matrix-core search takes
+// (|a| <= 16383, so a^4 < 2^56 and a row sum of 128 of them stays below
+// 2^63); a patch with a value beyond it takes the search's exact loop, which
+// does not use these energies.
+template <int POW>
+__device__ __forceinline__ long long int_pow(float f, float mf) {
+  const long long v = (int)(f - mf);
+  if (POW == 2) return v * v;
+  const long long c = v > 16383 ? 16383 : (v < -16383 ? -16383 : v);
+  return (c * c) * (c * c);
+}
+
+// Exact master energies sum_{y, x < 64} A[y + dy][x + dx]^POW of every lag of a
+// 128 x 128 window (xsearch = ysearch = 32), as the doubles the NCC uses: the
+// int64 sum converted (POW 2: exact, < 2^53) or the 128-bit sum rounded to
+// nearest (POW 4, as u128_to_double).  One block per patch.
+template <int POW>
 __global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const float* __restrict__ A1,
                                                             const int* __restrict__ rb,
                                                             const unsigned long long* __restrict__ sums,
@@ -2314,23 +2332,24 @@ __global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const f
   for (int k = 0; k < 32; k++) {
     const int r = wave + 4 * k;
     if (r >= 127) break;
-    const long long v0 = (int)(f0[k] - m1f);
-    const long long v1 = (int)(f1[k] - m1f);
-    const long long q0 = v0 * v0, q1 = v1 * v1;
+    const long long q0 = int_pow<POW>(f0[k], m1f), q1 = int_pow<POW>(f1[k], m1f);
     const long long p0 = wave_incl_scan_i64(q0), p1 = wave_incl_scan_i64(q1);
     const long long tot0 = wave_lane63_i64(p0);
     E[r * 64 + lane] = (tot0 - p0 + q0) + (p1 - q1);
   }
   __syncthreads();
   // aa[dy][dx] = sum_{y < 64} E[dy + y][dx]: thread (dx, d0) slides down lags d0 .. d0 + 15
+  using S_t = typename std::conditional<POW == 2, long long, u128>::type;
+  auto out = [](S_t v) { return POW == 2 ? (double)(long long)v : u128_to_double((u128)v); };
   const int dx = lane, d0 = 16 * wave;
-  long long s = 0;
-  for (int y = 0; y < 64; y++) s += E[(d0 + y) * 64 + dx];
+  S_t s = 0;
+  for (int y = 0; y < 64; y++) s += (S_t)E[(d0 + y) * 64 + dx];
   double* ob = aa + (size_t)b * 64 * 64;
-  ob[d0 * 64 + dx] = (double)s;
+  ob[d0 * 64 + dx] = out(s);
   for (int dy = d0 + 1; dy < d0 + 16; dy++) {
-    s += E[(dy + 63) * 64 + dx] - E[(dy - 1) * 64 + dx];
-    ob[dy * 64 + dx] = (double)s;
+    s += (S_t)E[(dy + 63) * 64 + dx];
+    s -= (S_t)E[(dy - 1) * 64 + dx];
+    ob[dy * 64 + dx] = out(s);
   }
 }
 
@@ -2369,9 +2388,10 @@ __device__ __forceinline__ int bl_chunk(int row, int c) { return 16 * ((c + (row
 // for every master row r that meets them, the Toeplitz rows A[r][x + dx]
 // times the template rows, one 16 x 16 x 64 int8 product per limb pair
 // (ka < KA, kb < KB), accumulated in acc[i][j][ka + kb].
-template <int KA, int KB>
+template <int KA, int KB, int NS>
 __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed char* Bl, int dx0, int dy0, int m,
-                                             int g, v4i_t (&acc)[2][2][5]) {
+                                             int g, v4i_t (&acc)[2][2][NS]) {
+  static_assert(KA + KB - 1 <= NS, "one accumulator per limb weight");
   auto load = [&](int r, v4i_t (&A)[2][KA], v4i_t (&B)[2][KB]) {
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -2423,33 +2443,50 @@ __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed
 
 // The window integers (int)(a - m1) of a 128 x 128 block (rows pa floats
 // apart from a) and the template integers (int)(t - m2) of a 64 x 64 block
-// (rows pb floats apart from t), as three 7-bit limbs each, into the LDS
-// layouts i8_lag_tiles reads: Al, 3 x 128 limb rows of kLimbPitch bytes (the
-// last 16 zero); Bl, 3 x kBRows rows of 64 bytes with kBPad zero rows on either
-// side of the 64 template rows.  big: a value outside the limb range (|v| >=
-// 2^20); hia / hib: limb 2 nonzero somewhere in the window / the template;
-// b2 += the template's sum of squares.  Four columns per thread and step: the
-// float4 loads of the window (16 per thread, in NH groups) and of the template
-// (4, with the first group) are issued before their first use; each limb row
-// takes one dword per 4 columns.  NH = 2 halves the registers the loads hold,
-// for callers that keep accumulators live across blocks.
-template <int NH = 1>
+// (rows pb floats apart from t), as NL = 3 signed 7-bit limbs each (SQ: their
+// squares as NL = 4 unsigned 7-bit limbs, for -time4), into the LDS layouts
+// i8_lag_tiles reads: Al, NL x 128 limb rows of kLimbPitch bytes (the last 16
+// zero); Bl, NL x kBRows rows of 64 bytes with kBPad zero rows on either side
+// of the 64 template rows.  big: a value outside the limb range (|v| >= 2^20;
+// SQ: |v| > 16383, so v^2 < 2^28); hia / hib: the top limb (limb 2; SQ: limb
+// 3) nonzero somewhere in the window / the template; e2 += the template's sum
+// of squares (SQ: of fourth powers, 128-bit).  Four columns per thread and
+// step: the float4 loads of the window (16 per thread, in NH groups) and of the
+// template (4, with the first group) are issued before their first use; each
+// limb row takes one dword per 4 columns.  NH = 2 halves the registers the
+// loads hold, for callers that keep accumulators live across blocks.
+template <int NH = 1, bool SQ = false>
 __device__ __forceinline__ void pack_i8_limbs(const float* __restrict__ a1, int pa, const float* __restrict__ t,
                                               int pb, float m1f, float m2f, signed char* Al, signed char* Bl,
-                                              int& big, int& hia, int& hib, long long& b2) {
+                                              int& big, int& hia, int& hib,
+                                              typename std::conditional<SQ, u128, long long>::type& e2s) {
   static_assert(kThreads == 256, "16 window and 4 template float4 per thread");
-  auto pack4 = [&](const float4 f, float mf, unsigned (&w)[3], int& hi, long long* e2) {
+  constexpr int NL = SQ ? 4 : 3;
+  using E_t = typename std::conditional<SQ, u128, long long>::type;
+  auto pack4 = [&](const float4 f, float mf, unsigned (&w)[NL], int& hi, E_t* e2) {
     const int v[4] = {(int)(f.x - mf), (int)(f.y - mf), (int)(f.z - mf), (int)(f.w - mf)};
-    w[0] = w[1] = w[2] = 0u;
+#pragma unroll
+    for (int k = 0; k < NL; k++) w[k] = 0u;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      big |= (v[q] >= (1 << 20) || v[q] <= -(1 << 20));
-      if (e2) *e2 += (long long)v[q] * v[q];
-      signed char l[3];
-      limbs7(v[q], l);
-      hi |= l[2];
+      if constexpr (SQ) {
+        const int av = v[q] < 0 ? -v[q] : v[q];
+        big |= av > 16383;
+        const unsigned c = (unsigned)(av > 16383 ? 16383 : av);
+        const unsigned sq = c * c;                // < 2^28: limbs (sq >> 7 k) & 127
+        if (e2) *e2 += (u128)((unsigned long long)sq * sq);
+        hi |= (int)(sq >> 21);
 #pragma unroll
-      for (int k = 0; k < 3; k++) w[k] |= (unsigned)(unsigned char)l[k] << (8 * q);
+        for (int k = 0; k < 4; k++) w[k] |= ((sq >> (7 * k)) & 127u) << (8 * q);
+      } else {
+        big |= (v[q] >= (1 << 20) || v[q] <= -(1 << 20));
+        if (e2) *e2 += (long long)v[q] * v[q];
+        signed char l[3];
+        limbs7(v[q], l);
+        hi |= l[2];
+#pragma unroll
+        for (int k = 0; k < 3; k++) w[k] |= (unsigned)(unsigned char)l[k] << (8 * q);
+      }
     }
   };
   static_assert(NH == 1 || NH == 2, "window loads in one or two groups");
@@ -2472,26 +2509,26 @@ __device__ __forceinline__ void pack_i8_limbs(const float* __restrict__ a1, int 
 #pragma unroll
     for (int k = 0; k < KW; k++) {
       const int idx = threadIdx.x + kThreads * (hh * KW + k), r = idx >> 5, x = 4 * (idx & 31);
-      unsigned w[3];
+      unsigned w[NL];
       pack4(fa[k], m1f, w, hia, nullptr);
 #pragma unroll
-      for (int l = 0; l < 3; l++) *(unsigned*)(Al + (l * 128 + r) * kLimbPitch + x) = w[l];
+      for (int l = 0; l < NL; l++) *(unsigned*)(Al + (l * 128 + r) * kLimbPitch + x) = w[l];
     }
   }
-  for (int e = threadIdx.x; e < 3 * 128 * 4; e += kThreads)      // columns 128 .. 143 of every limb row
+  for (int e = threadIdx.x; e < NL * 128 * 4; e += kThreads)     // columns 128 .. 143 of every limb row
     *(unsigned*)(Al + (e >> 2) * kLimbPitch + 128 + 4 * (e & 3)) = 0u;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int idx = threadIdx.x + kThreads * k, y = idx >> 4, x = 4 * (idx & 15);
-    unsigned w[3];
-    pack4(fb[k], m2f, w, hib, &b2);
+    unsigned w[NL];
+    pack4(fb[k], m2f, w, hib, &e2s);
 #pragma unroll
-    for (int l = 0; l < 3; l++) {
+    for (int l = 0; l < NL; l++) {
       const int row = l * kBRows + kBPad + y;
       *(unsigned*)(Bl + row * 64 + bl_chunk(row, x >> 4) + (x & 15)) = w[l];
     }
   }
-  for (int e = threadIdx.x; e < 3 * 64 * 4; e += kThreads) {       // the zero rows, 16 bytes a store
+  for (int e = threadIdx.x; e < NL * 64 * 4; e += kThreads) {      // the zero rows, 16 bytes a store
     const int l = e / 256, q = e % 256, row = q < 128 ? q >> 2 : kBPad + 64 + ((q - 128) >> 2);
     *(uint4*)(Bl + (l * kBRows + row) * 64 + 16 * (q & 3)) = make_uint4(0u, 0u, 0u, 0u);
   }
@@ -2632,18 +2669,22 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
   }
 }
 
-// Exact master energies sum_{y, x < 128} A[y + dy][x + dx]^2 of the 128 x 128
-// lags of a 256 x 256 window (xsearch = ysearch = 64), as doubles (the value
-// 1.0 * aa takes in the NCC).  One block per patch.  Row energies
-// E[r][dx] = sum_{x < 128} A[r][x + dx]^2 of 32 rows at a time go to LDS (lane
-// l of a wave holds A[r][64 s + l], s < 4: E[r][l] = suffix of segment 0 from l
-// + segment 1 + prefix of segment 2 below l, E[r][64 + l] likewise from segment
-// 1), then threads 0..127 run down one lag column each: with S(r) the sum of
-// the first r row energies, S(r) for r < 128 is parked in the output and
-// aa[dy] = S(dy + 128) - S(dy) overwrites it (int64 throughout, exact).
+// Exact master energies sum_{y, x < 128} A[y + dy][x + dx]^POW of the 128 x 128
+// lags of a 256 x 256 window (xsearch = ysearch = 64), as doubles (POW 2: the
+// value 1.0 * aa takes in the NCC; POW 4: the 128-bit sum rounded to nearest).
+// One block per patch.  Row energies E[r][dx] = sum_{x < 128} A[r][x + dx]^POW
+// of 32 rows at a time go to LDS (lane l of a wave holds A[r][64 s + l], s < 4:
+// E[r][l] = suffix of segment 0 from l + segment 1 + prefix of segment 2 below
+// l, E[r][64 + l] likewise from segment 1), then threads 0..127 run down one
+// lag column each: with S(r) the sum of the first r row energies, S(r) for
+// r < 128 is parked in the output (POW 4: its high 64 bits in hi, the patch's
+// surface slot, which the search overwrites later) and aa[dy] = S(dy + 128) -
+// S(dy) overwrites it (int64 or 128-bit throughout, exact).
+template <int POW>
 __global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const float* __restrict__ amp,
                                                              const unsigned long long* __restrict__ sums,
-                                                             double* __restrict__ aa, int nb) {
+                                                             double* __restrict__ aa, double* __restrict__ hi,
+                                                             int nb) {
   __shared__ long long E[32 * 128];
   const int b = xcd_item();
   if (b >= nb) return;
@@ -2652,8 +2693,10 @@ __global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const 
   const float* a1 = amp + (size_t)(2 * b) * P.N;
   double* ob = aa + (size_t)b * 128 * 128;
   long long* sb = (long long*)ob;
+  unsigned long long* hb = POW == 2 ? nullptr : (unsigned long long*)(hi + (size_t)b * 128 * 128);
   const int col = threadIdx.x & 127;
-  long long S = 0;
+  using S_t = typename std::conditional<POW == 2, long long, u128>::type;
+  S_t S = 0;
   for (int ch = 0; ch < 8; ch++) {
     float f[8][4];                                // rows 32 ch + wave + 4 k: all loads in flight together
 #pragma unroll
@@ -2665,8 +2708,7 @@ __global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const 
       long long q[4], pf[4], tt[4];
 #pragma unroll
       for (int sg = 0; sg < 4; sg++) {
-        const long long v = (int)(f[k][sg] - m1f);
-        q[sg] = v * v;
+        q[sg] = int_pow<POW>(f[k][sg], m1f);
         pf[sg] = wave_incl_scan_i64(q[sg]);
         tt[sg] = wave_lane63_i64(pf[sg]);
       }
@@ -2678,13 +2720,24 @@ __global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const 
     if (threadIdx.x < 128)
       for (int rr = 0; rr < 32; rr++) {
         const int r = 32 * ch + rr;               // S holds S(r)
-        if (r < 128) {
-          sb[r * 128 + col] = S;
+        const int o = (r & 127) * 128 + col;
+        if (POW == 2) {
+          if (r < 128) {
+            sb[o] = (long long)S;
+          } else {
+            const long long s0 = sb[o];
+            ob[o] = (double)((long long)S - s0);
+          }
         } else {
-          const long long s0 = sb[(r - 128) * 128 + col];
-          ob[(r - 128) * 128 + col] = (double)(S - s0);
+          if (r < 128) {
+            sb[o] = (long long)(unsigned long long)S;
+            hb[o] = (unsigned long long)((u128)S >> 64);
+          } else {
+            const u128 s0 = ((u128)hb[o] << 64) | (unsigned long long)sb[o];
+            ob[o] = u128_to_double((u128)S - s0);
+          }
         }
-        S += E[rr * 128 + col];
+        S += (S_t)E[rr * 128 + col];
       }
     __syncthreads();
   }
@@ -2784,6 +2837,135 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8q(DevParams P, co
         const int lag = dy * 128 + dx;
         const double den = sqrt(1.0 * ab_aa[lag] * d2d);
         sd[lag] = (den == 0.0) ? 0.0 : 100.0 * fabs((double)num / den);
+      }
+}
+
+// -time4 (Gatelli, do_time_int_xcorr.cu:103-144) on the int8 matrix cores for
+// xsearch = ysearch = XS = 32 (128 x 128 windows, one block per patch) or 64
+// (256 x 256 windows: four lag quadrants of four 64 x 64 template blocks each,
+// as k_time_search_i8q).  The numerator sum a^2 b^2 of every lag: with the
+// squares written as four unsigned 7-bit limbs (exact for |a|, |b| <= 16383),
+// the limb products of weight 2^(7 s), s = ka + kb <= 6, add into seven int32
+// accumulators, each of which sums at most 4 limb pairs x 4 blocks x 4096
+// products below 2^14 (< 2^31); sum_s acc_s 2^(7 s) (128-bit) is the exact
+// numerator.  The master energies sum a^4 come from k_time_energy32<4> /
+// k_time_energy256<4>, the template's sum b^4 from the packing (128-bit).
+// Limb 3 of the squares is zero throughout a block whose values lie within
+// +-1448; its products are skipped.  A patch with a value beyond 16383 takes
+// an exact 128-bit loop (k_time_search<1>'s sums).  The surface goes to surfd
+// for the peak scan (k_time_peak4k).
+template <int XS>
+__global__ void __launch_bounds__(kThreads, 1) k_time4_search_i8(DevParams P, const float* __restrict__ amp,
+                                                              const unsigned long long* __restrict__ sums,
+                                                              const double* __restrict__ aa,
+                                                              double* __restrict__ surfd, int nb) {
+  static_assert(XS == 32 || XS == 64, "128 x 128 or 256 x 256 windows");
+  constexpr int W = 4 * XS, NL = 2 * XS, NQ = XS == 64 ? 4 : 1;   // window side, lags (= template side), quadrants
+  __shared__ __attribute__((aligned(16))) signed char Al[4 * 128 * kLimbPitch];
+  __shared__ __attribute__((aligned(16))) signed char Bl[4 * kBRows * 64];
+  __shared__ int sbig[NQ], shi[NQ][2];            // per template block: values beyond the limbs; limb 3 in use
+  const int item = xcd_item();                    // the quadrants of a patch are consecutive items
+  if (item >= NQ * nb) return;
+  const int b = item / NQ, q = item % NQ;
+  const int dyq = 64 * (q >> 1), dxq = 64 * (q & 1);
+  const float m1f = (float)patch_mean(sums[2 * b], P.N);
+  const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
+  const float* a1 = amp + (size_t)(2 * b) * P.N;
+  const float* a2 = a1 + P.N;
+  if (threadIdx.x < NQ) {
+    sbig[threadIdx.x] = 0;
+    shi[threadIdx.x][0] = 0;
+    shi[threadIdx.x][1] = 0;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+  const int dx0 = __builtin_amdgcn_readfirstlane(32 * (wave & 1));
+  const int dy0 = __builtin_amdgcn_readfirstlane(32 * (wave >> 1));
+  v4i_t acc[2][2][7];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int w = 0; w < 7; w++) acc[i][j][w] = v4i_t{0, 0, 0, 0};
+  u128 b4 = 0;
+  bool anybig = false;
+  for (int ph = 0; ph < NQ; ph++) {
+    const int h = ph >> 1, k = ph & 1;
+    __syncthreads();                              // the flags are cleared; the last block's limbs are read
+    int big = 0, hia = 0, hib = 0;
+    pack_i8_limbs<2, true>(a1 + (size_t)(dyq + 64 * h) * W + dxq + 64 * k, W,
+                           a2 + (size_t)(P.ys + 64 * h) * W + P.xs + 64 * k, W, m1f, m2f, Al, Bl, big, hia, hib, b4);
+    if (big) atomicOr(&sbig[ph], 1);
+    if (hia) atomicOr(&shi[ph][0], 1);
+    if (hib) atomicOr(&shi[ph][1], 1);
+    __syncthreads();                              // the limbs and flags are published
+    anybig = anybig || sbig[ph];
+    if (anybig) continue;
+    if (shi[ph][0] && shi[ph][1]) i8_lag_tiles<4, 4>(Al, Bl, dx0, dy0, m, g, acc);
+    else if (shi[ph][0]) i8_lag_tiles<4, 3>(Al, Bl, dx0, dy0, m, g, acc);
+    else if (shi[ph][1]) i8_lag_tiles<3, 4>(Al, Bl, dx0, dy0, m, g, acc);
+    else i8_lag_tiles<3, 3>(Al, Bl, dx0, dy0, m, g, acc);
+  }
+  // the template's sum of b^4 over the block (every thread gets it): partial
+  // sums through LDS (over the window limbs, free once every wave is done)
+  u128* red = (u128*)Al;
+  auto block_sum_u128 = [&](u128 v) -> u128 {
+    __syncthreads();
+    red[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+      u128 t = 0;
+      for (int i = 0; i < 16; i++) t += red[16 * threadIdx.x + i];
+      red[kThreads + threadIdx.x] = t;
+    }
+    __syncthreads();
+    u128 t = 0;
+    for (int i = 0; i < 16; i++) t += red[kThreads + i];
+    return t;
+  };
+  double* sd = surfd + (size_t)b * NL * NL;
+  const double* ab_aa = aa + (size_t)b * NL * NL;
+  if (anybig) {                                   // exact 128-bit loop (values beyond the limb range)
+    u128 t4 = 0;
+    for (int e = threadIdx.x; e < NL * NL; e += kThreads) {
+      const long long bv = (int)(a2[(size_t)(P.ys + e / NL) * W + P.xs + e % NL] - m2f);
+      const unsigned long long ub = (unsigned long long)(bv * bv);
+      t4 += (u128)ub * ub;
+    }
+    const double d2d = u128_to_double(block_sum_u128(t4));
+    for (int l = threadIdx.x; l < 64 * 64; l += kThreads) {
+      const int dy = dyq + (l >> 6), dx = dxq + (l & 63);
+      u128 n4 = 0, a4 = 0;
+      for (int y = 0; y < NL; y++)
+        for (int x = 0; x < NL; x++) {
+          const long long av = (int)(a1[(size_t)(y + dy) * W + x + dx] - m1f);
+          const long long bv = (int)(a2[(size_t)(P.ys + y) * W + P.xs + x] - m2f);
+          const unsigned long long ua = (unsigned long long)(av * av), ub = (unsigned long long)(bv * bv);
+          n4 += (u128)ua * ub;
+          a4 += (u128)ua * ua;
+        }
+      const double den = sqrt(u128_to_double(a4) * d2d);
+      const double gg = (den == 0.0) ? 0.0 : fabs(u128_to_double(n4) / den);
+      sd[dy * NL + dx] = (gg <= 0.5) ? 0.0 : 100.0 * sqrt(gg * 2.0 - 1.0);
+    }
+    return;
+  }
+  const double d2d = u128_to_double(block_sum_u128(b4));
+  // lane holds lags dx = dxq + dx0 + 16 i + 4 g + q, dy = dyq + dy0 + 16 j + m
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int qq = 0; qq < 4; qq++) {
+        u128 num = 0;
+#pragma unroll
+        for (int w = 0; w < 7; w++) num += (u128)(unsigned)acc[i][j][w][qq] << (7 * w);
+        const int dx = dxq + dx0 + 16 * i + 4 * g + qq, dy = dyq + dy0 + 16 * j + m;
+        const int lag = dy * NL + dx;
+        const double den = sqrt(ab_aa[lag] * d2d);
+        const double gg = (den == 0.0) ? 0.0 : fabs(u128_to_double(num) / den);
+        sd[lag] = (gg <= 0.5) ? 0.0 : 100.0 * sqrt(gg * 2.0 - 1.0);
       }
 }
 
@@ -3021,7 +3203,7 @@ struct xcorr_hip_ctx {
   int tune_batch = 0;                     // locations per launch (0: max_batch)
   int nt_rows = kThreads;                 // threads of the generic row / column kernels
   bool reg_paths = true;                  // register-resident kernels where the shape has them
-  double* tcaa = nullptr;                 // -time, xsearch = ysearch = 32 or 64: master energies of every lag
+  double* tcaa = nullptr;                 // -time / -time4, xsearch = ysearch = 32 or 64: master energies of every lag
   // strip batches alternate between this context and a second one (own
   // buffers and stream, created on the first call with more than one batch),
   // so the kernels of one batch overlap those of the next
@@ -3056,7 +3238,7 @@ struct PatchBufs {
   double* partial;              // [B][max(groups4, nyc/G4)] surface partial sums
   int* tpeak;                   // [B][4] time-mode peaks
   double* tcave;                // [B] time-mode surface means
-  double* tcaa;                 // [B][nyc][nxc] -time on 128 x 128 and 256 x 256 windows: master energies of every lag
+  double* tcaa;                 // [B][nyc][nxc] -time / -time4 on 128 x 128 and 256 x 256 windows: master energies of every lag
 };
 
 static PatchBufs ctx_bufs(const xcorr_hip_ctx* c) {
@@ -3157,6 +3339,13 @@ static int groups4_for(const xcorr_hip_ctx* c) {
 // Strip contexts (square 128 / 256 / 512 windows, ri 2, frequency mode) run
 // the strips with the default radix plans; a caller-set plan runs the
 // per-patch kernels, whose buffers are allocated when such a plan is first set.
+// -time and -time4 on 128 x 128 and 256 x 256 windows (xsearch = ysearch = 32
+// or 64) run on the int8 matrix cores, with the master energies of every lag
+// in a buffer of their own (tcaa).
+static bool time_i8_shape(int corr_flag, int xs, int ys) {
+  return (corr_flag == XCORR_HIP_TIME || corr_flag == XCORR_HIP_TIME4) && (xs == 32 || xs == 64) && ys == xs;
+}
+
 static bool use_strips(const xcorr_hip_ctx* c) {
   const DevParams& d = c->dp;
   return c->strips && same_plan(d.plx, plan_for(d.NX)) && same_plan(d.ply, plan_for(d.NY));
@@ -3321,8 +3510,7 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
     alloc((void**)&c->partial, B * std::max(d.groups4, d.nyc / d.G4) * sizeof(double));
   } else {
     alloc((void**)&c->surfd, B * ns * sizeof(double));
-    if (p->corr_flag == XCORR_HIP_TIME && (p->xsearch == 32 || p->xsearch == 64) && p->ysearch == p->xsearch)
-      alloc((void**)&c->tcaa, B * ns * sizeof(double));
+    if (time_i8_shape(p->corr_flag, p->xsearch, p->ysearch)) alloc((void**)&c->tcaa, B * ns * sizeof(double));
     alloc((void**)&c->tpeak, B * 4 * sizeof(int));
     alloc((void**)&c->tcave, B * sizeof(double));
   }
@@ -3674,7 +3862,7 @@ static void launch_colpass(xcorr_hip_ctx* c, const PatchBufs& b, const DevParams
 // per patch (rb null: A1 = A2 = amp) or on strip lines from row bases rb.
 static void launch_time_i8(const PatchBufs& b, const DevParams& d, hipStream_t st, const float* A1, const float* A2,
                            const int* rb, int nb) {
-  hipLaunchKernelGGL(k_time_energy32, dim3((nb + 7) & ~7), dim3(kThreads), 0, st, d, A1, rb,
+  hipLaunchKernelGGL(k_time_energy32<2>, dim3((nb + 7) & ~7), dim3(kThreads), 0, st, d, A1, rb,
                      (const unsigned long long*)b.sums, b.tcaa, nb);
   hipLaunchKernelGGL(k_time_search_i8, dim3((nb + 7) & ~7), dim3(kThreads), 0, st, d, A1, A2, rb,
                      (const unsigned long long*)b.sums, (const double*)b.tcaa, b.surfd, b.tpeak, b.tcave, nb);
@@ -4031,14 +4219,31 @@ static int time_search(xcorr_hip_ctx* c, const PatchBufs& b, int nb, bool timing
   const int RCb = std::min(8, (5120 - (LRb - 1) * d.NX) / (d.NX + d.nx_corr));   // <= 40 KB of LDS
   const bool i8 = d.corr_flag == XCORR_HIP_TIME && d.xs == 32 && d.ys == 32 && b.tcaa;   // peak found in-kernel
   const bool i8q = d.corr_flag == XCORR_HIP_TIME && d.xs == 64 && d.ys == 64 && b.tcaa;  // 256 x 256 windows
-  if (i8)
+  const bool i8g = d.corr_flag == XCORR_HIP_TIME4 && time_i8_shape(d.corr_flag, d.xs, d.ys) && b.tcaa;
+  if (i8g)
+    rc = launch_timed(c, timing, "k_time_search4", [&] {
+      if (d.xs == 32) {
+        hipLaunchKernelGGL(k_time_energy32<4>, dim3((nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
+                           (const float*)b.amp, (const int*)nullptr, (const unsigned long long*)b.sums, b.tcaa, nb);
+        hipLaunchKernelGGL(k_time4_search_i8<32>, dim3((nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
+                           (const float*)b.amp, (const unsigned long long*)b.sums, (const double*)b.tcaa, b.surfd,
+                           nb);
+      } else {
+        hipLaunchKernelGGL(k_time_energy256<4>, dim3((nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
+                           (const float*)b.amp, (const unsigned long long*)b.sums, b.tcaa, b.surfd, nb);
+        hipLaunchKernelGGL(k_time4_search_i8<64>, dim3((4 * nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
+                           (const float*)b.amp, (const unsigned long long*)b.sums, (const double*)b.tcaa, b.surfd,
+                           nb);
+      }
+    });
+  else if (i8)
     rc = launch_timed(c, timing, "k_time_search", [&] {
       launch_time_i8(b, d, c->stream, (const float*)b.amp, (const float*)b.amp, nullptr, nb);
     });
   else if (i8q)
     rc = launch_timed(c, timing, "k_time_search", [&] {
-      hipLaunchKernelGGL(k_time_energy256, dim3((nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
-                         (const float*)b.amp, (const unsigned long long*)b.sums, b.tcaa, nb);
+      hipLaunchKernelGGL(k_time_energy256<2>, dim3((nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
+                         (const float*)b.amp, (const unsigned long long*)b.sums, b.tcaa, (double*)nullptr, nb);
       hipLaunchKernelGGL(k_time_search_i8q, dim3((4 * nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
                          (const float*)b.amp, (const unsigned long long*)b.sums, (const double*)b.tcaa, b.surfd, nb);
     });
@@ -4304,7 +4509,7 @@ extern "C" int xcorr_hip_correlate_noalloc(xcorr_hip_ctx* c, const xcorr_hip_win
   const int rs = noalloc_scratch(c, &base);
   if (rs) return rs;
   const DevParams& d = c->dp;
-  const bool i8 = d.corr_flag == XCORR_HIP_TIME && (d.xs == 32 || d.xs == 64) && d.ys == d.xs;   // energies too
+  const bool i8 = time_i8_shape(d.corr_flag, d.xs, d.ys);        // energies too
   PatchBufs b;
   const size_t one = carve_patch_bufs(d, i8, 1, nullptr, &b);
   if (one > kNoAllocBytes)

@@ -98,12 +98,12 @@ def _scaled(pair, fm, fs, left_only=False):
     m, s = pair
     out = []
     for img, f in ((m, fm), (s, fs)):
-        v = img.astype(np.int64)
+        v = img.astype(np.float64)
         if left_only:
             v[:, : v.shape[1] // 2] *= f
         else:
             v *= f
-        out.append(np.clip(v, -32768, 32767).astype(np.int16))
+        out.append(np.clip(np.rint(v), -32768, 32767).astype(np.int16))
     return tuple(out)
 
 
@@ -167,9 +167,58 @@ def test_time_xs64_matches_int64_bruteforce(xg, orc, pair, fm, fs, left):
 
 
 def test_time4_xs64_bit_exact(xg, orc, pair):
-    """-time4 keeps its 128-bit search at the default search size."""
+    """-time4 at the default search size (the int8 matrix-core search)."""
     _, got, ref, _, _ = _run(xg, orc, pair, 64, 64, 1, nxl=2, nyl=1)
     _exact(got, ref)
+
+
+def _gatelli_surface(i1, i2, xs):
+    """The -time4 surface of one window (do_time_int_xcorr.cu:103-144) from
+    exact integer sums: with A = a^2, B = b^2 split as A = Ah 2^14 + Al, every
+    partial sum of Ah B, Al B, Ah A and Al A stays below 2^63 in int64 and the
+    totals are Python integers; float() rounds them to nearest as the 128-bit
+    conversion does, then the ratio and 100 sqrt(2 g - 1) in float64."""
+    import math
+    from numpy.lib.stride_tricks import sliding_window_view
+    n = 2 * xs
+    A = i1 * i1
+    B = i2[xs:xs + n, xs:xs + n] ** 2
+    assert A.max() < 2 ** 31 and B.max() < 2 ** 31          # Ah B summed over 2^14 terms < 2^63
+    b4 = (int(((B >> 14) * B).sum()) << 14) + int(((B & 16383) * B).sum())
+    out = np.zeros((n, n))
+    for dy in range(n):
+        win = sliding_window_view(A[dy:dy + n, :2 * n - 1], (n, n))[0]      # (dx, y, x)
+        wh, wl = win >> 14, win & 16383
+        nh, nl = np.einsum("jkl,kl->j", wh, B), np.einsum("jkl,kl->j", wl, B)
+        ah, al = np.einsum("jkl,jkl->j", wh, win), np.einsum("jkl,jkl->j", wl, win)
+        for dx in range(n):
+            num = (int(nh[dx]) << 14) + int(nl[dx])
+            a4 = (int(ah[dx]) << 14) + int(al[dx])
+            den = math.sqrt(float(a4) * float(b4))
+            g = 0.0 if den == 0.0 else abs(float(num) / den)
+            out[dy, dx] = 0.0 if g <= 0.5 else 100.0 * math.sqrt(g * 2.0 - 1.0)
+    return out
+
+
+@pytest.mark.parametrize("xs", [32, 64])
+@pytest.mark.parametrize("fm,fs,left", [(1, 1, False), (0.25, 0.25, False), (0.25, 1, False), (1, 0.25, False),
+                                        (0.25, 1, True), (8, 8, True)],
+                         ids=["limb3_both", "limb3_none", "limb3_template", "limb3_master", "mixed",
+                              "beyond_limbs"])
+def test_time4_matches_exact_sums(xg, orc, pair, xs, fm, fs, left):
+    """-time4 on 128 x 128 and 256 x 256 windows runs on the int8 matrix cores
+    (the squares as four 7-bit limbs; limb 3 skipped where the values stay
+    within +-1448; values beyond 16383 take the exact 128-bit loop): bit-exact
+    against the restatement, and the surface equal to the one from exact
+    integer sums."""
+    sp = _scaled(pair, fm, fs, left)
+    p, got, ref, surf, (a1, a2, means) = _run(xg, orc, sp, xs, xs, 1, nxl=2, nyl=2)
+    _exact(got, ref)
+    m1f, m2f = np.float32(means[0]), np.float32(means[1])
+    i1 = (a1 - m1f).astype(np.int64)
+    i2 = (a2 - m2f).astype(np.int64)
+    print("max |v| master %d template %d" % (np.abs(i1).max(), np.abs(i2[xs:3 * xs, xs:3 * xs]).max()))
+    assert np.array_equal(surf, _gatelli_surface(i1, i2, xs))
 
 
 @pytest.mark.parametrize("xs", [32, 64])
