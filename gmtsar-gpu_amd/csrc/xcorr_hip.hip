@@ -2288,29 +2288,46 @@ __device__ __forceinline__ long long wave_lane63_i64(long long x) {
   return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
-// The power POW (2 or 4) of a window integer for the master energies; for
-// -time4 the value is clamped to the range the matrix-core search takes
-// (|a| <= 16383, so a^4 < 2^56 and a row sum of 128 of them stays below
-// 2^63); a patch with a value beyond it takes the search's exact loop, which
-// does not use these energies.
+// The square of a window integer for -time4, below 2^40 (six unsigned 7-bit
+// limbs) for |v| < 2^20; a magnitude beyond is clamped to 2^20, whose square
+// kSqLimit flags the patch for the exact loop.
+constexpr unsigned long long kSqLimit = 1ull << 40;
+__device__ __forceinline__ unsigned long long sq_clamped(int v) {
+  const unsigned a = (unsigned)(v < 0 ? -v : v);
+  const unsigned long long c = a > (1u << 20) - 1 ? (1u << 20) : a;
+  return c * c;
+}
+
+// The power POW of a window integer for the master energies: a^2 in lo (POW
+// 2), or the low and high 40 bits of a^4 < 2^80 (POW 4, |a| < 2^20; a patch
+// with a value beyond takes the search's exact loop, which does not use these
+// energies), so that sums of 128 parts stay far below 2^63.
 template <int POW>
-__device__ __forceinline__ long long int_pow(float f, float mf) {
-  const long long v = (int)(f - mf);
-  if (POW == 2) return v * v;
-  const long long c = v > 16383 ? 16383 : (v < -16383 ? -16383 : v);
-  return (c * c) * (c * c);
+__device__ __forceinline__ void pow_parts(float f, float mf, long long& lo, long long& hi) {
+  const int v = (int)(f - mf);
+  if (POW == 2) {
+    lo = (long long)v * v;
+    hi = 0;
+  } else {
+    const unsigned long long sq = sq_clamped(v);
+    const u128 q = (u128)sq * sq;
+    lo = (long long)(unsigned long long)(q & ((1ull << 40) - 1));
+    hi = (long long)(unsigned long long)(q >> 40);
+  }
 }
 
 // Exact master energies sum_{y, x < 64} A[y + dy][x + dx]^POW of every lag of a
 // 128 x 128 window (xsearch = ysearch = 32), as the doubles the NCC uses: the
 // int64 sum converted (POW 2: exact, < 2^53) or the 128-bit sum rounded to
-// nearest (POW 4, as u128_to_double).  One block per patch.
+// nearest (POW 4, as u128_to_double; the row energies in two 40-bit parts).
+// One block per patch.
 template <int POW>
 __global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const float* __restrict__ A1,
                                                             const int* __restrict__ rb,
                                                             const unsigned long long* __restrict__ sums,
                                                             double* __restrict__ aa, int nb) {
-  __shared__ long long E[127 * 64];
+  constexpr int NP = POW == 2 ? 1 : 2;           // parts per row energy
+  __shared__ long long E[NP * 127 * 64];
   const int b = xcd_item();                     // neighbours in a strip share an XCD (and its L2)
   if (b >= nb) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2328,27 +2345,37 @@ __global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const f
     f0[k] = a1[r * 128 + lane];
     f1[k] = a1[r * 128 + 64 + lane];
   }
+  auto row_e = [&](long long q0, long long q1) {
+    const long long p0 = wave_incl_scan_i64(q0), p1 = wave_incl_scan_i64(q1);
+    const long long tot0 = wave_lane63_i64(p0);
+    return (tot0 - p0 + q0) + (p1 - q1);
+  };
 #pragma unroll
   for (int k = 0; k < 32; k++) {
     const int r = wave + 4 * k;
     if (r >= 127) break;
-    const long long q0 = int_pow<POW>(f0[k], m1f), q1 = int_pow<POW>(f1[k], m1f);
-    const long long p0 = wave_incl_scan_i64(q0), p1 = wave_incl_scan_i64(q1);
-    const long long tot0 = wave_lane63_i64(p0);
-    E[r * 64 + lane] = (tot0 - p0 + q0) + (p1 - q1);
+    long long l0, h0, l1, h1;
+    pow_parts<POW>(f0[k], m1f, l0, h0);
+    pow_parts<POW>(f1[k], m1f, l1, h1);
+    E[r * 64 + lane] = row_e(l0, l1);
+    if (NP == 2) E[127 * 64 + r * 64 + lane] = row_e(h0, h1);
   }
   __syncthreads();
   // aa[dy][dx] = sum_{y < 64} E[dy + y][dx]: thread (dx, d0) slides down lags d0 .. d0 + 15
   using S_t = typename std::conditional<POW == 2, long long, u128>::type;
   auto out = [](S_t v) { return POW == 2 ? (double)(long long)v : u128_to_double((u128)v); };
   const int dx = lane, d0 = 16 * wave;
+  auto row = [&](int r) -> S_t {                   // row energy r at column dx
+    if constexpr (POW == 2) return E[r * 64 + dx];
+    else return ((u128)(unsigned long long)E[127 * 64 + r * 64 + dx] << 40) + (unsigned long long)E[r * 64 + dx];
+  };
   S_t s = 0;
-  for (int y = 0; y < 64; y++) s += (S_t)E[(d0 + y) * 64 + dx];
+  for (int y = 0; y < 64; y++) s += row(d0 + y);
   double* ob = aa + (size_t)b * 64 * 64;
   ob[d0 * 64 + dx] = out(s);
   for (int dy = d0 + 1; dy < d0 + 16; dy++) {
-    s += (S_t)E[(dy + 63) * 64 + dx];
-    s -= (S_t)E[(dy - 1) * 64 + dx];
+    s += row(dy + 63);
+    s -= row(dy - 1);
     ob[dy * 64 + dx] = out(s);
   }
 }
@@ -2427,30 +2454,41 @@ __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed
     }
   };
   // master rows r = dy0 .. dy0 + 94 meet the wave's lag rows; the operands of
-  // row r + 1 load while the products of row r run
+  // row r + 1 load while the products of row r run (with more than 16 limb
+  // products per tile and row, one operand set: the products cover the loads)
   const int r_end = dy0 + 32 + 63;
-  v4i_t A0[2][KA], B0[2][KB], A1[2][KA], B1[2][KB];
-  load(dy0, A0, B0);
-  int r = dy0;
-  for (; r + 1 < r_end; r += 2) {
-    load(r + 1, A1, B1);
-    mma(r, A0, B0);
-    if (r + 2 < r_end) load(r + 2, A0, B0);
-    mma(r + 1, A1, B1);
+  if constexpr (KA * KB > 16) {
+    for (int r = dy0; r < r_end; r++) {
+      v4i_t A0[2][KA], B0[2][KB];
+      load(r, A0, B0);
+      mma(r, A0, B0);
+    }
+  } else {
+    v4i_t A0[2][KA], B0[2][KB], A1[2][KA], B1[2][KB];
+    load(dy0, A0, B0);
+    int r = dy0;
+    for (; r + 1 < r_end; r += 2) {
+      load(r + 1, A1, B1);
+      mma(r, A0, B0);
+      if (r + 2 < r_end) load(r + 2, A0, B0);
+      mma(r + 1, A1, B1);
+    }
+    if (r < r_end) mma(r, A0, B0);
   }
-  if (r < r_end) mma(r, A0, B0);
 }
 
 // The window integers (int)(a - m1) of a 128 x 128 block (rows pa floats
 // apart from a) and the template integers (int)(t - m2) of a 64 x 64 block
-// (rows pb floats apart from t), as NL = 3 signed 7-bit limbs each (SQ: their
-// squares as NL = 4 unsigned 7-bit limbs, for -time4), into the LDS layouts
-// i8_lag_tiles reads: Al, NL x 128 limb rows of kLimbPitch bytes (the last 16
-// zero); Bl, NL x kBRows rows of 64 bytes with kBPad zero rows on either side
-// of the 64 template rows.  big: a value outside the limb range (|v| >= 2^20;
-// SQ: |v| > 16383, so v^2 < 2^28); hia / hib: the top limb (limb 2; SQ: limb
-// 3) nonzero somewhere in the window / the template; e2 += the template's sum
-// of squares (SQ: of fourth powers, 128-bit).  Four columns per thread and
+// (rows pb floats apart from t), as NL = 3 signed 7-bit limbs each (SQ, for
+// -time4: their squares, below 2^40, as six unsigned 7-bit limbs, of which the
+// first NL = 4 are written here and limbs 4 and 5 by pack_i8_sq_high), into
+// the LDS layouts i8_lag_tiles reads: Al, NL x 128 limb rows of kLimbPitch
+// bytes (the last 16 zero); Bl, NL x kBRows rows of 64 bytes with kBPad zero
+// rows on either side of the 64 template rows.  big: a value outside the limb
+// range (|v| >= 2^20); hia / hib for the window / the template: limb 2 nonzero
+// somewhere (SQ: bit 0, limb 3 nonzero, a square >= 2^21; bit 1, limbs 4 and 5
+// in use, a square >= 2^28); e2 += the template's sum of squares (SQ: of
+// fourth powers, 128-bit).  Four columns per thread and
 // step: the float4 loads of the window (16 per thread, in NH groups) and of the
 // template (4, with the first group) are issued before their first use; each
 // limb row takes one dword per 4 columns.  NH = 2 halves the registers the
@@ -2470,14 +2508,12 @@ __device__ __forceinline__ void pack_i8_limbs(const float* __restrict__ a1, int 
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       if constexpr (SQ) {
-        const int av = v[q] < 0 ? -v[q] : v[q];
-        big |= av > 16383;
-        const unsigned c = (unsigned)(av > 16383 ? 16383 : av);
-        const unsigned sq = c * c;                // < 2^28: limbs (sq >> 7 k) & 127
-        if (e2) *e2 += (u128)((unsigned long long)sq * sq);
-        hi |= (int)(sq >> 21);
+        const unsigned long long sq = sq_clamped(v[q]);   // < 2^40: limbs (sq >> 7 k) & 127
+        big |= sq >= kSqLimit;
+        if (e2) *e2 += (u128)sq * sq;
+        hi |= ((sq >> 21) ? 1 : 0) | ((sq >> 28) ? 2 : 0);
 #pragma unroll
-        for (int k = 0; k < 4; k++) w[k] |= ((sq >> (7 * k)) & 127u) << (8 * q);
+        for (int k = 0; k < 4; k++) w[k] |= (unsigned)((sq >> (7 * k)) & 127u) << (8 * q);
       } else {
         big |= (v[q] >= (1 << 20) || v[q] <= -(1 << 20));
         if (e2) *e2 += (long long)v[q] * v[q];
@@ -2530,6 +2566,49 @@ __device__ __forceinline__ void pack_i8_limbs(const float* __restrict__ a1, int 
   }
   for (int e = threadIdx.x; e < NL * 64 * 4; e += kThreads) {      // the zero rows, 16 bytes a store
     const int l = e / 256, q = e % 256, row = q < 128 ? q >> 2 : kBPad + 64 + ((q - 128) >> 2);
+    *(uint4*)(Bl + (l * kBRows + row) * 64 + 16 * (q & 3)) = make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+
+// Limbs 4 and 5 of the squares (bits 28 .. 41) of the window and template
+// blocks of pack_i8_limbs<NH, true>, for a block pair in which a square
+// reaches 2^28: the same layouts, planes 4 and 5 (their tails and zero rows
+// too).
+__device__ __forceinline__ void pack_i8_sq_high(const float* __restrict__ a1, int pa, const float* __restrict__ t,
+                                                int pb, float m1f, float m2f, signed char* Al, signed char* Bl) {
+  auto high4 = [](const float4 f, float mf, unsigned (&w)[2]) {
+    const int v[4] = {(int)(f.x - mf), (int)(f.y - mf), (int)(f.z - mf), (int)(f.w - mf)};
+    w[0] = w[1] = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const unsigned long long sq = sq_clamped(v[q]);
+      w[0] |= (unsigned)((sq >> 28) & 127u) << (8 * q);
+      w[1] |= (unsigned)((sq >> 35) & 127u) << (8 * q);
+    }
+  };
+#pragma unroll 4
+  for (int k = 0; k < 16; k++) {
+    const int idx = threadIdx.x + kThreads * k, r = idx >> 5, x = 4 * (idx & 31);   // row idx / 32, columns 4 (idx % 32) ..
+    unsigned w[2];
+    high4(((const float4*)(a1 + (size_t)r * pa))[idx & 31], m1f, w);
+#pragma unroll
+    for (int l = 0; l < 2; l++) *(unsigned*)(Al + ((4 + l) * 128 + r) * kLimbPitch + x) = w[l];
+  }
+  for (int e = threadIdx.x; e < 2 * 128 * 4; e += kThreads)      // columns 128 .. 143 of the two planes
+    *(unsigned*)(Al + (4 * 128 + (e >> 2)) * kLimbPitch + 128 + 4 * (e & 3)) = 0u;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int idx = threadIdx.x + kThreads * k, y = idx >> 4, x = 4 * (idx & 15);
+    unsigned w[2];
+    high4(((const float4*)(t + (size_t)y * pb))[idx & 15], m2f, w);
+#pragma unroll
+    for (int l = 0; l < 2; l++) {
+      const int row = (4 + l) * kBRows + kBPad + y;
+      *(unsigned*)(Bl + row * 64 + bl_chunk(row, x >> 4) + (x & 15)) = w[l];
+    }
+  }
+  for (int e = threadIdx.x; e < 2 * 64 * 4; e += kThreads) {       // the zero rows of the two planes
+    const int l = 4 + e / 256, q = e % 256, row = q < 128 ? q >> 2 : kBPad + 64 + ((q - 128) >> 2);
     *(uint4*)(Bl + (l * kBRows + row) * 64 + 16 * (q & 3)) = make_uint4(0u, 0u, 0u, 0u);
   }
 }
@@ -2673,7 +2752,8 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
 // lags of a 256 x 256 window (xsearch = ysearch = 64), as doubles (POW 2: the
 // value 1.0 * aa takes in the NCC; POW 4: the 128-bit sum rounded to nearest).
 // One block per patch.  Row energies E[r][dx] = sum_{x < 128} A[r][x + dx]^POW
-// of 32 rows at a time go to LDS (lane l of a wave holds A[r][64 s + l], s < 4:
+// (POW 4: in two 40-bit parts) of 32 rows at a time go to LDS (lane l of a
+// wave holds A[r][64 s + l], s < 4:
 // E[r][l] = suffix of segment 0 from l + segment 1 + prefix of segment 2 below
 // l, E[r][64 + l] likewise from segment 1), then threads 0..127 run down one
 // lag column each: with S(r) the sum of the first r row energies, S(r) for
@@ -2685,7 +2765,8 @@ __global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const 
                                                              const unsigned long long* __restrict__ sums,
                                                              double* __restrict__ aa, double* __restrict__ hi,
                                                              int nb) {
-  __shared__ long long E[32 * 128];
+  constexpr int NP = POW == 2 ? 1 : 2;           // parts per row energy
+  __shared__ long long E[NP * 32 * 128];
   const int b = xcd_item();
   if (b >= nb) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2703,18 +2784,25 @@ __global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const 
     for (int k = 0; k < 8; k++)
 #pragma unroll
       for (int sg = 0; sg < 4; sg++) f[k][sg] = a1[(32 * ch + wave + 4 * k) * 256 + 64 * sg + lane];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      long long q[4], pf[4], tt[4];
+    // E[r][l] and E[r][64 + l] of one part (lane l holds segment sg's q[sg])
+    auto row_e = [&](const long long (&q)[4], long long* e) {
+      long long pf[4], tt[4];
 #pragma unroll
       for (int sg = 0; sg < 4; sg++) {
-        q[sg] = int_pow<POW>(f[k][sg], m1f);
         pf[sg] = wave_incl_scan_i64(q[sg]);
         tt[sg] = wave_lane63_i64(pf[sg]);
       }
+      e[lane] = (tt[0] - pf[0] + q[0]) + tt[1] + (pf[2] - q[2]);
+      e[64 + lane] = (tt[1] - pf[1] + q[1]) + tt[2] + (pf[3] - q[3]);
+    };
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      long long ql[4], qh[4];
+#pragma unroll
+      for (int sg = 0; sg < 4; sg++) pow_parts<POW>(f[k][sg], m1f, ql[sg], qh[sg]);
       const int rr = wave + 4 * k;
-      E[rr * 128 + lane] = (tt[0] - pf[0] + q[0]) + tt[1] + (pf[2] - q[2]);
-      E[rr * 128 + 64 + lane] = (tt[1] - pf[1] + q[1]) + tt[2] + (pf[3] - q[3]);
+      row_e(ql, E + rr * 128);
+      if (NP == 2) row_e(qh, E + 32 * 128 + rr * 128);
     }
     __syncthreads();
     if (threadIdx.x < 128)
@@ -2737,7 +2825,8 @@ __global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const 
             ob[o] = u128_to_double((u128)S - s0);
           }
         }
-        S += (S_t)E[rr * 128 + col];
+        if constexpr (POW == 2) S += E[rr * 128 + col];
+        else S += ((u128)(unsigned long long)E[32 * 128 + rr * 128 + col] << 40) + (unsigned long long)E[rr * 128 + col];
       }
     __syncthreads();
   }
@@ -2844,16 +2933,18 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8q(DevParams P, co
 // xsearch = ysearch = XS = 32 (128 x 128 windows, one block per patch) or 64
 // (256 x 256 windows: four lag quadrants of four 64 x 64 template blocks each,
 // as k_time_search_i8q).  The numerator sum a^2 b^2 of every lag: with the
-// squares written as four unsigned 7-bit limbs (exact for |a|, |b| <= 16383),
-// the limb products of weight 2^(7 s), s = ka + kb <= 6, add into seven int32
-// accumulators, each of which sums at most 4 limb pairs x 4 blocks x 4096
-// products below 2^14 (< 2^31); sum_s acc_s 2^(7 s) (128-bit) is the exact
-// numerator.  The master energies sum a^4 come from k_time_energy32<4> /
-// k_time_energy256<4>, the template's sum b^4 from the packing (128-bit).
-// Limb 3 of the squares is zero throughout a block whose values lie within
-// +-1448; its products are skipped.  A patch with a value beyond 16383 takes
-// an exact 128-bit loop (k_time_search<1>'s sums).  The surface goes to surfd
-// for the peak scan (k_time_peak4k).
+// squares (below 2^40 for |a|, |b| < 2^20) written as six unsigned 7-bit
+// limbs, the limb products of weight 2^(7 s), s = ka + kb <= 10, add into
+// eleven int32 accumulators, each of which sums at most 6 limb pairs x 4
+// blocks x 4096 products below 2^14 (< 2^31); sum_s acc_s 2^(7 s) (128-bit)
+// is the exact numerator.  The master energies sum a^4 come from
+// k_time_energy32<4> / k_time_energy256<4>, the template's sum b^4 from the
+// packing (128-bit).  Per block pair only the limbs in use are multiplied:
+// limbs 4 and 5 (and their second packing pass) where a square reaches 2^28
+// (|v| > 16383), else limb 3 where one reaches 2^21 (|v| > 1448), else limbs
+// 0 .. 2.  A patch with a value beyond 2^20 (float input) takes an exact
+// 128-bit loop (k_time_search<1>'s sums).  The surface goes to surfd for the
+// peak scan (k_time_peak4k).
 template <int XS>
 __global__ void __launch_bounds__(kThreads, 1) k_time4_search_i8(DevParams P, const float* __restrict__ amp,
                                                               const unsigned long long* __restrict__ sums,
@@ -2861,9 +2952,9 @@ __global__ void __launch_bounds__(kThreads, 1) k_time4_search_i8(DevParams P, co
                                                               double* __restrict__ surfd, int nb) {
   static_assert(XS == 32 || XS == 64, "128 x 128 or 256 x 256 windows");
   constexpr int W = 4 * XS, NL = 2 * XS, NQ = XS == 64 ? 4 : 1;   // window side, lags (= template side), quadrants
-  __shared__ __attribute__((aligned(16))) signed char Al[4 * 128 * kLimbPitch];
-  __shared__ __attribute__((aligned(16))) signed char Bl[4 * kBRows * 64];
-  __shared__ int sbig[NQ], shi[NQ][2];            // per template block: values beyond the limbs; limb 3 in use
+  __shared__ __attribute__((aligned(16))) signed char Al[6 * 128 * kLimbPitch];
+  __shared__ __attribute__((aligned(16))) signed char Bl[6 * kBRows * 64];
+  __shared__ int sbig[NQ], shi[NQ][2];            // per template block: values beyond the limbs; limbs in use
   const int item = xcd_item();                    // the quadrants of a patch are consecutive items
   if (item >= NQ * nb) return;
   const int b = item / NQ, q = item % NQ;
@@ -2880,31 +2971,43 @@ __global__ void __launch_bounds__(kThreads, 1) k_time4_search_i8(DevParams P, co
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
   const int dx0 = __builtin_amdgcn_readfirstlane(32 * (wave & 1));
   const int dy0 = __builtin_amdgcn_readfirstlane(32 * (wave >> 1));
-  v4i_t acc[2][2][7];
+  v4i_t acc[2][2][11];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++)
 #pragma unroll
-      for (int w = 0; w < 7; w++) acc[i][j][w] = v4i_t{0, 0, 0, 0};
+      for (int w = 0; w < 11; w++) acc[i][j][w] = v4i_t{0, 0, 0, 0};
   u128 b4 = 0;
   bool anybig = false;
+#pragma unroll 1
   for (int ph = 0; ph < NQ; ph++) {
     const int h = ph >> 1, k = ph & 1;
     __syncthreads();                              // the flags are cleared; the last block's limbs are read
     int big = 0, hia = 0, hib = 0;
-    pack_i8_limbs<2, true>(a1 + (size_t)(dyq + 64 * h) * W + dxq + 64 * k, W,
-                           a2 + (size_t)(P.ys + 64 * h) * W + P.xs + 64 * k, W, m1f, m2f, Al, Bl, big, hia, hib, b4);
+    const float* wb = a1 + (size_t)(dyq + 64 * h) * W + dxq + 64 * k;
+    const float* tb = a2 + (size_t)(P.ys + 64 * h) * W + P.xs + 64 * k;
+    pack_i8_limbs<2, true>(wb, W, tb, W, m1f, m2f, Al, Bl, big, hia, hib, b4);
     if (big) atomicOr(&sbig[ph], 1);
-    if (hia) atomicOr(&shi[ph][0], 1);
-    if (hib) atomicOr(&shi[ph][1], 1);
+    if (hia) atomicOr(&shi[ph][0], hia);
+    if (hib) atomicOr(&shi[ph][1], hib);
     __syncthreads();                              // the limbs and flags are published
     anybig = anybig || sbig[ph];
     if (anybig) continue;
-    if (shi[ph][0] && shi[ph][1]) i8_lag_tiles<4, 4>(Al, Bl, dx0, dy0, m, g, acc);
-    else if (shi[ph][0]) i8_lag_tiles<4, 3>(Al, Bl, dx0, dy0, m, g, acc);
-    else if (shi[ph][1]) i8_lag_tiles<3, 4>(Al, Bl, dx0, dy0, m, g, acc);
-    else i8_lag_tiles<3, 3>(Al, Bl, dx0, dy0, m, g, acc);
+    const int fa = shi[ph][0], fb = shi[ph][1];
+    if ((fa | fb) & 2) {                          // a square >= 2^28: all six limbs
+      pack_i8_sq_high(wb, W, tb, W, m1f, m2f, Al, Bl);
+      __syncthreads();
+      i8_lag_tiles<6, 6>(Al, Bl, dx0, dy0, m, g, acc);
+    } else if (fa && fb) {
+      i8_lag_tiles<4, 4>(Al, Bl, dx0, dy0, m, g, acc);
+    } else if (fa) {
+      i8_lag_tiles<4, 3>(Al, Bl, dx0, dy0, m, g, acc);
+    } else if (fb) {
+      i8_lag_tiles<3, 4>(Al, Bl, dx0, dy0, m, g, acc);
+    } else {
+      i8_lag_tiles<3, 3>(Al, Bl, dx0, dy0, m, g, acc);
+    }
   }
   // the template's sum of b^4 over the block (every thread gets it): partial
   // sums through LDS (over the window limbs, free once every wave is done)
@@ -2960,7 +3063,7 @@ __global__ void __launch_bounds__(kThreads, 1) k_time4_search_i8(DevParams P, co
       for (int qq = 0; qq < 4; qq++) {
         u128 num = 0;
 #pragma unroll
-        for (int w = 0; w < 7; w++) num += (u128)(unsigned)acc[i][j][w][qq] << (7 * w);
+        for (int w = 0; w < 11; w++) num += (u128)(unsigned)acc[i][j][w][qq] << (7 * w);
         const int dx = dxq + dx0 + 16 * i + 4 * g + qq, dy = dyq + dy0 + 16 * j + m;
         const int lag = dy * NL + dx;
         const double den = sqrt(ab_aa[lag] * d2d);

@@ -202,15 +202,14 @@ def _gatelli_surface(i1, i2, xs):
 
 @pytest.mark.parametrize("xs", [32, 64])
 @pytest.mark.parametrize("fm,fs,left", [(1, 1, False), (0.25, 0.25, False), (0.25, 1, False), (1, 0.25, False),
-                                        (0.25, 1, True), (8, 8, True)],
+                                        (0.25, 1, True), (8, 8, True), (0.25, 8, False)],
                          ids=["limb3_both", "limb3_none", "limb3_template", "limb3_master", "mixed",
-                              "beyond_limbs"])
+                              "six_limbs_mixed", "six_limbs_template"])
 def test_time4_matches_exact_sums(xg, orc, pair, xs, fm, fs, left):
     """-time4 on 128 x 128 and 256 x 256 windows runs on the int8 matrix cores
-    (the squares as four 7-bit limbs; limb 3 skipped where the values stay
-    within +-1448; values beyond 16383 take the exact 128-bit loop): bit-exact
-    against the restatement, and the surface equal to the one from exact
-    integer sums."""
+    (the squares as up to six 7-bit limbs: limb 3 only where a value leaves
+    +-1448, limbs 4 and 5 only where one leaves +-16383): bit-exact against the
+    restatement, and the surface equal to the one from exact integer sums."""
     sp = _scaled(pair, fm, fs, left)
     p, got, ref, surf, (a1, a2, means) = _run(xg, orc, sp, xs, xs, 1, nxl=2, nyl=2)
     _exact(got, ref)
@@ -222,21 +221,23 @@ def test_time4_matches_exact_sums(xg, orc, pair, xs, fm, fs, left):
 
 
 @pytest.mark.parametrize("xs", [32, 64])
-def test_time_values_beyond_limbs(xg, orc, pair, xs):
-    """Float (-real) input whose window integers reach past 2^20, beyond three
-    7-bit limbs: the matrix-core searches hand such a patch to their exact
-    int64 loop, and the result stays bit-exact."""
+@pytest.mark.parametrize("corr_flag", [0, 1], ids=["time", "time4"])
+def test_time_values_beyond_limbs(xg, orc, pair, xs, corr_flag):
+    """Float (-real) input whose window integers reach past 2^20, beyond the
+    limbs (three of the values for -time, six of the squares for -time4): the
+    matrix-core searches hand such a patch to their exact loop (int64 /
+    128-bit), and the result stays bit-exact."""
     m, s = pair
     mf = np.ascontiguousarray(m[..., 0].astype(np.float32) * np.float32(1500.0))
     sf = np.ascontiguousarray(s[..., 0].astype(np.float32) * np.float32(1500.0))
     assert np.abs(mf).max() > 2 * (1 << 20) and np.abs(sf).max() > 2 * (1 << 20)
-    p = xg.make_params(xs, xs, format=xg.FLOAT32, corr_flag=xg.TIME)
+    p = xg.make_params(xs, xs, format=xg.FLOAT32, corr_flag=xg.TIME4 if corr_flag else xg.TIME)
     x, y = xg.get_locations(512, 768, 2, 2, xs, xs)
     w = xg.make_windows(x, y, p.npx, p.npy)
     with xg.Engine(p) as eng:
         eng.set_images(mf, sf)
         got = eng.correlate(w)
-    op = orc.make_params(xs, xs, corr_flag=0)
+    op = orc.make_params(xs, xs, corr_flag=corr_flag)
     locs = np.stack([w["mx0"], w["my0"], w["sx0"], w["sy0"]], 1).astype(np.int32)
     ref = orc.run(op, mf, sf, locs, precision=32, fmt=1)
     _exact(got, ref)
