@@ -6,8 +6,8 @@ point's bit for bit, on two calls in a row -- the second with fewer (and
 different) windows, so nothing the first call left in the shared array can
 leak into the second -- for every kind of kernel chain: the 256 x 256
 register path (strip-shaped contexts), 512 x 512, the generic LDS kernels,
--time on the int8 matrix cores (128 x 128 and 256 x 256 windows) and
--time4.  Windows include some partly outside the images.
+and -time and -time4 on the int8 matrix cores (128 x 128 and 256 x 256
+windows).  Windows include some partly outside the images.
 """
 import numpy as np
 import pytest
@@ -26,7 +26,8 @@ CASES = [
     (16, 32, 2, 16, 500, 50),       # 388 per batch
     (32, 32, 0, 8, 400, 37),        # 339 per batch
     (64, 64, 0, 16, 100, 9),        # 256 x 256 -time on the int8 matrix cores: 85 per batch
-    (32, 32, 1, 16, 450, 9),        # 407 per batch
+    (32, 32, 1, 16, 450, 9),        # -time4, 339 per batch
+    (64, 64, 1, 16, 100, 9),        # 256 x 256 -time4: 85 per batch
 ]
 
 
