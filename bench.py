@@ -169,11 +169,13 @@ def main():
                     help="strip batches one after another on one stream (default: consecutive batches overlap on "
                          "two contexts); rocprof kernel durations are then those of each kernel alone")
     ap.add_argument("--small", action="store_true", help="the small workload (1024 x 2048, 1000 patches) instead")
-    ap.add_argument("--workload", default="strip", choices=("strip", "small", "batched", "proposal", "dense"),
+    ap.add_argument("--workload", default="strip",
+                    choices=("strip", "small", "batched", "proposal", "dense", "time"),
                     help="strip: 10800 x 27648 pair, 40,000 patches (default); small: 1024 x 2048, 1000 patches; "
                          "batched: 64 uneven pairs with mixed patch sizes in one graph launch; proposal: the "
                          "learned-proposal form beside plain xcorr on the strip workload; dense: 291k 128 x 128 "
-                         "windows on a 32-px grid, half time-domain, half frequency, one graph launch")
+                         "windows on a 32-px grid, half time-domain, half frequency, one graph launch; time: "
+                         "-time and -time4 at xsearch 32 and 64 on the strip pair")
     ap.add_argument("--tau", type=float, default=0.9, help="proposal: confidence threshold of the shrunken search")
     ap.add_argument("--tau-skip", type=float, default=1.01, help="proposal: confidence above which high-res is skipped")
     ap.add_argument("--xs-small", type=int, default=32, help="proposal: xsearch of the shrunken search")
@@ -228,7 +230,7 @@ def main():
         print("no GPU available", file=sys.stderr)
         return 1
     dev = torch.device("cuda:0")
-    others = {"batched": bench_batched, "proposal": bench_proposal, "dense": bench_dense}
+    others = {"batched": bench_batched, "proposal": bench_proposal, "dense": bench_dense, "time": bench_time}
     if args.workload in others:
         print(json.dumps(others[args.workload](args, xg, synth, torch, dev)), flush=True)
         return 0
@@ -406,14 +408,15 @@ def main():
 
 
 def other_workloads(args, xg, synth, torch, dev):
-    """Compact records of the batched, proposal and dense workloads, run after
-    the headline one with few steps and a short CPU sample each (their full
-    records: --workload batched / proposal / dense)."""
+    """Compact records of the batched, proposal, dense and time workloads, run
+    after the headline one with few steps and a short CPU sample each (their
+    full records: --workload batched / proposal / dense / time)."""
     sub = argparse.Namespace(**vars(args))
     sub.steps, sub.warmup = min(args.steps, 3), 1
     sub.cpu_seconds = max(1.0, args.cpu_seconds / 4)
     recs = {}
-    for name, fn in (("batched", bench_batched), ("proposal", bench_proposal), ("dense", bench_dense)):
+    for name, fn in (("batched", bench_batched), ("proposal", bench_proposal), ("dense", bench_dense),
+                     ("time", bench_time)):
         t0 = time.time()
         r = fn(sub, xg, synth, torch, dev)
         torch.cuda.empty_cache()
@@ -424,6 +427,9 @@ def other_workloads(args, xg, synth, torch, dev):
                "parity": r.get("parity"), "wall_s": round(time.time() - t0, 1)}
         if "accuracy" in r:
             rec["accuracy"] = r["accuracy"]
+        if "cases" in r:
+            rec["cases"] = [{k: c[k] for k in ("mode", "xsearch", "windows_per_s", "ms_per_step", "bit_exact_vs_cpu",
+                                               "cpu_windows_per_s")} for c in r["cases"]]
         if "proposal" in r:
             pp = r["proposal"]
             rec["proposal"] = {k: pp[k] for k in ("plain_patches_per_s", "proposal_patches_per_s", "speedup",
@@ -703,6 +709,108 @@ def bench_dense(args, xg, synth, torch, dev):
            "cpu_baseline": cpu, "accuracy": accuracy, "parity": parity}
     mu.close()
     return out
+
+
+def bench_time(args, xg, synth, torch, dev):
+    """The time-domain searches (do_time_int_xcorr.cu:103-189: -time, the exact
+    integer NCC over every lag, and -time4, Gatelli's fourth-power form) on the
+    strip workload's 10800 x 27648 pair at -nx 100 -ny 400: -xsearch/-ysearch
+    32 (128 x 128 windows, interp 8) and 64 (256 x 256 windows, the reference's
+    default, interp 16), each on the int8 matrix cores.  Per case: windows/s
+    over the timed steps, per-kernel times, and the first windows against the
+    FP32 CPU restatement bit for bit (its rate is the CPU baseline).  `value`
+    is the -time rate at xsearch 64."""
+    n_range, n_az, nxl, nyl = 10800, 27648, 100, 400
+    t0 = time.time()
+    m, s = synth.strip_pair_torch(n_range, n_az, lambda x: 2.1 + 1e-4 * x, lambda y: -7.6 + 3e-4 * y,
+                                  device="cuda")
+    torch.cuda.synchronize()
+    log("time-mode pair ready in %.1f s" % (time.time() - t0))
+    nthreads, host = host_cpu()
+    cases = []
+    for xs, ifc in ((32, 8), (64, 16)):
+        for cf, mode in ((xg.TIME, "-time"), (xg.TIME4, "-time4")):
+            p = xg.make_params(xs, xs, interp_factor=ifc, corr_flag=cf)
+            x, y = xg.get_locations(n_range, n_az, nxl, nyl, xs, xs)
+            w = xg.make_windows(x, y, p.npx, p.npy)
+            n = len(w)
+            dout = torch.zeros(n * xg.RESULT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+            with xg.Engine(p) as eng:
+                eng.set_images(m, s)
+                for _ in range(args.warmup):
+                    eng.correlate(w, out=dout, flags=xg.NO_SYNC)
+                eng.sync()
+                torch.cuda.synchronize()
+                trace_marker(torch)
+                t1 = time.perf_counter()
+                for _ in range(args.steps):
+                    eng.correlate(w, out=dout, flags=xg.NO_SYNC)
+                eng.sync()
+                step_s = (time.perf_counter() - t1) / args.steps
+                trace_marker(torch)
+                eng.correlate(w, out=dout, flags=xg.TIMING)
+                kt = {k: round(v[0], 3) for k, v in eng.kernel_times().items()}
+            got = dout.cpu().numpy().view(xg.RESULT_DTYPE)
+            rec = {"mode": mode, "xsearch": xs, "interp": ifc, "windows": n, "ms_per_step": round(step_s * 1e3, 3),
+                   "windows_per_s": round(n / step_s, 1), "kernel_ms": kt}
+            if not args.no_cpu:
+                rec.update(time_cpu_sample(args, xg, p, w, got, m, s, nthreads))
+            log("%s xsearch %d: %.0f windows/s%s" % (mode, xs, n / step_s,
+                ", CPU %.1f windows/s, bit-exact %s" % (rec["cpu_windows_per_s"], rec["bit_exact_vs_cpu"])
+                if "cpu_windows_per_s" in rec else ""))
+            rec.setdefault("cpu_windows_per_s", None)
+            rec.setdefault("bit_exact_vs_cpu", None)
+            cases.append(rec)
+            del dout
+    del m, s
+    head = next(c for c in cases if c["mode"] == "-time" and c["xsearch"] == 64)
+    same = [c["bit_exact_vs_cpu"] for c in cases if c["bit_exact_vs_cpu"] is not None]
+    cpu = None
+    if head["cpu_windows_per_s"]:
+        cpu = {"value": head["cpu_windows_per_s"], "unit": "patches/s", "cores": nthreads, "kind": "port",
+               "sample": "first %d windows of -time at xsearch 64 (FP32 CPU restatement, OpenMP over patches); "
+                         "every case's own sample is in cases" % head["cpu_sample"], **host}
+    return {"metric": METRIC, "value": head["windows_per_s"], "unit": "patches/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "int8 matrix-core limb products, exact integer sums", "data": "synthetic",
+            "config": {"workload": "time-domain search (-time, -time4) on the 10800 x 27648 cint16 strip pair, "
+                                   "-nx 100 -ny 400, -xsearch/-ysearch 32 (interp 8) and 64 (interp 16)",
+                       "patches": head["windows"]},
+            "cases": cases, "cpu_baseline": cpu,
+            "parity": {"sample": sum(c.get("cpu_sample", 0) for c in cases),
+                       "bit_exact_vs_cpu": all(same) if same else None}}
+
+
+def time_cpu_sample(args, xg, p, w, got, m, s, nthreads):
+    """The first windows of one time-mode case on the FP32 CPU restatement:
+    a probe of one window per thread sizes a sample of about a quarter of the
+    CPU budget; bit-exactness of the peaks, offsets and correlations."""
+    import oracle
+    oracle.lib()
+    op = oracle.make_params(p.xsearch, p.ysearch, ri=p.ri, interp_flag=p.interp_flag,
+                            interp_factor=p.interp_factor, corr_flag=p.corr_flag)
+
+    def run(cnt):
+        lo = max(int(min(w["my0"][:cnt].min(), w["sy0"][:cnt].min())), 0)
+        hi = int(max(w["my0"][:cnt].max(), w["sy0"][:cnt].max())) + p.npy
+        mh, sh = m[lo:hi].cpu().numpy(), s[lo:hi].cpu().numpy()
+        locs = np.stack([w["mx0"], w["my0"] - lo, w["sx0"], w["sy0"] - lo], 1).astype(np.int32)[:cnt]
+        tp = time.perf_counter()
+        ref = oracle.run(op, mh, sh, locs, precision=32, nthreads=nthreads)
+        return ref, time.perf_counter() - tp
+
+    probe = min(len(w), nthreads)
+    ref, ts = run(probe)
+    cnt = int(min(len(w), max(probe, (args.cpu_seconds / 4) / (ts / probe))))
+    if cnt > probe:
+        ref, ts = run(cnt)
+    same = True
+    for f in ("xoff", "yoff", "corr", "xfrac", "yfrac", "ip", "jp"):
+        a, b = got[f][:cnt], ref[f]
+        same &= bool(np.array_equal(a.view(np.uint32), b.view(np.uint32)) if a.dtype.kind == "f"
+                     else np.array_equal(a, b))
+    return {"cpu_sample": cnt, "cpu_windows_per_s": round(cnt / ts, 2), "bit_exact_vs_cpu": same}
 
 
 def dense_accuracy(xg, synth, field, work):
