@@ -167,7 +167,8 @@ def main():
                     help="strip workload only: skip the compact batched / proposal / dense records")
     ap.add_argument("--serial", action="store_true",
                     help="strip batches one after another on one stream (default: consecutive batches overlap on "
-                         "two contexts); rocprof kernel durations are then those of each kernel alone")
+                         "two contexts); rocprof kernel durations are then those of each kernel alone (strip and "
+                         "time workloads)")
     ap.add_argument("--small", action="store_true", help="the small workload (1024 x 2048, 1000 patches) instead")
     ap.add_argument("--workload", default="strip",
                     choices=("strip", "small", "batched", "proposal", "dense", "time"),
@@ -735,16 +736,17 @@ def bench_time(args, xg, synth, torch, dev):
             w = xg.make_windows(x, y, p.npx, p.npy)
             n = len(w)
             dout = torch.zeros(n * xg.RESULT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+            flags = xg.NO_SYNC | (xg.SERIAL if args.serial else 0)
             with xg.Engine(p) as eng:
                 eng.set_images(m, s)
                 for _ in range(args.warmup):
-                    eng.correlate(w, out=dout, flags=xg.NO_SYNC)
+                    eng.correlate(w, out=dout, flags=flags)
                 eng.sync()
                 torch.cuda.synchronize()
                 trace_marker(torch)
                 t1 = time.perf_counter()
                 for _ in range(args.steps):
-                    eng.correlate(w, out=dout, flags=xg.NO_SYNC)
+                    eng.correlate(w, out=dout, flags=flags)
                 eng.sync()
                 step_s = (time.perf_counter() - t1) / args.steps
                 trace_marker(torch)

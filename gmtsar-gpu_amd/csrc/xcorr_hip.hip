@@ -726,8 +726,9 @@ struct LineSum {
 // z = amp1 + i amp2 on the template columns, where some patch uses the line).
 // Work item b < blk_n (blocks in xcd_item order) covers kStripBlockLines
 // lines from global line blk[2 (b + blk_off) + 1] of strip blk[2 (b + blk_off)].
-// EDGE: the strips' windows may cross the image sides.
-template <int FMT, bool EDGE>
+// EDGE: the strips' windows may cross the image sides.  Without ROWFFT (the
+// time modes) amplitudes and line sums only.
+template <int FMT, bool EDGE, bool ROWFFT = true>
 __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 4) k_stripA256(DevParams P, const void* __restrict__ master,
                                                         const void* __restrict__ slave,
                                                         const StripDesc* __restrict__ strips,
@@ -774,7 +775,7 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 4) k_stripA256(DevParams 
       for (int q = 0; q < 8; q++) {
         const float2 a = make_float2(ea[q], oa[q]);
         if (act) st_nt(o + j + 16 * q, a);
-        tile[j + 16 * q] = a;                      // through the tile into column order
+        if (ROWFFT) tile[j + 16 * q] = a;          // through the tile into column order
         s.add(ea[q]);
         s.add(oa[q]);
       }
@@ -783,12 +784,12 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 4) k_stripA256(DevParams 
     }
     float m1v[16];                              // master amplitude of column j + 16 r
     wave_sync();
-    {
+    if (ROWFFT) {
       const float* m1 = (const float*)tile;
 #pragma unroll
       for (int r = 0; r < 16; r++) m1v[r] = m1[j + 16 * r];
+      wave_sync();                              // read before the slave's transforms reuse the tile
     }
-    wave_sync();                                // read before the slave's transforms reuse the tile
     {
       float ea[8], oa[8];
       interp_line256<FMT>(us, j, tile, twl, sph, ea, oa);
@@ -798,7 +799,7 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 4) k_stripA256(DevParams 
       for (int q = 0; q < 8; q++) {
         const float2 a = make_float2(ea[q], oa[q]);
         if (wm && q >= 2 && q < 6) st_nt(o + j + 16 * q, a);
-        tile[j + 16 * q] = a;                      // the tile doubles as the slave line buffer
+        if (ROWFFT) tile[j + 16 * q] = a;          // the tile doubles as the slave line buffer
         s.add(ea[q]);
         s.add(oa[q]);
       }
@@ -806,6 +807,7 @@ __global__ void __launch_bounds__(kThreads, EDGE ? 2 : 4) k_stripA256(DevParams 
       if (act && j == 15) lsum[2 * l + 1] = st;
     }
     wave_sync();                                // the slave line is in LDS
+    if (!ROWFFT) continue;                      // time modes: amplitudes and sums only
     const float* a2 = (const float*)tile;
     float2 v[16];
 #pragma unroll
@@ -2762,6 +2764,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
 // S(dy) overwrites it (int64 or 128-bit throughout, exact).
 template <int POW>
 __global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const float* __restrict__ amp,
+                                                             const int* __restrict__ rb,
                                                              const unsigned long long* __restrict__ sums,
                                                              double* __restrict__ aa, double* __restrict__ hi,
                                                              int nb) {
@@ -2771,7 +2774,8 @@ __global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const 
   if (b >= nb) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float m1f = (float)patch_mean(sums[2 * b], P.N);
-  const float* a1 = amp + (size_t)(2 * b) * P.N;
+  // amplitudes of patch b: rows of 256 from row base rb[b] of the strip lines, or per patch
+  const float* a1 = rb ? amp + (size_t)rb[b] * 256 : amp + (size_t)(2 * b) * P.N;
   double* ob = aa + (size_t)b * 128 * 128;
   long long* sb = (long long*)ob;
   unsigned long long* hb = POW == 2 ? nullptr : (unsigned long long*)(hi + (size_t)b * 128 * 128);
@@ -2805,29 +2809,41 @@ __global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const 
       if (NP == 2) row_e(qh, E + 32 * 128 + rr * 128);
     }
     __syncthreads();
-    if (threadIdx.x < 128)
-      for (int rr = 0; rr < 32; rr++) {
-        const int r = 32 * ch + rr;               // S holds S(r)
-        const int o = (r & 127) * 128 + col;
-        if (POW == 2) {
-          if (r < 128) {
-            sb[o] = (long long)S;
-          } else {
-            const long long s0 = sb[o];
-            ob[o] = (double)((long long)S - s0);
-          }
-        } else {
-          if (r < 128) {
-            sb[o] = (long long)(unsigned long long)S;
-            hb[o] = (unsigned long long)((u128)S >> 64);
-          } else {
-            const u128 s0 = ((u128)hb[o] << 64) | (unsigned long long)sb[o];
-            ob[o] = u128_to_double((u128)S - s0);
+    if (threadIdx.x < 128) {
+      // rows r >= 128 (chunks 4 ..) subtract S(r - 128), parked 128 rows
+      // earlier: the parked sums are loaded G rows at a time
+      constexpr int G = POW == 2 ? 16 : 8;
+      for (int r0 = 0; r0 < 32; r0 += G) {
+        unsigned long long pl[G], ph[G];
+        if (ch >= 4) {
+#pragma unroll
+          for (int q = 0; q < G; q++) {
+            const int o = ((32 * ch + r0 + q) & 127) * 128 + col;
+            pl[q] = (unsigned long long)sb[o];
+            ph[q] = POW == 2 ? 0ull : hb[o];
           }
         }
-        if constexpr (POW == 2) S += E[rr * 128 + col];
-        else S += ((u128)(unsigned long long)E[32 * 128 + rr * 128 + col] << 40) + (unsigned long long)E[rr * 128 + col];
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+          const int rr = r0 + q;                  // S holds S(32 ch + rr)
+          const int o = ((32 * ch + rr) & 127) * 128 + col;
+          if constexpr (POW == 2) {
+            if (ch < 4) sb[o] = S;
+            else ob[o] = (double)(S - (long long)pl[q]);
+            S += E[rr * 128 + col];
+          } else {
+            if (ch < 4) {
+              sb[o] = (long long)(unsigned long long)S;
+              hb[o] = (unsigned long long)(S >> 64);
+            } else {
+              ob[o] = u128_to_double(S - (((u128)ph[q] << 64) | pl[q]));
+            }
+            S += ((u128)(unsigned long long)E[32 * 128 + rr * 128 + col] << 40) +
+                 (unsigned long long)E[rr * 128 + col];
+          }
+        }
       }
+    }
     __syncthreads();
   }
 }
@@ -2842,7 +2858,9 @@ __global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const 
 // at most 3 x 4 x 4096 limb products of magnitude <= 2^12: still exact.  The
 // surface goes to surfd for the peak scan (k_time_peak4k); a patch with a
 // value outside the limb range takes the exact int64 loop.
-__global__ void __launch_bounds__(kThreads, 2) k_time_search_i8q(DevParams P, const float* __restrict__ amp,
+__global__ void __launch_bounds__(kThreads, 2) k_time_search_i8q(DevParams P, const float* __restrict__ A1,
+                                                              const float* __restrict__ A2,
+                                                              const int* __restrict__ rb,
                                                               const unsigned long long* __restrict__ sums,
                                                               const double* __restrict__ aa,
                                                               double* __restrict__ surfd, int nb) {
@@ -2856,8 +2874,9 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8q(DevParams P, co
   const int dyq = 64 * (q >> 1), dxq = 64 * (q & 1);
   const float m1f = (float)patch_mean(sums[2 * b], P.N);
   const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
-  const float* a1 = amp + (size_t)(2 * b) * P.N;
-  const float* a2 = a1 + P.N;
+  // amplitudes of patch b: rows of 256 from row base rb[b] of the strip lines, or per patch
+  const float* a1 = rb ? A1 + (size_t)rb[b] * 256 : A1 + (size_t)(2 * b) * P.N;
+  const float* a2 = rb ? A2 + (size_t)rb[b] * 256 : a1 + P.N;
   if (threadIdx.x < 4) {
     sbig[threadIdx.x] = 0;
     shi[threadIdx.x][0] = 0;
@@ -2946,7 +2965,9 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8q(DevParams P, co
 // 128-bit loop (k_time_search<1>'s sums).  The surface goes to surfd for the
 // peak scan (k_time_peak4k).
 template <int XS>
-__global__ void __launch_bounds__(kThreads, 1) k_time4_search_i8(DevParams P, const float* __restrict__ amp,
+__global__ void __launch_bounds__(kThreads, 1) k_time4_search_i8(DevParams P, const float* __restrict__ A1,
+                                                              const float* __restrict__ A2,
+                                                              const int* __restrict__ rb,
                                                               const unsigned long long* __restrict__ sums,
                                                               const double* __restrict__ aa,
                                                               double* __restrict__ surfd, int nb) {
@@ -2961,8 +2982,9 @@ __global__ void __launch_bounds__(kThreads, 1) k_time4_search_i8(DevParams P, co
   const int dyq = 64 * (q >> 1), dxq = 64 * (q & 1);
   const float m1f = (float)patch_mean(sums[2 * b], P.N);
   const float m2f = (float)patch_mean(sums[2 * b + 1], P.N);
-  const float* a1 = amp + (size_t)(2 * b) * P.N;
-  const float* a2 = a1 + P.N;
+  // amplitudes of patch b: rows of W from row base rb[b] of the strip lines, or per patch
+  const float* a1 = rb ? A1 + (size_t)rb[b] * W : A1 + (size_t)(2 * b) * P.N;
+  const float* a2 = rb ? A2 + (size_t)rb[b] * W : a1 + P.N;
   if (threadIdx.x < NQ) {
     sbig[threadIdx.x] = 0;
     shi[threadIdx.x][0] = 0;
@@ -3580,11 +3602,12 @@ extern "C" int xcorr_hip_create(const xcorr_hip_params* p, int max_batch, int de
   auto alloc = [&](void** ptr, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(ptr, bytes ? bytes : 16);
   };
-  // strips: square 128 / 256 / 512 windows in frequency mode, and -time on
-  // 128 x 128 windows (xs = ys = 32, the int8 search); ri 2
+  // strips: square 128 / 256 / 512 windows in frequency mode, and -time /
+  // -time4 on 128 x 128 and 256 x 256 windows (xs = ys = 32 or 64, the int8
+  // searches); ri 2
   c->strips = d.NX == d.NY && d.ri == 2 && phi_sym &&
               ((p->corr_flag == XCORR_HIP_FREQ && (d.NX == 128 || d.NX == 256 || d.NX == 512)) ||
-               (p->corr_flag == XCORR_HIP_TIME && p->xsearch == 32 && p->ysearch == 32));
+               time_i8_shape(p->corr_flag, p->xsearch, p->ysearch));
   if (c->strips) {
     // half the lines of unshared windows: overlapping windows need far fewer
     c->max_lines = std::max(max_batch * (d.NY / 2), d.NY + kStripBlockLines);
@@ -3960,15 +3983,34 @@ static void launch_colpass(xcorr_hip_ctx* c, const PatchBufs& b, const DevParams
                      rb, (const unsigned long long*)b.sums, b.D, (const float2*)c->tabs, nwork);
 }
 
-// The -time search of 128 x 128 windows (xs = ys = 32) on the int8 matrix
-// cores: sliding energies, then the search with its peak scan.  Amplitudes
-// per patch (rb null: A1 = A2 = amp) or on strip lines from row bases rb.
+// The -time / -time4 search of 128 x 128 and 256 x 256 windows (xs = ys = 32
+// or 64, time_i8_shape) on the int8 matrix cores: sliding energies, then the
+// search (-time on 128 x 128 windows with its peak scan), then, for the
+// others, the peak scan.  Amplitudes per patch (rb null: A1 = A2 = amp) or on
+// strip lines from row bases rb.
 static void launch_time_i8(const PatchBufs& b, const DevParams& d, hipStream_t st, const float* A1, const float* A2,
                            const int* rb, int nb) {
-  hipLaunchKernelGGL(k_time_energy32<2>, dim3((nb + 7) & ~7), dim3(kThreads), 0, st, d, A1, rb,
-                     (const unsigned long long*)b.sums, b.tcaa, nb);
-  hipLaunchKernelGGL(k_time_search_i8, dim3((nb + 7) & ~7), dim3(kThreads), 0, st, d, A1, A2, rb,
-                     (const unsigned long long*)b.sums, (const double*)b.tcaa, b.surfd, b.tpeak, b.tcave, nb);
+  const dim3 g1((nb + 7) & ~7), g4((4 * nb + 7) & ~7), blk(kThreads);
+  const unsigned long long* sums = b.sums;
+  if (d.corr_flag == XCORR_HIP_TIME && d.xs == 32) {
+    hipLaunchKernelGGL(k_time_energy32<2>, g1, blk, 0, st, d, A1, rb, sums, b.tcaa, nb);
+    hipLaunchKernelGGL(k_time_search_i8, g1, blk, 0, st, d, A1, A2, rb, sums, (const double*)b.tcaa, b.surfd,
+                       b.tpeak, b.tcave, nb);
+    return;
+  }
+  if (d.corr_flag == XCORR_HIP_TIME) {
+    hipLaunchKernelGGL(k_time_energy256<2>, g1, blk, 0, st, d, A1, rb, sums, b.tcaa, (double*)nullptr, nb);
+    hipLaunchKernelGGL(k_time_search_i8q, g4, blk, 0, st, d, A1, A2, rb, sums, (const double*)b.tcaa, b.surfd, nb);
+  } else if (d.xs == 32) {
+    hipLaunchKernelGGL(k_time_energy32<4>, g1, blk, 0, st, d, A1, rb, sums, b.tcaa, nb);
+    hipLaunchKernelGGL(k_time4_search_i8<32>, g1, blk, 0, st, d, A1, A2, rb, sums, (const double*)b.tcaa, b.surfd,
+                       nb);
+  } else {
+    hipLaunchKernelGGL(k_time_energy256<4>, g1, blk, 0, st, d, A1, rb, sums, b.tcaa, b.surfd, nb);
+    hipLaunchKernelGGL(k_time4_search_i8<64>, g4, blk, 0, st, d, A1, A2, rb, sums, (const double*)b.tcaa, b.surfd,
+                       nb);
+  }
+  hipLaunchKernelGGL(k_time_peak4k, dim3(nb), blk, 0, st, d, (const double*)b.surfd, b.tpeak, b.tcave);
 }
 
 // Kernels of one strip batch whose plan is in device memory at d_plan, on
@@ -3991,15 +4033,18 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   });
   if (rc) return rc;
   const bool ci = d.format == XCORR_HIP_CINT16;
-  const bool freq = d.corr_flag == XCORR_HIP_FREQ;    // else -time on 128 x 128 windows
+  const bool freq = d.corr_flag == XCORR_HIP_FREQ;    // else -time / -time4 on 128 x 128 or 256 x 256 windows
   rc = launch_timed(c, timing, d.NX == 256 ? "k_stripA256" : "k_stripA", [&] {
     using KA = void (*)(DevParams, const void*, const void*, const StripDesc*, const int*, int, int,
                         const unsigned char*, const unsigned char*, float*, float*, long long*, float2*,
                         const float2*);
     KA kin, kedge;
-    if (d.NX == 256) {
+    if (d.NX == 256 && freq) {
       kin = ci ? k_stripA256<XCORR_HIP_CINT16, false> : k_stripA256<XCORR_HIP_FLOAT32, false>;
       kedge = ci ? k_stripA256<XCORR_HIP_CINT16, true> : k_stripA256<XCORR_HIP_FLOAT32, true>;
+    } else if (d.NX == 256) {
+      kin = ci ? k_stripA256<XCORR_HIP_CINT16, false, false> : k_stripA256<XCORR_HIP_FLOAT32, false, false>;
+      kedge = ci ? k_stripA256<XCORR_HIP_CINT16, true, false> : k_stripA256<XCORR_HIP_FLOAT32, true, false>;
     } else if (d.NX == 128 && !freq) {
       kin = ci ? k_stripA<128, XCORR_HIP_CINT16, false, false> : k_stripA<128, XCORR_HIP_FLOAT32, false, false>;
       kedge = ci ? k_stripA<128, XCORR_HIP_CINT16, true, false> : k_stripA<128, XCORR_HIP_FLOAT32, true, false>;
@@ -4026,7 +4071,7 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
   });
   if (rc) return rc;
   if (!freq) {
-    rc = launch_timed(c, timing, "k_time_search", [&] {
+    rc = launch_timed(c, timing, d.corr_flag == XCORR_HIP_TIME4 ? "k_time_search4" : "k_time_search", [&] {
       launch_time_i8(ctx_bufs(c), d, st, (const float*)c->amp1s, (const float*)c->amp2s, d_rb, nb);
     });
   } else {
@@ -4320,37 +4365,12 @@ static int time_search(xcorr_hip_ctx* c, const PatchBufs& b, int nb, bool timing
   constexpr int TL = 8;                           // lags per thread
   const int LRb = std::max(1, std::min(d.nyc, kThreads * TL / d.nxc));
   const int RCb = std::min(8, (5120 - (LRb - 1) * d.NX) / (d.NX + d.nx_corr));   // <= 40 KB of LDS
-  const bool i8 = d.corr_flag == XCORR_HIP_TIME && d.xs == 32 && d.ys == 32 && b.tcaa;   // peak found in-kernel
-  const bool i8q = d.corr_flag == XCORR_HIP_TIME && d.xs == 64 && d.ys == 64 && b.tcaa;  // 256 x 256 windows
-  const bool i8g = d.corr_flag == XCORR_HIP_TIME4 && time_i8_shape(d.corr_flag, d.xs, d.ys) && b.tcaa;
-  if (i8g)
-    rc = launch_timed(c, timing, "k_time_search4", [&] {
-      if (d.xs == 32) {
-        hipLaunchKernelGGL(k_time_energy32<4>, dim3((nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
-                           (const float*)b.amp, (const int*)nullptr, (const unsigned long long*)b.sums, b.tcaa, nb);
-        hipLaunchKernelGGL(k_time4_search_i8<32>, dim3((nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
-                           (const float*)b.amp, (const unsigned long long*)b.sums, (const double*)b.tcaa, b.surfd,
-                           nb);
-      } else {
-        hipLaunchKernelGGL(k_time_energy256<4>, dim3((nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
-                           (const float*)b.amp, (const unsigned long long*)b.sums, b.tcaa, b.surfd, nb);
-        hipLaunchKernelGGL(k_time4_search_i8<64>, dim3((4 * nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
-                           (const float*)b.amp, (const unsigned long long*)b.sums, (const double*)b.tcaa, b.surfd,
-                           nb);
-      }
-    });
-  else if (i8)
-    rc = launch_timed(c, timing, "k_time_search", [&] {
+  // 128 x 128 and 256 x 256 windows: the matrix-core searches, peak scan included
+  if (time_i8_shape(d.corr_flag, d.xs, d.ys) && b.tcaa)
+    return launch_timed(c, timing, d.corr_flag == XCORR_HIP_TIME4 ? "k_time_search4" : "k_time_search", [&] {
       launch_time_i8(b, d, c->stream, (const float*)b.amp, (const float*)b.amp, nullptr, nb);
     });
-  else if (i8q)
-    rc = launch_timed(c, timing, "k_time_search", [&] {
-      hipLaunchKernelGGL(k_time_energy256<2>, dim3((nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
-                         (const float*)b.amp, (const unsigned long long*)b.sums, b.tcaa, (double*)nullptr, nb);
-      hipLaunchKernelGGL(k_time_search_i8q, dim3((4 * nb + 7) & ~7), dim3(kThreads), 0, c->stream, d,
-                         (const float*)b.amp, (const unsigned long long*)b.sums, (const double*)b.tcaa, b.surfd, nb);
-    });
-  else if (d.corr_flag == XCORR_HIP_TIME && d.nxc % TL == 0 && RCb >= 1)
+  if (d.corr_flag == XCORR_HIP_TIME && d.nxc % TL == 0 && RCb >= 1)
     rc = launch_timed(c, timing, "k_time_search", [&] {
       const size_t shb = ((size_t)(RCb + LRb - 1) * d.NX + (size_t)RCb * d.nx_corr) * sizeof(double);
       hipLaunchKernelGGL(k_time_search_blk<TL>, dim3(nb * (d.nyc / LRb)), dim3(kThreads), shb, c->stream, d,
@@ -4366,7 +4386,7 @@ static int time_search(xcorr_hip_ctx* c, const PatchBufs& b, int nb, bool timing
       hipLaunchKernelGGL(k_time_search<1>, grid, dim3(kThreads), sh, c->stream, d, (const float*)b.amp,
                          (const unsigned long long*)b.sums, b.surfd);
     });
-  if (rc || i8) return rc;
+  if (rc) return rc;
   return launch_timed(c, timing, "k_time_peak", [&] {
     if ((d.nxc * d.nyc) % 4096 == 0)
       hipLaunchKernelGGL(k_time_peak4k, dim3(nb), dim3(kThreads), 0, c->stream, d, (const double*)b.surfd, b.tpeak,
