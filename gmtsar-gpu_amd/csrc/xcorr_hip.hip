@@ -1935,9 +1935,10 @@ __global__ void __launch_bounds__(kThreads) k_finish(DevParams P, const float* _
       const long k = (long)(ip - 4 + iy) * nxc + (jp - 4 + ix);
       float q = 0.0f;
       if (k >= 0 && k < (long)nxc * nyc) {
-        // nbr: the row kernel left only this neighbourhood, at the start of the slot (NbrWindow)
-        const float v = freq ? (float)((double)surf[(size_t)b * nxc * nyc + (nbr ? threadIdx.x : k)] / s_msc)
-                             : (float)surfd[(size_t)b * nxc * nyc + k];
+        // nbr: the row kernel (the fused -time search) left only this
+        // neighbourhood, at the start of the slot (NbrWindow)
+        const size_t e = (size_t)b * nxc * nyc + (nbr ? threadIdx.x : k);
+        const float v = freq ? (float)((double)surf[e] / s_msc) : (float)surfd[e];
         q = sqrtf(sqrtf(v));
       }
       md[threadIdx.x] = q;
@@ -2663,7 +2664,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
         }
       const double den = sqrt(1.0 * ab_aa[lag] * d2d);
       const double gv = (den == 0.0) ? 0.0 : 100.0 * fabs((double)num / den);
-      sd[lag] = gv;
+      if (P.keep_surf) sd[lag] = gv;
       gs[lag + (lag >> 4)] = gv;
     }
   } else {
@@ -2699,7 +2700,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
         const int lag = dy * 64 + dx;
         const double den = sqrt(1.0 * ab_aa[lag] * d2d);
         const double gv = (den == 0.0) ? 0.0 : 100.0 * fabs((double)num / den);
-        sd[lag] = gv;
+        if (P.keep_surf) sd[lag] = gv;
         gs[lag + (lag >> 4)] = gv;
       }
   }
@@ -2740,6 +2741,14 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
                                         : (wmax[1] > wmax[2] ? (wmax[1] > wmax[3] ? wmax[1] : wmax[3])
                                                              : (wmax[2] > wmax[3] ? wmax[2] : wmax[3]));
   const double ssum = block_sum_f64(sm, (double*)redk);
+  // Without keep, the slot gets only the 8 x 8 neighbourhood of the peak that
+  // k_finish reads (flat index k = (ip - 4 + iy) 64 + jp - 4 + ix, as
+  // d_cppow_strip, conv_ampl_demean.cu:75-95), entry (iy, ix) at iy * 8 + ix.
+  if (!P.keep_surf && t < 64) {
+    const int idx = (int)key - 1;
+    const int k = (idx / 64 - 4 + (t >> 3)) * 64 + idx % 64 - 4 + (t & 7);
+    sd[t] = (k >= 0 && k < 64 * 64) ? gs[k + (k >> 4)] : 0.0;
+  }
   if (t == 0) {
     const int idx = (int)key - 1;
     tpeak[4 * b + 0] = idx / 64;
@@ -3983,6 +3992,12 @@ static void launch_colpass(xcorr_hip_ctx* c, const PatchBufs& b, const DevParams
                      rb, (const unsigned long long*)b.sums, b.D, (const float2*)c->tabs, nwork);
 }
 
+// The fused -time search of 128 x 128 windows leaves only the peak's
+// neighbourhood in the surface slot (unless the surface is kept).
+static bool time_nbr(const DevParams& d, const PatchBufs& b) {
+  return d.corr_flag == XCORR_HIP_TIME && d.xs == 32 && d.ys == 32 && b.tcaa;
+}
+
 // The -time / -time4 search of 128 x 128 and 256 x 256 windows (xs = ys = 32
 // or 64, time_i8_shape) on the int8 matrix cores: sliding energies, then the
 // search (-time on 128 x 128 windows with its peak scan), then, for the
@@ -4105,7 +4120,7 @@ static int launch_strip_batch(xcorr_hip_ctx* c, const DevParams& d, const void* 
                        (const unsigned long long*)c->sums, (const float*)c->surf, (const double*)c->surfd,
                        (const unsigned long long*)c->peak, (const double*)c->partial, (const int*)c->tpeak,
                        (const double*)c->tcave, dev_out ? out : c->d_out, (const float2*)c->tabs, nb,
-                       (int)(freq && !d.keep_surf));
+                       (int)((freq || time_nbr(d, ctx_bufs(c))) && !d.keep_surf));
   });
 }
 
@@ -4350,7 +4365,7 @@ static int run_batch(xcorr_hip_ctx* c, const PatchBufs& b, const xcorr_hip_windo
                        (const unsigned long long*)b.sums, (const float*)b.surf, (const double*)b.surfd,
                        (const unsigned long long*)b.peak, (const double*)b.partial, (const int*)b.tpeak,
                        (const double*)b.tcave, d_out, (const float2*)c->tabs, nb,
-                       (int)(freq && (regA || row256) && !d.keep_surf));
+                       (int)(((freq && (regA || row256)) || time_nbr(d, b)) && !d.keep_surf));
   });
   return rc;
 }
