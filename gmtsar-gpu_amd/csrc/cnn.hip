@@ -59,24 +59,40 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
 // ---------------------------------------------------------------------------
 // Tile functions (shared by the layer kernels and the fused inference kernel)
 // ---------------------------------------------------------------------------
+// conv1's im2col for one lane: lane (m, g) supplies K values 8 g .. 8 g + 7,
+// i.e. taps 4 g .. 4 g + 3 with both input channels of a tap adjacent (one
+// dword of x0); taps past 8 (K >= 18) are zero.  Per tap: the input offset
+// (ky - 1, kx - 1), or off.
+struct Conv1Lane {
+  int dy[4], dx[4];
+  bool on[4];
+};
+__device__ __forceinline__ Conv1Lane conv1_lane(int lane) {
+  Conv1Lane L;
+#pragma unroll
+  for (int jj = 0; jj < 4; jj++) {
+    const int tap = 4 * (lane >> 4) + jj;
+    L.on[jj] = tap < 9;
+    L.dy[jj] = tap / 3 - 1;
+    L.dx[jj] = tap - 3 * (tap / 3) - 1;
+  }
+  return L;
+}
+
 // conv1 (2 -> 32, stride 2, 64 x 64 -> 32 x 32): output pixels p0 .. p0 + 15 of
-// one sample; xb = the sample's x0[64][64][2].
-__device__ __forceinline__ void conv1_tile(const bf16_t* xb, const bf16_t* W1k, int p0, int lane, f32x4 (&acc)[2]) {
+// one sample; xb = the sample's x0[64][64][2] (4-byte aligned).
+__device__ __forceinline__ void conv1_tile(const bf16_t* xb, const bf16_t* W1k, int p0, int lane, const Conv1Lane& L,
+                                           f32x4 (&acc)[2]) {
   const int m = lane & 15, g = lane >> 4;
   const int p = p0 + m, oy = p >> 5, ox = p & 31;
-  unsigned short av[8];
+  unsigned av[4];
 #pragma unroll
-  for (int j = 0; j < 8; j++) {
-    const int k = 8 * g + j;
-    unsigned short v = 0;
-    if (k < 18) {
-      const int tap = k >> 1, ci = k & 1, ky = tap / 3, kx = tap - 3 * ky;
-      const int iy = 2 * oy + ky - 1, ix = 2 * ox + kx - 1;
-      if (iy >= 0 && iy < 64 && ix >= 0 && ix < 64) v = xb[(iy * 64 + ix) * 2 + ci];
-    }
-    av[j] = v;
+  for (int jj = 0; jj < 4; jj++) {
+    const int iy = 2 * oy + L.dy[jj], ix = 2 * ox + L.dx[jj];
+    av[jj] = (L.on[jj] && iy >= 0 && iy < 64 && ix >= 0 && ix < 64)
+                 ? *(const unsigned*)(xb + (iy * 64 + ix) * 2) : 0u;
   }
-  const bf16x8 a = __builtin_bit_cast(bf16x8, *(const uint4*)av);
+  const bf16x8 a = __builtin_bit_cast(bf16x8, make_uint4(av[0], av[1], av[2], av[3]));
   acc[0] = mfma(a, ld8(W1k + m * 32 + 8 * g), f32x4{0.f, 0.f, 0.f, 0.f});
   acc[1] = mfma(a, ld8(W1k + (16 + m) * 32 + 8 * g), f32x4{0.f, 0.f, 0.f, 0.f});
 }
@@ -127,16 +143,30 @@ __device__ __forceinline__ void load_w(const bf16_t* Wf, int lane, bf16x8 (&bw)[
   }
 }
 
-// bias + ReLU + bf16 store of a conv tile: y[(p0 + 4g + i) * 32 + co]
+// bias + ReLU + bf16 store of a conv tile: y[(p0 + 4g + i) * 32 + co], lane
+// (m, g) holding channel co = 16 t + m of pixels p0 + 4g + i.  Lanes m and m ^ 1
+// swap one value (a DPP quad permutation) so that each stores a dword of two
+// channels: even lanes channels (m, m + 1) of pixel i, odd lanes channels
+// (m - 1, m) of pixel i + 1, for i = 0, 2.
 __device__ __forceinline__ void store_tile(bf16_t* yb, const float* bias, int p0, int lane, const f32x4 (&acc)[2]) {
   const int m = lane & 15, g = lane >> 4;
+  const bool odd = m & 1;
 #pragma unroll
   for (int t = 0; t < 2; t++) {
     const float bv = bias[16 * t + m];
+    unsigned h[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const float z = acc[t][i] + bv;
-      yb[(p0 + 4 * g + i) * kC + 16 * t + m] = f2bf(z > 0.f ? z : 0.f);
+      h[i] = f2bf(z > 0.f ? z : 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i += 2) {
+      const unsigned send = odd ? h[i] : h[i + 1];
+      const unsigned got = (unsigned)__builtin_amdgcn_mov_dpp((int)send, 0xB1, 0xf, 0xf, false);   // quad_perm 1,0,3,2
+      const unsigned wv = odd ? (got | (h[i + 1] << 16)) : (h[i] | (got << 16));
+      const int px = p0 + 4 * g + i + (odd ? 1 : 0), co = 16 * t + m - (odd ? 1 : 0);
+      *(unsigned*)(yb + px * kC + co) = wv;
     }
   }
 }
@@ -170,7 +200,7 @@ __global__ void __launch_bounds__(256) k_conv1_fwd(const bf16_t* __restrict__ x0
   if (tile >= n * 64) return;                 // 1024 pixels = 64 tiles per sample
   const int b = tile >> 6, p0 = (tile & 63) * 16;
   f32x4 acc[2];
-  conv1_tile(x0 + (size_t)b * 64 * 64 * 2, q + kQ1, p0, lane, acc);
+  conv1_tile(x0 + (size_t)b * 64 * 64 * 2, q + kQ1, p0, lane, conv1_lane(lane), acc);
   store_tile(y + (size_t)b * 1024 * kC, prm + kB1, p0, lane, acc);
 }
 
@@ -388,48 +418,72 @@ __global__ void k_box4(const void* __restrict__ img, int nx, int ny, int fmt, fl
   out[e] = s * (1.0f / 16.0f);
 }
 
-// Normalised 64 x 64 crop of one channel into dst[(y * 64 + x) * 2 + ch]
-// (block of 256 threads; red: 8 floats of shared scratch).
-__device__ void crop_norm(const float* __restrict__ ds, int dnx, int dny, int x0, int y0, bf16_t* dst, int ch,
-                          float* red) {
-  float v[16];
-  float s = 0.f;
+// Normalised 64 x 64 crops of the master (channel 0, corner (xm, ym) of dsm)
+// and the slave (channel 1, corner (xs, ys) of dss) into dst[(y * 64 + x) * 2 +
+// ch] (4-byte aligned; block of 256 threads; red: 8 floats of shared scratch).
+// Each channel is normalised by its own mean and standard deviation; the two
+// channels' loads are in flight together, their reductions share barriers,
+// and a pixel's two values go out as one dword.
+__device__ void crop_norm2(const float* __restrict__ dsm, int dnx_m, int dny_m, int xm, int ym,
+                           const float* __restrict__ dss, int dnx_s, int dny_s, int xs, int ys, bf16_t* dst,
+                           float* red) {
+  float v[16], w[16];
+  float s0 = 0.f, s1 = 0.f;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    const int e = threadIdx.x + 256 * k, y = e >> 6, x = e & 63, gy = y0 + y, gx = x0 + x;
-    v[k] = (gy >= 0 && gy < dny && gx >= 0 && gx < dnx) ? ds[(size_t)gy * dnx + gx] : 0.f;
-    s += v[k];
+    const int e = threadIdx.x + 256 * k, y = e >> 6, x = e & 63;
+    const int gy = ym + y, gx = xm + x, hy = ys + y, hx = xs + x;
+    v[k] = (gy >= 0 && gy < dny_m && gx >= 0 && gx < dnx_m) ? dsm[(size_t)gy * dnx_m + gx] : 0.f;
+    w[k] = (hy >= 0 && hy < dny_s && hx >= 0 && hx < dnx_s) ? dss[(size_t)hy * dnx_s + hx] : 0.f;
   }
-  auto bsum = [&](float a) {
-    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    s0 += v[k];
+    s1 += w[k];
+  }
+  auto bsum2 = [&](float a, float b) {
+    for (int o = 32; o > 0; o >>= 1) {
+      a += __shfl_xor(a, o);
+      b += __shfl_xor(b, o);
+    }
     __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    if ((threadIdx.x & 63) == 0) {
+      red[threadIdx.x >> 6] = a;
+      red[4 + (threadIdx.x >> 6)] = b;
+    }
     __syncthreads();
-    return red[0] + red[1] + red[2] + red[3];
+    return make_float2(red[0] + red[1] + red[2] + red[3], red[4] + red[5] + red[6] + red[7]);
   };
-  const float mean = bsum(s) * (1.0f / 4096.0f);
-  float q = 0.f;
+  const float2 sm = bsum2(s0, s1);
+  const float mean0 = sm.x * (1.0f / 4096.0f), mean1 = sm.y * (1.0f / 4096.0f);
+  float q0 = 0.f, q1 = 0.f;
 #pragma unroll
-  for (int k = 0; k < 16; k++) q += (v[k] - mean) * (v[k] - mean);
-  const float sd = sqrtf(bsum(q) * (1.0f / 4096.0f));
-  const float den = fmaxf(sd, 1e-6f);
+  for (int k = 0; k < 16; k++) {
+    q0 += (v[k] - mean0) * (v[k] - mean0);
+    q1 += (w[k] - mean1) * (w[k] - mean1);
+  }
+  const float2 sq = bsum2(q0, q1);
+  const float den0 = fmaxf(sqrtf(sq.x * (1.0f / 4096.0f)), 1e-6f);
+  const float den1 = fmaxf(sqrtf(sq.y * (1.0f / 4096.0f)), 1e-6f);
 #pragma unroll
-  for (int k = 0; k < 16; k++) dst[(threadIdx.x + 256 * k) * 2 + ch] = f2bf((v[k] - mean) / den);
+  for (int k = 0; k < 16; k++)
+    *(unsigned*)(dst + (threadIdx.x + 256 * k) * 2) =
+        (unsigned)f2bf((v[k] - mean0) / den0) | ((unsigned)f2bf((w[k] - mean1) / den1) << 16);
 }
 
 __global__ void __launch_bounds__(256) k_crops(const float* __restrict__ dsm, int dnx_m, int dny_m,
                                                const float* __restrict__ dss, int dnx_s, int dny_s,
                                                const int4* __restrict__ corners, bf16_t* __restrict__ x0) {
-  __shared__ float red[4];
+  __shared__ float red[8];
   const int b = blockIdx.x;
   const int4 c = corners[b];
-  bf16_t* dst = x0 + (size_t)b * 64 * 64 * 2;
-  crop_norm(dsm, dnx_m, dny_m, c.x, c.y, dst, 0, red);
-  crop_norm(dss, dnx_s, dny_s, c.z, c.w, dst, 1, red);
+  crop_norm2(dsm, dnx_m, dny_m, c.x, c.y, dss, dnx_s, dny_s, c.z, c.w, x0 + (size_t)b * 64 * 64 * 2, red);
 }
 
 // Fused inference: one block (4 waves) per sample, activations in LDS.
-// R0 (16 KB): x0, then act2, then act4.  R1 (64 KB): act1, then act3.
+// R0 (16 KB): x0, then act2, then act4.  R1 (64 KB): the crop's reduction
+// scratch, act1, act3, then the head's channel means.  80 KB in all, so two
+// blocks share a CU.
 template <bool CROP>
 __global__ void __launch_bounds__(256) k_infer(const bf16_t* __restrict__ x0g, const float* __restrict__ dsm,
                                                int dnx_m, int dny_m, const float* __restrict__ dss, int dnx_s,
@@ -438,21 +492,21 @@ __global__ void __launch_bounds__(256) k_infer(const bf16_t* __restrict__ x0g, c
                                                float* __restrict__ out) {
   __shared__ __attribute__((aligned(16))) bf16_t R0[64 * 64 * 2];
   __shared__ __attribute__((aligned(16))) bf16_t R1[32 * 32 * kC];
-  __shared__ float red[4];
-  __shared__ float fmean[kC];
+  float* red = (float*)R1;                                  // crop_norm2 scratch, before act1
+  float* fmean = (float*)R1;                                // after conv4 has read act3
   const int b = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (CROP) {
     const int4 c = corners[b];
-    crop_norm(dsm, dnx_m, dny_m, c.x, c.y, R0, 0, red);
-    crop_norm(dss, dnx_s, dny_s, c.z, c.w, R0, 1, red);
+    crop_norm2(dsm, dnx_m, dny_m, c.x, c.y, dss, dnx_s, dny_s, c.z, c.w, R0, red);
   } else {
     const uint4* src = (const uint4*)(x0g + (size_t)b * 64 * 64 * 2);
     for (int e = threadIdx.x; e < 64 * 64 * 2 / 8; e += 256) ((uint4*)R0)[e] = src[e];
   }
   __syncthreads();
   f32x4 acc[2];
+  const Conv1Lane L1 = conv1_lane(lane);
   for (int t = wave; t < 64; t += 4) {                      // conv1 -> R1 (32 x 32 x 32)
-    conv1_tile(R0, q + kQ1, t * 16, lane, acc);
+    conv1_tile(R0, q + kQ1, t * 16, lane, L1, acc);
     store_tile(R1, prm + kB1, t * 16, lane, acc);
   }
   bf16x8 bw[9][2];                                          // a layer's B fragments, loaded once per wave
