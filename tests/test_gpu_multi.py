@@ -9,6 +9,9 @@
 * A multi-pair workload (mixed window sizes, interpolation factors, a time-
   domain class, the 256 x 256 strip class) captured as one HIP graph gives,
   replay after replay, exactly the results of one engine per pair.
+* The matrix-core time searches (-time and -time4 on 128 x 128 and 256 x 256
+  windows, all on strip lines) as classes of one graph: every replay equals
+  one engine per pair and the FP32 restatement.
 """
 import numpy as np
 import pytest
@@ -234,3 +237,37 @@ def test_multi_tuned_matches_oracle(xg, orc, torch, pairs):
                 ref = orc.run(_oracle_params(orc, p), m_h, s_h, _locs(q["windows"]), precision=32)
             _same(got, ref, "tuned pair %d %s" % (i, t), fields=FIELDS[:-1])
             np.testing.assert_allclose(got["cave"], ref["cave"], rtol=1e-12)
+
+
+def test_multi_time_classes_match_engines_and_oracle(xg, orc, torch, pairs):
+    dev = torch.device("cuda:0")
+    spec = [("c", 64, 64, 16, xg.TIME, 3, 3), ("b", 32, 32, 8, xg.TIME4, 5, 4), ("c", 64, 64, 16, xg.TIME4, 2, 3),
+            ("a", 32, 32, 8, xg.TIME, 4, 5)]
+    work = []
+    for key, xs, ys, ifc, cf, nxl, nyl in spec:
+        m_h, s_h = pairs[key]
+        p = xg.make_params(xs, ys, interp_factor=ifc, corr_flag=cf)
+        w = _windows(xg, p, m_h.shape[1], m_h.shape[0], nxl, nyl)
+        res = torch.zeros(len(w) * xg.RESULT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        work.append(dict(params=p, master=torch.from_numpy(m_h).to(dev), slave=torch.from_numpy(s_h).to(dev),
+                         windows=w, out=res, host=(m_h, s_h)))
+    expect = []
+    for q in work:
+        with xg.Engine(q["params"], max_batch=4) as eng:
+            eng.set_images(*q["host"])
+            expect.append(eng.correlate(q["windows"]))
+        ref = orc.run(_oracle_params(orc, q["params"]), *q["host"], _locs(q["windows"]), precision=32)
+        _same(expect[-1], ref, "engine vs restatement, class xs%d corr%d" % (q["params"].xsearch,
+                                                                          q["params"].corr_flag), fields=FIELDS[:-1])
+    with xg.Multi(work) as mu:
+        classes, pc = mu.classes()
+        assert len(classes) == 4
+        for rep in range(3):
+            for q in work:
+                q["out"].fill_(0xff)
+            torch.cuda.synchronize()
+            mu.run()
+            torch.cuda.synchronize()
+            for i, q in enumerate(work):
+                got = q["out"].cpu().numpy().view(xg.RESULT_DTYPE)
+                _same(got, expect[i], "replay %d pair %d" % (rep, i))
