@@ -43,6 +43,16 @@ def _exact(got, ref):
     np.testing.assert_allclose(got["cave"], ref["cave"], rtol=1e-12)
 
 
+def _template_ints(a2, m2f, xs, ys=None):
+    """The template's integers (int)(a - m2) (C truncation) in a window-sized
+    array, zero elsewhere: strip lines and the time path leave the slave
+    amplitudes outside the template unset."""
+    ys = xs if ys is None else ys
+    i2 = np.zeros(a2.shape, np.int64)
+    i2[ys:3 * ys, xs:3 * xs] = (a2[ys:3 * ys, xs:3 * xs] - m2f).astype(np.int64)
+    return i2
+
+
 @pytest.mark.parametrize("xs,ys", [(8, 8), (16, 8), (16, 32), (32, 32)])
 @pytest.mark.parametrize("corr_flag", [0, 1], ids=["time", "time4"])
 def test_time_bit_exact(xg, orc, pair, xs, ys, corr_flag):
@@ -56,7 +66,7 @@ def test_time_surface_matches_int64_bruteforce(xg, orc, pair):
     p, got, ref, surf, (a1, a2, means) = _run(xg, orc, pair, 8, 8, 0, nxl=2, nyl=1)
     m1f, m2f = np.float32(means[0]), np.float32(means[1])
     i1 = (a1 - m1f).astype(np.int64)            # C truncation toward zero
-    i2 = (a2 - m2f).astype(np.int64)
+    i2 = _template_ints(a2, m2f, 8)
     xs = ys = 8
     b = i2[ys:ys + 2 * ys, xs:xs + 2 * xs]
     brute = np.zeros((2 * ys, 2 * xs))
@@ -83,7 +93,7 @@ def test_time_surface_xs32_matches_int64_bruteforce(xg, orc, pair):
     _exact(got, ref)
     m1f, m2f = np.float32(means[0]), np.float32(means[1])
     i1 = (a1 - m1f).astype(np.int64)
-    i2 = (a2 - m2f).astype(np.int64)
+    i2 = _template_ints(a2, m2f, 32)
     b = i2[32:96, 32:96]
     win = sliding_window_view(i1[:127, :127], (64, 64))          # (64, 64, 64, 64): lag (dy, dx), then (y, x)
     num = np.einsum("ijkl,kl->ij", win, b)
@@ -120,7 +130,7 @@ def test_time_xs32_limb_counts(xg, orc, pair, fm, fs, left):
     _exact(got, ref)
     m1f, m2f = np.float32(means[0]), np.float32(means[1])
     i1 = (a1 - m1f).astype(np.int64)
-    i2 = (a2 - m2f).astype(np.int64)
+    i2 = _template_ints(a2, m2f, 32)
     print("max |v| master %d template %d" % (np.abs(i1).max(), np.abs(i2[32:96, 32:96]).max()))
     b = i2[32:96, 32:96]
     win = sliding_window_view(i1[:127, :127], (64, 64))
