@@ -54,8 +54,8 @@ def _windows(xg, n, npx, npy, seed, m_nx=1536, m_ny=2048):
     return w
 
 
-@pytest.mark.parametrize("xs,corr", [(64, 2), (16, 2), (32, 0), (64, 0), (32, 1)],
-                         ids=["strips", "per_patch", "time", "time256", "time4"])
+@pytest.mark.parametrize("xs,corr", [(64, 2), (16, 2), (32, 0), (64, 0), (32, 1), (64, 1), (16, 1)],
+                         ids=["strips", "per_patch", "time", "time256", "time4", "time4_256", "time4_per_patch"])
 def test_empty_call(xg, pair, xs, corr):
     import torch
     m, s = pair
@@ -110,7 +110,7 @@ def test_largest_range_interpolation(xg, orc, pair, ri):
     _same(got, orc.run(_oracle_params(orc, p), m, s, _locs(w), precision=32), "ri %d" % ri)
 
 
-@pytest.mark.parametrize("xs,corr", [(64, 2), (16, 2), (32, 0), (64, 0)])
+@pytest.mark.parametrize("xs,corr", [(64, 2), (16, 2), (32, 0), (64, 0), (32, 1), (64, 1)])
 def test_high_res_factor_one(xg, orc, pair, xs, corr):
     m, s = pair
     p = xg.make_params(xs, xs, interp_factor=1, corr_flag=corr)
@@ -121,7 +121,7 @@ def test_high_res_factor_one(xg, orc, pair, xs, corr):
     _same(got, orc.run(_oracle_params(orc, p), m, s, _locs(w), precision=32), "interp 1, xs %d" % xs)
 
 
-@pytest.mark.parametrize("xs,corr", [(64, 2), (32, 2), (16, 2), (32, 0), (64, 0)])
+@pytest.mark.parametrize("xs,corr", [(64, 2), (32, 2), (16, 2), (32, 0), (64, 0), (32, 1), (64, 1)])
 def test_zero_images(xg, orc, xs, corr):
     if not gpu_present():
         pytest.skip("no GPU")
@@ -136,11 +136,13 @@ def test_zero_images(xg, orc, xs, corr):
     assert np.all(got["corr"] == 0) and np.all(got["ip"] == ref["ip"])
 
 
-def test_time256_windows_across_the_edges(xg, orc, pair):
-    """-time on 256 x 256 windows that hang over every side of the images (the
+@pytest.mark.parametrize("xs,corr", [(64, 0), (64, 1), (32, 1)], ids=["time256", "time4_256", "time4"])
+def test_time_windows_across_the_edges(xg, orc, pair, xs, corr):
+    """The matrix-core time searches (-time on 256 x 256 windows, -time4 on 256 x
+    256 and 128 x 128) on windows that hang over every side of the images (the
     samples outside read as zero) and one entirely outside."""
     m, s = pair
-    p = xg.make_params(64, 64, corr_flag=xg.TIME)
+    p = xg.make_params(xs, xs, corr_flag=corr)
     w = np.zeros(6, xg.WINDOW_DTYPE)
     corners = [(-100, 40), (1536 - 150, 300), (500, -90), (700, 2048 - 120), (-60, -70), (-400, 900)]
     for i, (x0, y0) in enumerate(corners):
@@ -148,4 +150,4 @@ def test_time256_windows_across_the_edges(xg, orc, pair):
     with xg.Engine(p, max_batch=4) as eng:
         eng.set_images(m, s)
         got = eng.correlate(w)
-    _same(got, orc.run(_oracle_params(orc, p), m, s, _locs(w), precision=32), "time 256 edges")
+    _same(got, orc.run(_oracle_params(orc, p), m, s, _locs(w), precision=32), "time edges, xs %d corr %d" % (xs, corr))
