@@ -2319,18 +2319,15 @@ __device__ __forceinline__ void pow_parts(float f, float mf, long long& lo, long
   }
 }
 
-// Exact master energies sum_{y, x < 64} A[y + dy][x + dx]^POW of every lag of a
-// 128 x 128 window (xsearch = ysearch = 32), as the doubles the NCC uses: the
-// int64 sum converted (POW 2: exact, < 2^53) or the 128-bit sum rounded to
-// nearest (POW 4, as u128_to_double; the row energies in two 40-bit parts).
-// One block per patch.
-template <int POW>
+// Exact master energies sum_{y, x < 64} A[y + dy][x + dx]^2 of every lag of a
+// 128 x 128 window (xsearch = ysearch = 32; int64 sliding sums), as doubles
+// (exact: < 2^53).  One block per patch.  (-time4's fourth powers need
+// 128-bit sums: k_time_energy_rows<4, 128>.)
 __global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const float* __restrict__ A1,
                                                             const int* __restrict__ rb,
                                                             const unsigned long long* __restrict__ sums,
                                                             double* __restrict__ aa, int nb) {
-  constexpr int NP = POW == 2 ? 1 : 2;           // parts per row energy
-  __shared__ long long E[NP * 127 * 64];
+  __shared__ long long E[127 * 64];
   const int b = xcd_item();                     // neighbours in a strip share an XCD (and its L2)
   if (b >= nb) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2348,38 +2345,27 @@ __global__ void __launch_bounds__(kThreads) k_time_energy32(DevParams P, const f
     f0[k] = a1[r * 128 + lane];
     f1[k] = a1[r * 128 + 64 + lane];
   }
-  auto row_e = [&](long long q0, long long q1) {
-    const long long p0 = wave_incl_scan_i64(q0), p1 = wave_incl_scan_i64(q1);
-    const long long tot0 = wave_lane63_i64(p0);
-    return (tot0 - p0 + q0) + (p1 - q1);
-  };
 #pragma unroll
   for (int k = 0; k < 32; k++) {
     const int r = wave + 4 * k;
     if (r >= 127) break;
-    long long l0, h0, l1, h1;
-    pow_parts<POW>(f0[k], m1f, l0, h0);
-    pow_parts<POW>(f1[k], m1f, l1, h1);
-    E[r * 64 + lane] = row_e(l0, l1);
-    if (NP == 2) E[127 * 64 + r * 64 + lane] = row_e(h0, h1);
+    long long q0, q1, unused;
+    pow_parts<2>(f0[k], m1f, q0, unused);
+    pow_parts<2>(f1[k], m1f, q1, unused);
+    const long long p0 = wave_incl_scan_i64(q0), p1 = wave_incl_scan_i64(q1);
+    const long long tot0 = wave_lane63_i64(p0);
+    E[r * 64 + lane] = (tot0 - p0 + q0) + (p1 - q1);
   }
   __syncthreads();
   // aa[dy][dx] = sum_{y < 64} E[dy + y][dx]: thread (dx, d0) slides down lags d0 .. d0 + 15
-  using S_t = typename std::conditional<POW == 2, long long, u128>::type;
-  auto out = [](S_t v) { return POW == 2 ? (double)(long long)v : u128_to_double((u128)v); };
   const int dx = lane, d0 = 16 * wave;
-  auto row = [&](int r) -> S_t {                   // row energy r at column dx
-    if constexpr (POW == 2) return E[r * 64 + dx];
-    else return ((u128)(unsigned long long)E[127 * 64 + r * 64 + dx] << 40) + (unsigned long long)E[r * 64 + dx];
-  };
-  S_t s = 0;
-  for (int y = 0; y < 64; y++) s += row(d0 + y);
+  long long s = 0;
+  for (int y = 0; y < 64; y++) s += E[(d0 + y) * 64 + dx];
   double* ob = aa + (size_t)b * 64 * 64;
-  ob[d0 * 64 + dx] = out(s);
+  ob[d0 * 64 + dx] = (double)s;
   for (int dy = d0 + 1; dy < d0 + 16; dy++) {
-    s += row(dy + 63);
-    s -= row(dy - 1);
-    ob[dy * 64 + dx] = out(s);
+    s += E[(dy + 63) * 64 + dx] - E[(dy - 1) * 64 + dx];
+    ob[dy * 64 + dx] = (double)s;
   }
 }
 
@@ -2759,75 +2745,83 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8(DevParams P, con
   }
 }
 
-// Exact master energies sum_{y, x < 128} A[y + dy][x + dx]^POW of the 128 x 128
-// lags of a 256 x 256 window (xsearch = ysearch = 64), as doubles (POW 2: the
-// value 1.0 * aa takes in the NCC; POW 4: the 128-bit sum rounded to nearest).
-// One block per patch.  Row energies E[r][dx] = sum_{x < 128} A[r][x + dx]^POW
-// (POW 4: in two 40-bit parts) of 32 rows at a time go to LDS (lane l of a
-// wave holds A[r][64 s + l], s < 4:
-// E[r][l] = suffix of segment 0 from l + segment 1 + prefix of segment 2 below
-// l, E[r][64 + l] likewise from segment 1), then threads 0..127 run down one
-// lag column each: with S(r) the sum of the first r row energies, S(r) for
-// r < 128 is parked in the output (POW 4: its high 64 bits in hi, the patch's
-// surface slot, which the search overwrites later) and aa[dy] = S(dy + 128) -
-// S(dy) overwrites it (int64 or 128-bit throughout, exact).
-template <int POW>
-__global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const float* __restrict__ amp,
-                                                             const int* __restrict__ rb,
-                                                             const unsigned long long* __restrict__ sums,
-                                                             double* __restrict__ aa, double* __restrict__ hi,
-                                                             int nb) {
+// Exact master energies sum_{y, x < NL} A[y + dy][x + dx]^POW of the NL x NL
+// lags of a W x W window, NL = W / 2 (W = 256: xsearch = ysearch = 64; W = 128:
+// 32), as doubles (POW 2: the value 1.0 * aa takes in the NCC; POW 4: the
+// 128-bit sum rounded to nearest).  One block per patch.  Row energies
+// E[r][dx] = sum_{x < NL} A[r][x + dx]^POW (POW 4: in two 40-bit parts) of 32
+// rows at a time go to LDS: lane l of a wave holds A[r][64 s + l], s < W / 64,
+// and E[r][l] = suffix of segment 0 from l + the whole segments between +
+// prefix of the next segment below l (W = 256: E[r][l] from segments 0..2,
+// E[r][64 + l] from segments 1..3).  Then threads 0..NL - 1 run down one lag
+// column each: with S(r) the sum of the first r row energies, S(r) for r < NL
+// is parked in the output (POW 4: its high 64 bits in hi, the patch's surface
+// slot, which the search overwrites later) and aa[dy] = S(dy + NL) - S(dy)
+// overwrites it (int64 or 128-bit throughout, exact).
+template <int POW, int W>
+__global__ void __launch_bounds__(kThreads) k_time_energy_rows(DevParams P, const float* __restrict__ amp,
+                                                               const int* __restrict__ rb,
+                                                               const unsigned long long* __restrict__ sums,
+                                                               double* __restrict__ aa, double* __restrict__ hi,
+                                                               int nb) {
+  static_assert(W == 128 || W == 256, "128 x 128 or 256 x 256 windows");
+  constexpr int NL = W / 2, SG = W / 64;          // lags per side, 64-sample segments per row
   constexpr int NP = POW == 2 ? 1 : 2;           // parts per row energy
-  __shared__ long long E[NP * 32 * 128];
+  __shared__ long long E[NP * 32 * NL];
   const int b = xcd_item();
   if (b >= nb) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float m1f = (float)patch_mean(sums[2 * b], P.N);
-  // amplitudes of patch b: rows of 256 from row base rb[b] of the strip lines, or per patch
-  const float* a1 = rb ? amp + (size_t)rb[b] * 256 : amp + (size_t)(2 * b) * P.N;
-  double* ob = aa + (size_t)b * 128 * 128;
+  // amplitudes of patch b: rows of W from row base rb[b] of the strip lines, or per patch
+  const float* a1 = rb ? amp + (size_t)rb[b] * W : amp + (size_t)(2 * b) * P.N;
+  double* ob = aa + (size_t)b * NL * NL;
   long long* sb = (long long*)ob;
-  unsigned long long* hb = POW == 2 ? nullptr : (unsigned long long*)(hi + (size_t)b * 128 * 128);
-  const int col = threadIdx.x & 127;
+  unsigned long long* hb = POW == 2 ? nullptr : (unsigned long long*)(hi + (size_t)b * NL * NL);
+  const int col = threadIdx.x & (NL - 1);
   using S_t = typename std::conditional<POW == 2, long long, u128>::type;
   S_t S = 0;
-  for (int ch = 0; ch < 8; ch++) {
-    float f[8][4];                                // rows 32 ch + wave + 4 k: all loads in flight together
+  for (int ch = 0; ch < W / 32; ch++) {
+    float f[8][SG];                               // rows 32 ch + wave + 4 k: all loads in flight together
 #pragma unroll
     for (int k = 0; k < 8; k++)
 #pragma unroll
-      for (int sg = 0; sg < 4; sg++) f[k][sg] = a1[(32 * ch + wave + 4 * k) * 256 + 64 * sg + lane];
-    // E[r][l] and E[r][64 + l] of one part (lane l holds segment sg's q[sg])
-    auto row_e = [&](const long long (&q)[4], long long* e) {
-      long long pf[4], tt[4];
+      for (int sg = 0; sg < SG; sg++) f[k][sg] = a1[(32 * ch + wave + 4 * k) * W + 64 * sg + lane];
+    // E[r][l] (and E[r][64 + l]) of one part (lane l holds segment sg's q[sg])
+    auto row_e = [&](const long long (&q)[SG], long long* e) {
+      long long pf[SG], tt[SG];
 #pragma unroll
-      for (int sg = 0; sg < 4; sg++) {
+      for (int sg = 0; sg < SG; sg++) {
         pf[sg] = wave_incl_scan_i64(q[sg]);
         tt[sg] = wave_lane63_i64(pf[sg]);
       }
-      e[lane] = (tt[0] - pf[0] + q[0]) + tt[1] + (pf[2] - q[2]);
-      e[64 + lane] = (tt[1] - pf[1] + q[1]) + tt[2] + (pf[3] - q[3]);
+      if constexpr (W == 256) {
+        e[lane] = (tt[0] - pf[0] + q[0]) + tt[1] + (pf[2] - q[2]);
+        e[64 + lane] = (tt[1] - pf[1] + q[1]) + tt[2] + (pf[SG - 1] - q[SG - 1]);
+      } else {
+        e[lane] = (tt[0] - pf[0] + q[0]) + (pf[1] - q[1]);
+      }
     };
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      long long ql[4], qh[4];
+      long long ql[SG], qh[SG];
 #pragma unroll
-      for (int sg = 0; sg < 4; sg++) pow_parts<POW>(f[k][sg], m1f, ql[sg], qh[sg]);
+      for (int sg = 0; sg < SG; sg++) pow_parts<POW>(f[k][sg], m1f, ql[sg], qh[sg]);
       const int rr = wave + 4 * k;
-      row_e(ql, E + rr * 128);
-      if (NP == 2) row_e(qh, E + 32 * 128 + rr * 128);
+      row_e(ql, E + rr * NL);
+      if (NP == 2) row_e(qh, E + 32 * NL + rr * NL);
     }
     __syncthreads();
-    if (threadIdx.x < 128) {
-      // rows r >= 128 (chunks 4 ..) subtract S(r - 128), parked 128 rows
+    if (threadIdx.x < NL) {
+      // rows r >= NL (chunks NL / 32 ..) subtract S(r - NL), parked NL rows
       // earlier: the parked sums are loaded G rows at a time
       constexpr int G = POW == 2 ? 16 : 8;
+      const bool late = ch >= NL / 32;
       for (int r0 = 0; r0 < 32; r0 += G) {
         unsigned long long pl[G], ph[G];
-        if (ch >= 4) {
+        if (late) {
 #pragma unroll
           for (int q = 0; q < G; q++) {
-            const int o = ((32 * ch + r0 + q) & 127) * 128 + col;
+            const int o = ((32 * ch + r0 + q) & (NL - 1)) * NL + col;
             pl[q] = (unsigned long long)sb[o];
             ph[q] = POW == 2 ? 0ull : hb[o];
           }
@@ -2835,20 +2829,20 @@ __global__ void __launch_bounds__(kThreads) k_time_energy256(DevParams P, const 
 #pragma unroll
         for (int q = 0; q < G; q++) {
           const int rr = r0 + q;                  // S holds S(32 ch + rr)
-          const int o = ((32 * ch + rr) & 127) * 128 + col;
+          const int o = ((32 * ch + rr) & (NL - 1)) * NL + col;
           if constexpr (POW == 2) {
-            if (ch < 4) sb[o] = S;
+            if (!late) sb[o] = S;
             else ob[o] = (double)(S - (long long)pl[q]);
-            S += E[rr * 128 + col];
+            S += E[rr * NL + col];
           } else {
-            if (ch < 4) {
+            if (!late) {
               sb[o] = (long long)(unsigned long long)S;
               hb[o] = (unsigned long long)(S >> 64);
             } else {
               ob[o] = u128_to_double(S - (((u128)ph[q] << 64) | pl[q]));
             }
-            S += ((u128)(unsigned long long)E[32 * 128 + rr * 128 + col] << 40) +
-                 (unsigned long long)E[rr * 128 + col];
+            S += ((u128)(unsigned long long)E[32 * NL + rr * NL + col] << 40) +
+                 (unsigned long long)E[rr * NL + col];
           }
         }
       }
@@ -2966,8 +2960,8 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8q(DevParams P, co
 // eleven int32 accumulators, each of which sums at most 6 limb pairs x 4
 // blocks x 4096 products below 2^14 (< 2^31); sum_s acc_s 2^(7 s) (128-bit)
 // is the exact numerator.  The master energies sum a^4 come from
-// k_time_energy32<4> / k_time_energy256<4>, the template's sum b^4 from the
-// packing (128-bit).  Per block pair only the limbs in use are multiplied:
+// k_time_energy_rows<4, W>, the template's sum b^4 from the packing
+// (128-bit).  Per block pair only the limbs in use are multiplied:
 // limbs 4 and 5 (and their second packing pass) where a square reaches 2^28
 // (|v| > 16383), else limb 3 where one reaches 2^21 (|v| > 1448), else limbs
 // 0 .. 2.  A patch with a value beyond 2^20 (float input) takes an exact
@@ -4008,20 +4002,20 @@ static void launch_time_i8(const PatchBufs& b, const DevParams& d, hipStream_t s
   const dim3 g1((nb + 7) & ~7), g4((4 * nb + 7) & ~7), blk(kThreads);
   const unsigned long long* sums = b.sums;
   if (d.corr_flag == XCORR_HIP_TIME && d.xs == 32) {
-    hipLaunchKernelGGL(k_time_energy32<2>, g1, blk, 0, st, d, A1, rb, sums, b.tcaa, nb);
+    hipLaunchKernelGGL(k_time_energy32, g1, blk, 0, st, d, A1, rb, sums, b.tcaa, nb);
     hipLaunchKernelGGL(k_time_search_i8, g1, blk, 0, st, d, A1, A2, rb, sums, (const double*)b.tcaa, b.surfd,
                        b.tpeak, b.tcave, nb);
     return;
   }
   if (d.corr_flag == XCORR_HIP_TIME) {
-    hipLaunchKernelGGL(k_time_energy256<2>, g1, blk, 0, st, d, A1, rb, sums, b.tcaa, (double*)nullptr, nb);
+    hipLaunchKernelGGL((k_time_energy_rows<2, 256>), g1, blk, 0, st, d, A1, rb, sums, b.tcaa, (double*)nullptr, nb);
     hipLaunchKernelGGL(k_time_search_i8q, g4, blk, 0, st, d, A1, A2, rb, sums, (const double*)b.tcaa, b.surfd, nb);
   } else if (d.xs == 32) {
-    hipLaunchKernelGGL(k_time_energy32<4>, g1, blk, 0, st, d, A1, rb, sums, b.tcaa, nb);
+    hipLaunchKernelGGL((k_time_energy_rows<4, 128>), g1, blk, 0, st, d, A1, rb, sums, b.tcaa, b.surfd, nb);
     hipLaunchKernelGGL(k_time4_search_i8<32>, g1, blk, 0, st, d, A1, A2, rb, sums, (const double*)b.tcaa, b.surfd,
                        nb);
   } else {
-    hipLaunchKernelGGL(k_time_energy256<4>, g1, blk, 0, st, d, A1, rb, sums, b.tcaa, b.surfd, nb);
+    hipLaunchKernelGGL((k_time_energy_rows<4, 256>), g1, blk, 0, st, d, A1, rb, sums, b.tcaa, b.surfd, nb);
     hipLaunchKernelGGL(k_time4_search_i8<64>, g4, blk, 0, st, d, A1, A2, rb, sums, (const double*)b.tcaa, b.surfd,
                        nb);
   }
