@@ -2404,7 +2404,7 @@ __device__ __forceinline__ int bl_chunk(int row, int c) { return 16 * ((c + (row
 // for every master row r that meets them, the Toeplitz rows A[r][x + dx]
 // times the template rows, one 16 x 16 x 64 int8 product per limb pair
 // (ka < KA, kb < KB), accumulated in acc[i][j][ka + kb].
-template <int KA, int KB, int NS>
+template <int KA, int KB, bool DB = (KA * KB <= 16), int NS>
 __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed char* Bl, int dx0, int dy0, int m,
                                              int g, v4i_t (&acc)[2][2][NS]) {
   static_assert(KA + KB - 1 <= NS, "one accumulator per limb weight");
@@ -2446,7 +2446,7 @@ __device__ __forceinline__ void i8_lag_tiles(const signed char* Al, const signed
   // row r + 1 load while the products of row r run (with more than 16 limb
   // products per tile and row, one operand set: the products cover the loads)
   const int r_end = dy0 + 32 + 63;
-  if constexpr (KA * KB > 16) {
+  if constexpr (!DB) {
     for (int r = dy0; r < r_end; r++) {
       v4i_t A0[2][KA], B0[2][KB];
       load(r, A0, B0);
@@ -2910,9 +2910,9 @@ __global__ void __launch_bounds__(kThreads, 2) k_time_search_i8q(DevParams P, co
     anybig = anybig || sbig[ph];
     if (anybig) continue;
     // limb 2 is zero throughout a block whose values all lie in [-8256, 8127]
-    if (shi[ph][0] && shi[ph][1]) i8_lag_tiles<3, 3>(Al, Bl, dx0, dy0, m, g, acc);
-    else if (shi[ph][0]) i8_lag_tiles<3, 2>(Al, Bl, dx0, dy0, m, g, acc);
-    else if (shi[ph][1]) i8_lag_tiles<2, 3>(Al, Bl, dx0, dy0, m, g, acc);
+    if (shi[ph][0] && shi[ph][1]) i8_lag_tiles<3, 3, false>(Al, Bl, dx0, dy0, m, g, acc);
+    else if (shi[ph][0]) i8_lag_tiles<3, 2, false>(Al, Bl, dx0, dy0, m, g, acc);
+    else if (shi[ph][1]) i8_lag_tiles<2, 3, false>(Al, Bl, dx0, dy0, m, g, acc);
     else i8_lag_tiles<2, 2>(Al, Bl, dx0, dy0, m, g, acc);
   }
   const double d2d = (double)block_sum_i64(b2, redl);
