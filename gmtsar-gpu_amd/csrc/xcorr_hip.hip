@@ -2380,13 +2380,21 @@ __device__ __forceinline__ void limbs7(int v, signed char* l) {
 
 // 16 bytes of a limb row starting at byte offset o (any alignment): the five
 // dwords that cover them and one byte-align per output dword
-// (v_alignbyte_b32: bytes sh .. sh + 3 of the pair {hi, lo}).
+// (v_alignbyte_b32: bytes sh .. sh + 3 of the pair {hi, lo}).  The callers'
+// offsets differ by the lane within each group of four lanes (o & 3 = lane &
+// 3, the same dword-aligned start), so the four lanes share the five dwords:
+// lane k of the quad reads dword k, all read dword 4, and DPP quad broadcasts
+// hand each lane the other three (all lanes active).
 __device__ __forceinline__ v4i_t ld16_unaligned(const signed char* row, int o) {
   const unsigned* wp = (const unsigned*)(row + (o & ~3));
   const unsigned sh = (unsigned)(o & 3);
+  const unsigned mine = wp[threadIdx.x & 3];
   unsigned w[5];
-#pragma unroll
-  for (int k = 0; k < 5; k++) w[k] = wp[k];
+  w[0] = (unsigned)__builtin_amdgcn_mov_dpp((int)mine, 0x00, 0xf, 0xf, false);   // quad_perm 0,0,0,0
+  w[1] = (unsigned)__builtin_amdgcn_mov_dpp((int)mine, 0x55, 0xf, 0xf, false);   // quad_perm 1,1,1,1
+  w[2] = (unsigned)__builtin_amdgcn_mov_dpp((int)mine, 0xAA, 0xf, 0xf, false);   // quad_perm 2,2,2,2
+  w[3] = (unsigned)__builtin_amdgcn_mov_dpp((int)mine, 0xFF, 0xf, 0xf, false);   // quad_perm 3,3,3,3
+  w[4] = wp[4];
   v4i_t r;
 #pragma unroll
   for (int k = 0; k < 4; k++) r[k] = (int)__builtin_amdgcn_alignbyte(w[k + 1], w[k], sh);
